@@ -1,0 +1,226 @@
+#!/usr/bin/env python3
+"""Benchmark of the sparse 3D pixel-pair accumulation on one MI355X.
+
+Workload (the sparse photon-list configuration): flat 1024x1024 detector,
+75 um pixels at 0.5 m; 10,000 frames of Poisson(0.002) photons per pixel
+(~2,100 photons/frame) stored as (uint32 pixel, uint16 count) lists; every
+ordered photon pair of a frame is binned (nearest neighbour, double mapping,
+identity rotation) into a 257^3 uint64 q-difference mesh with the reference
+CPU path's fixed point.  One step = zero the mesh + one pass over all frames
+(iac_ac3d_sparse_batch).  Inputs are resident in HBM before timing starts.
+
+metric: ordered pair-updates (pairs within Max_Q, self pairs once) per second.
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import sys
+import time
+from pathlib import Path
+
+import numpy as np
+
+ROOT = Path(__file__).resolve().parent
+sys.path.insert(0, str(ROOT))
+
+HBM_PEAK_GBS = 8000.0  # MI355X HBM3E, public spec
+BYTES_PER_UPDATE = 16  # one 8 B bin read + one 8 B bin write per pair-update
+
+
+def log(*a):
+    print(*a, file=sys.stderr, flush=True)
+
+
+def parse():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=5)
+    ap.add_argument("--warmup", type=int, default=2)
+    ap.add_argument("--frames", type=int, default=10000)
+    ap.add_argument("--seed", type=int, default=2)
+    ap.add_argument("--cpu-seconds", type=float, default=15.0,
+                    help="target CPU work of the bounded CPU-baseline sample")
+    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--check", action="store_true", help="bit-exact check of a frame subset")
+    return ap.parse_args()
+
+
+def main():
+    args = parse()
+    if args.gpus != 1:
+        print(f"bench.py measures one GPU only; --gpus {args.gpus} is not supported "
+              "(frames shard across GPUs by design, see DESIGN.md)", file=sys.stderr)
+        sys.exit(2)
+
+    import torch
+
+    import incohautocorrelate_amd as iac
+    from incohautocorrelate_amd import workloads as W
+
+    dev = torch.device("cuda:0")
+    torch.cuda.set_device(dev)
+    t0 = time.time()
+    _, kmap, max_q = W.detector(1024)
+    shape = iac.mesh_shape_big(max_q, 255)
+    data = W.poisson_photon_frames(1024 * 1024, args.frames, 0.002, args.seed)
+    n_total = int(data["frame_off"][-1])
+    log(f"[bench] workload generated in {time.time() - t0:.1f}s: {args.frames} frames, "
+        f"{n_total} photons ({n_total / args.frames:.1f}/frame), mesh {shape.size_ab}^3")
+
+    d_kmap = torch.from_numpy(kmap).to(dev)
+    d_pix = torch.from_numpy(data["pix"].view(np.int32)).to(dev)
+    d_cnt = torch.from_numpy(data["cnt"].view(np.int16)).to(dev)
+    d_off = torch.from_numpy(data["frame_off"].view(np.int64)).to(dev)
+    nvox = shape.size_ab * shape.size_ab * shape.size_c
+    d_mesh = torch.zeros(nvox, dtype=torch.int64, device=dev)
+    ctx = iac.Context(0)
+
+    # pair-updates of the workload: the same pass with unit counts (val(1) = 1000)
+    d_one = torch.ones_like(d_cnt)
+    ctx.ac3d_sparse_batch(d_kmap, d_pix, d_one, d_off, n_total, shape, d_mesh)
+    torch.cuda.synchronize()
+    updates = int(d_mesh.sum().item()) // 1000
+    del d_one
+    log(f"[bench] ordered pair-updates per step: {updates:.4e} ({updates / args.frames:.4e}/frame)")
+
+    def step():
+        d_mesh.zero_()
+        ctx.ac3d_sparse_batch(d_kmap, d_pix, d_cnt, d_off, n_total, shape, d_mesh)
+
+    for _ in range(args.warmup):
+        step()
+    torch.cuda.synchronize()
+    log("[bench] warmup done")
+
+    ctx.enable_timing(True)
+    ctx.kernel_time(reset=True)
+    torch.cuda.synchronize()
+    t1 = time.perf_counter()
+    for _ in range(args.steps):
+        step()
+    torch.cuda.synchronize()
+    t2 = time.perf_counter()
+    kms, klaunch = ctx.kernel_time(reset=True)
+    ctx.enable_timing(False)
+    ms_per_step = (t2 - t1) * 1e3 / args.steps
+    value = updates / ((t2 - t1) / args.steps)
+    k_avg_ms = kms / max(klaunch, 1)
+    achieved = updates * BYTES_PER_UPDATE / (k_avg_ms * 1e-3) / 1e9
+    log(f"[bench] {ms_per_step:.3f} ms/step, kernel {k_avg_ms:.3f} ms avg over {klaunch} launches, "
+        f"{value:.4e} pair-updates/s")
+
+    if ctx.dropped():
+        log(f"[bench] WARNING: {ctx.dropped()} updates fell outside the mesh")
+
+    if args.check:
+        check_subset(iac, W, kmap, data, shape, dev)
+
+    cpu = None
+    if not args.no_cpu:
+        cpu = cpu_baseline(kmap, data, shape, ctx, iac, d_kmap, dev, args)
+
+    out = {
+        "metric": "pixel-pair updates/sec into q-mesh (sparse 1024^2, 257^3 uint64, bit-exact)",
+        "value": value,
+        "unit": "pair-updates/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": ms_per_step,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "u64",
+        "data": "synthetic",
+        "config": {
+            "workload": "sparse photon-list AC_uw: 1024x1024 detector, 10,000 frames Poisson(0.002) "
+                        "as (uint32 pixel, uint16 count), 257^3 uint64 mesh, NN/NN double-map, identity R",
+            "frames": args.frames,
+            "photons": n_total,
+            "pair_updates_per_step": updates,
+            "seed": args.seed,
+        },
+        "roofline": {
+            "bound": "hbm",
+            "achieved": achieved,
+            "peak": HBM_PEAK_GBS,
+            "unit": "GB/s",
+            "frac": achieved / HBM_PEAK_GBS,
+            "traffic": None,
+            "kernel": "ac3d_tiles_kernel<true>",
+            "kernel_avg_ms": k_avg_ms,
+            "bytes_per_update": BYTES_PER_UPDATE,
+        },
+        "cpu_baseline": cpu,
+    }
+    print(json.dumps(out), flush=True)
+    ctx.close()
+
+
+def check_subset(iac, W, kmap, data, shape, dev):
+    """Bit-exact check of the first frames against the oracle."""
+    import torch
+
+    sys.path.insert(0, str(ROOT / "oracle"))
+    import oracle as O
+
+    nf = 20
+    off = data["frame_off"][: nf + 1]
+    n = int(off[-1])
+    ref, _ = O.ac3d_frames(kmap, data["pix"][:n], data["cnt"][:n], off, shape)
+    ctx = iac.Context(0)
+    m = torch.zeros(ref.size, dtype=torch.int64, device=dev)
+    ctx.ac3d_sparse_batch(torch.from_numpy(kmap).to(dev),
+                          torch.from_numpy(data["pix"][:n].view(np.int32)).to(dev),
+                          torch.from_numpy(data["cnt"][:n].view(np.int16)).to(dev),
+                          torch.from_numpy(off.view(np.int64)).to(dev), n, shape, m)
+    got = m.cpu().numpy().view(np.uint64)
+    ok = bool((got == ref).all())
+    log(f"[bench] bit-exact check on {nf} frames: {'OK' if ok else 'MISMATCH'}")
+    if not ok:
+        raise SystemExit("bit-exact check failed")
+
+
+def cpu_baseline(kmap, data, shape, ctx, iac, d_kmap, dev, args):
+    """The oracle's restatement of the reference CPU path (OpenMP over the outer
+    photon, omp atomic uint64 adds) on a bounded prefix of the same frames."""
+    import torch
+
+    sys.path.insert(0, str(ROOT / "oracle"))
+    import oracle as O
+
+    cores = len(os.sched_getaffinity(0))
+    cores = min(cores, 16)
+    off = data["frame_off"]
+    # calibrate on a few frames, then size the sample to ~cpu_seconds
+    nf = 4
+    t = time.perf_counter()
+    O.ac3d_frames(kmap, data["pix"][: off[nf]], data["cnt"][: off[nf]], off[: nf + 1], shape,
+                  nthreads=cores)
+    per = (time.perf_counter() - t) / nf
+    nf = int(max(8, min(len(off) - 1, args.cpu_seconds / max(per, 1e-6))))
+    n = int(off[nf])
+    t = time.perf_counter()
+    O.ac3d_frames(kmap, data["pix"][:n], data["cnt"][:n], off[: nf + 1], shape, nthreads=cores)
+    dt = time.perf_counter() - t
+    # pair-updates of the sample (unit counts on the GPU, val(1) = 1000)
+    m = torch.zeros(shape.size_ab ** 3, dtype=torch.int64, device=dev)
+    ctx.ac3d_sparse_batch(d_kmap, torch.from_numpy(data["pix"][:n].view(np.int32)).to(dev),
+                          torch.ones(n, dtype=torch.int16, device=dev),
+                          torch.from_numpy(off[: nf + 1].view(np.int64)).to(dev), n, shape, m)
+    upd = int(m.sum().item()) // 1000
+    log(f"[bench] CPU oracle: {nf} frames, {upd:.3e} updates in {dt:.2f}s on {cores} threads")
+    return {
+        "value": upd / dt,
+        "unit": "pair-updates/s",
+        "cores": cores,
+        "kind": "port",
+        "sample": f"first {nf} of the {len(off) - 1} frames ({upd} pair-updates), "
+                  f"OpenMP {cores} threads, -O3 -fopenmp -ffp-contract=off",
+    }
+
+
+if __name__ == "__main__":
+    main()

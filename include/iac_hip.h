@@ -1,0 +1,132 @@
+/*
+ * iac_hip.h -- C ABI of the MI355X (gfx950) pixel-pair autocorrelation path.
+ *
+ * Plain C: no C++ or torch types cross this boundary.  Device pointers
+ * (prefix d_) are HIP device addresses owned by the caller unless stated;
+ * host pointers (prefix h_ or none) are read during the call only.  Every
+ * entry point returns IAC_OK (0) or a negative iac error code; iac_strerror()
+ * names the code and iac_last_error() gives the detail of the last failure on
+ * the calling thread.  An iac_ctx is bound to one device and one HIP stream and
+ * must be used by one host thread at a time; work is enqueued on the stream
+ * and results are visible after iac_ctx_sync() (or any blocking copy).
+ *
+ * The entry points replace the OpenCL dispatch blocks of the reference host
+ * methods (paths relative to the reference root, IAC = IncohAutoCorrelate/).
+ */
+#ifndef IAC_HIP_H
+#define IAC_HIP_H
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define IAC_ABI_VERSION 1
+
+/* error codes */
+#define IAC_OK 0
+#define IAC_ERR_INVALID_ARG (-1)  /* null pointer, zero size, bad enum */
+#define IAC_ERR_EVEN_MESH (-2)    /* IAC/ACMesh.cpp:76-81 throws std::invalid_argument */
+#define IAC_ERR_NO_DEVICE (-3)    /* IAC/Settings.cpp:348-360 (no usable device) */
+#define IAC_ERR_HIP (-4)          /* a HIP runtime call failed (checkErr, IAC/Settings.cpp:322-329) */
+#define IAC_ERR_OOM (-5)          /* device allocation failed */
+#define IAC_ERR_SHAPE (-6)        /* shape mismatch, e.g. IAC/Detector.cpp:1324-1329 */
+#define IAC_ERR_PRECONDITION (-7) /* e.g. no hit list / no event, IAC/Detector.cpp:1012-1023 */
+#define IAC_ERR_LIMIT (-8)        /* a size exceeds a documented maximum */
+
+/* Settings::Interpolation, IAC/Settings.h:158 */
+#define IAC_NEAREST 0
+#define IAC_LINEAR 1
+
+/* ACMesh::MeshShape, IAC/ACMesh.h:21-29 (same fields, same meaning) */
+typedef struct {
+    uint32_t k_a, k_b, k_c;   /* axis alignment, k_a largest extent */
+    uint32_t size_ab, size_c; /* edge sizes including the 2 padding voxels */
+    uint32_t center[3];       /* voxel holding q = 0 */
+    float dq_per_voxel;
+    uint32_t voxel_per_dq;
+    float max_q;
+} iac_mesh_shape;
+
+typedef struct iac_ctx iac_ctx;
+
+const char *iac_strerror(int code);
+const char *iac_last_error(void);
+int iac_abi_version(void);
+
+/* ---- device, context, memory --------------------------------------------------- */
+/* Replaces Settings::SetUp_OpenCL device discovery, IAC/Settings.cpp:331-430. */
+int iac_device_count(int *count);
+/* Replaces Settings::OCL_ReserveDevice + the per-call cl::CommandQueue,
+ * IAC/Settings.cpp:432-455, IAC/Detector.cpp:1074-1090.  Creates a context on
+ * `device` with a stream of default priority. */
+int iac_ctx_create(int device, iac_ctx **out);
+/* Replaces Settings::OCL_FreeDevice, IAC/Settings.cpp:457-470. */
+int iac_ctx_destroy(iac_ctx *ctx);
+/* Enqueue on an external HIP stream (hipStream_t) instead of the context's
+ * own; NULL selects the device's default (null) stream.
+ * iac_ctx_use_own_stream() returns to the context's own stream. */
+int iac_ctx_set_stream(iac_ctx *ctx, void *hip_stream);
+int iac_ctx_use_own_stream(iac_ctx *ctx);
+void *iac_ctx_get_stream(iac_ctx *ctx);
+int iac_ctx_sync(iac_ctx *ctx);
+/* Replace the CL_MEM_COPY_HOST_PTR buffers and blocking enqueueReadBuffer of
+ * every reference launch site (e.g. IAC/Detector.cpp:1100-1140). */
+int iac_dev_alloc(iac_ctx *ctx, size_t bytes, void **d_ptr);
+int iac_dev_free(iac_ctx *ctx, void *d_ptr);
+int iac_memcpy_h2d(iac_ctx *ctx, void *d_dst, const void *h_src, size_t bytes);
+int iac_memcpy_d2h(iac_ctx *ctx, void *h_dst, const void *d_src, size_t bytes);
+int iac_memset_zero(iac_ctx *ctx, void *d_ptr, size_t bytes);
+/* Kernel timing for measurement: while enabled, the dominant kernel of every
+ * call is bracketed by HIP events on the context's stream.  iac_ctx_timing()
+ * synchronizes, returns the summed milliseconds and the number of timed
+ * launches since the last reset, and resets when `reset` is nonzero. */
+int iac_ctx_enable_timing(iac_ctx *ctx, int enable);
+int iac_ctx_timing(iac_ctx *ctx, double *ms, uint64_t *launches, int reset);
+/* Updates whose voxel fell outside the mesh since the context was created.
+ * The reference writes such updates out of bounds; this path drops them. */
+int iac_dropped_updates(iac_ctx *ctx, uint64_t *count);
+
+/* ---- host geometry (the reference computes these on the host as well) --------- */
+/* Simulator::GeneratePixelMap, IAC/Simulator.cpp:495-519; pixmap is [size_a][size_b][3]. */
+int iac_generate_pixel_map(int size_a, int size_b, float pixel_size, const float center[3],
+                           const float vec_a[3], const float vec_b[3], float *pixmap);
+/* Detector::Calc_kMap, IAC/Detector.cpp:310-361 */
+int iac_calc_kmap(const float *pixmap, uint32_t npix, float *kmap, float max_k[3],
+                  float min_k[3], float max_q[3]);
+/* ACMesh::CreateBigMeshForDetector, IAC/ACMesh.cpp:74-106 (and the identical
+ * CreateBigMesh_CofQ_ForDetector, :107-139) */
+int iac_mesh_shape_big(const float max_q[3], int edge_size, float q_zoom, iac_mesh_shape *out);
+/* ACMesh::CreateSmallMesh_CofQ_ForDetector, IAC/ACMesh.cpp:140-206 */
+int iac_mesh_shape_small_cofq(const float max_q[3], int perp_size, float q_zoom,
+                              iac_mesh_shape *out);
+
+/* ---- 3D pair accumulation (AC_uw) -------------------------------------------- */
+/* Detector::AutoCorrelateSparseList, IAC/Detector.cpp:1010-1181, with the CPU
+ * branch's arithmetic (ACMesh::Atomic_Add_q_Entry, IAC/ACMesh.cpp:209-385):
+ * every ordered pair (a,b) of the hit list, the self pair once, is binned and
+ * `(unsigned)FloatToInt(v_a*v_b)` is added to the uint64 mesh (accumulating).
+ * d_hitlist4 is the reference's SparseHitList layout {kx,ky,kz,value} x n
+ * (IAC/Detector.h:120).  rot is the event's row-major RotMatrix
+ * (IAC/Settings.h:107), host memory.  d_mesh is size_ab^2*size_c uint64. */
+int iac_ac3d_sparse(iac_ctx *ctx, const float *d_hitlist4, uint32_t n, const float rot[9],
+                    const iac_mesh_shape *shape, int interp1, int interp2, int double_map,
+                    uint64_t *d_mesh);
+/* The RunIAC frame loop (IAC/RunIAC.cpp:524-550) in one call: frame f holds
+ * the photon entries [d_frame_off[f], d_frame_off[f+1]) of (pixel index,
+ * count) lists d_pix/d_cnt; k vectors are gathered from d_kmap ([npix][3]).
+ * d_rots is [nframes][9] floats on the device or NULL for the identity.
+ * n_total = d_frame_off[nframes] (host copy, sizes the scratch).  One pass of
+ * a persistent, load-balanced kernel covers all frames; the mesh is identical
+ * to calling iac_ac3d_sparse once per frame. */
+int iac_ac3d_sparse_batch(iac_ctx *ctx, const float *d_kmap, uint32_t npix,
+                          const uint32_t *d_pix, const uint16_t *d_cnt,
+                          const uint64_t *d_frame_off, uint32_t nframes, uint64_t n_total,
+                          const float *d_rots, const iac_mesh_shape *shape, int interp1,
+                          int interp2, int double_map, uint64_t *d_mesh);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

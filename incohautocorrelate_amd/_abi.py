@@ -1,0 +1,108 @@
+"""ctypes binding of libiac_hip.so (the C ABI declared in include/iac_hip.h).
+
+The shared library is the product; this module only declares its symbols so
+Python tests and the benchmark can call it.  Loading fails loudly when the
+library has not been built: there is no CPU fallback behind it.
+"""
+from __future__ import annotations
+
+import ctypes as C
+from pathlib import Path
+
+_PKG = Path(__file__).resolve().parent
+LIB_PATH = _PKG / "libiac_hip.so"
+
+
+class MeshShape(C.Structure):
+    """ACMesh::MeshShape (IncohAutoCorrelate/ACMesh.h:21-29)."""
+
+    _fields_ = [
+        ("k_a", C.c_uint32), ("k_b", C.c_uint32), ("k_c", C.c_uint32),
+        ("size_ab", C.c_uint32), ("size_c", C.c_uint32),
+        ("center", C.c_uint32 * 3),
+        ("dq_per_voxel", C.c_float),
+        ("voxel_per_dq", C.c_uint32),
+        ("max_q", C.c_float),
+    ]
+
+    def as_dict(self) -> dict:
+        return {
+            "k_a": self.k_a, "k_b": self.k_b, "k_c": self.k_c,
+            "size_ab": self.size_ab, "size_c": self.size_c,
+            "center": tuple(self.center), "dq_per_voxel": self.dq_per_voxel,
+            "voxel_per_dq": self.voxel_per_dq, "max_q": self.max_q,
+        }
+
+
+P = C.c_void_p
+F3 = C.POINTER(C.c_float)
+U32 = C.c_uint32
+U64 = C.c_uint64
+I32 = C.c_int
+
+# name -> (restype, argtypes); mirrors include/iac_hip.h line by line
+SIGNATURES: dict[str, tuple] = {
+    "iac_strerror": (C.c_char_p, [I32]),
+    "iac_last_error": (C.c_char_p, []),
+    "iac_abi_version": (I32, []),
+    "iac_device_count": (I32, [C.POINTER(I32)]),
+    "iac_ctx_create": (I32, [I32, C.POINTER(P)]),
+    "iac_ctx_destroy": (I32, [P]),
+    "iac_ctx_set_stream": (I32, [P, P]),
+    "iac_ctx_use_own_stream": (I32, [P]),
+    "iac_ctx_get_stream": (P, [P]),
+    "iac_ctx_sync": (I32, [P]),
+    "iac_dev_alloc": (I32, [P, C.c_size_t, C.POINTER(P)]),
+    "iac_dev_free": (I32, [P, P]),
+    "iac_memcpy_h2d": (I32, [P, P, P, C.c_size_t]),
+    "iac_memcpy_d2h": (I32, [P, P, P, C.c_size_t]),
+    "iac_memset_zero": (I32, [P, P, C.c_size_t]),
+    "iac_ctx_enable_timing": (I32, [P, I32]),
+    "iac_ctx_timing": (I32, [P, C.POINTER(C.c_double), C.POINTER(U64), I32]),
+    "iac_dropped_updates": (I32, [P, C.POINTER(U64)]),
+    "iac_generate_pixel_map": (I32, [I32, I32, C.c_float, F3, F3, F3, F3]),
+    "iac_calc_kmap": (I32, [F3, U32, F3, F3, F3, F3]),
+    "iac_mesh_shape_big": (I32, [F3, I32, C.c_float, C.POINTER(MeshShape)]),
+    "iac_mesh_shape_small_cofq": (I32, [F3, I32, C.c_float, C.POINTER(MeshShape)]),
+    "iac_ac3d_sparse": (I32, [P, P, U32, F3, C.POINTER(MeshShape), I32, I32, I32, P]),
+    "iac_ac3d_sparse_batch": (I32, [P, P, U32, P, P, P, U32, U64, P, C.POINTER(MeshShape),
+                                    I32, I32, I32, P]),
+}
+
+
+class IACError(RuntimeError):
+    def __init__(self, code: int, where: str, detail: str):
+        super().__init__(f"{where}: error {code}: {detail}")
+        self.code = code
+
+
+_lib = None
+
+
+def lib() -> C.CDLL:
+    """Load libiac_hip.so once; raise if it is missing (build it first)."""
+    global _lib
+    if _lib is None:
+        if not LIB_PATH.exists():
+            raise ImportError(
+                f"{LIB_PATH} is missing: run __graft_entry__.build() (or python "
+                "incohautocorrelate_amd/build.py) before using the HIP path")
+        lib_ = C.CDLL(str(LIB_PATH))
+        for name, (res, args) in SIGNATURES.items():
+            fn = getattr(lib_, name)
+            fn.restype = res
+            fn.argtypes = args
+        _lib = lib_
+    return _lib
+
+
+def check(code: int, where: str) -> None:
+    if code != 0:
+        L = lib()
+        detail = (L.iac_last_error() or b"").decode() or (L.iac_strerror(code) or b"").decode()
+        raise IACError(code, where, detail)
+
+
+def fptr(arr) -> F3:
+    """Pointer to the data of a contiguous float32 numpy array."""
+    return arr.ctypes.data_as(F3)

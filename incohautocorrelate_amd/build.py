@@ -1,0 +1,108 @@
+"""Build recipe for libiac_hip.so (gfx950 HIP kernels + C ABI).
+
+Every HIP translation unit under csrc/ is compiled with hipcc for gfx950 and
+linked into incohautocorrelate_amd/libiac_hip.so.  Floating-point flags keep
+the per-pair index arithmetic bit-identical to the reference's x86-64 CPU
+path: no FMA contraction, correctly rounded fp32 division and square root.
+"""
+from __future__ import annotations
+
+import concurrent.futures as cf
+import os
+import shutil
+import subprocess
+import sys
+from pathlib import Path
+
+PKG = Path(__file__).resolve().parent
+CSRC = PKG / "csrc"
+HOST = PKG / "host"
+BUILD = PKG.parent / "_build"
+LIB = PKG / "libiac_hip.so"
+HOST_LIB = PKG / "libiac_host.so"
+
+HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
+
+HIP_FLAGS = [
+    "--offload-arch=gfx950",
+    "-O3",
+    "-fPIC",
+    "-std=c++17",
+    "-ffp-contract=off",
+    "-fno-fast-math",
+    "-fhip-fp32-correctly-rounded-divide-sqrt",
+    "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
+    "-Wall",
+    "-Wno-unused-function",
+]
+
+
+def _run(cmd: list[str]) -> None:
+    r = subprocess.run(cmd, capture_output=True, text=True)
+    if r.returncode != 0:
+        sys.stderr.write(r.stdout + r.stderr)
+        raise RuntimeError("build step failed: " + " ".join(cmd))
+    # hipcc's host pass warns that it does not know the packed-fp32 feature;
+    # that is expected and harmless, everything else is shown.
+    noise = "'-packed-fp32-ops' is not a recognized feature"
+    for line in (r.stdout + r.stderr).splitlines():
+        if line.strip() and noise not in line:
+            sys.stderr.write(line + "\n")
+
+
+def _compile(src: Path) -> Path:
+    obj = BUILD / (src.stem + ".o")
+    _run([HIPCC, *HIP_FLAGS, "-I", str(PKG.parent / "include"), "-c", str(src), "-o", str(obj)])
+    return obj
+
+
+def build_hip(verbose: bool = True) -> Path:
+    BUILD.mkdir(exist_ok=True)
+    srcs = sorted(CSRC.glob("*.hip"))
+    with cf.ThreadPoolExecutor(max_workers=min(8, len(srcs))) as ex:
+        objs = list(ex.map(_compile, srcs))
+    tmp = LIB.with_suffix(".so.tmp")
+    _run([HIPCC, "--offload-arch=gfx950", "-shared", "-fPIC", "-o", str(tmp), *map(str, objs)])
+    os.replace(tmp, LIB)
+    if verbose:
+        print(f"built {LIB}")
+    return LIB
+
+
+def build_host(verbose: bool = True) -> Path | None:
+    """C++ mirror of the reference host classes, linked against libiac_hip.so."""
+    srcs = sorted(HOST.glob("*.cpp"))
+    if not srcs:
+        return None
+    tmp = HOST_LIB.with_suffix(".so.tmp")
+    _run([
+        "g++", "-O3", "-fPIC", "-shared", "-std=c++17", "-fopenmp", "-ffp-contract=off",
+        "-I", str(PKG.parent / "include"), "-I", str(HOST),
+        "-o", str(tmp), *map(str, srcs),
+        "-L", str(PKG), "-liac_hip", "-Wl,-rpath,$ORIGIN",
+    ])
+    os.replace(tmp, HOST_LIB)
+    if verbose:
+        print(f"built {HOST_LIB}")
+    return HOST_LIB
+
+
+def build_oracle(verbose: bool = True) -> Path:
+    """The CPU oracle (test infrastructure), built with the reference's flags."""
+    oracle = PKG.parent / "oracle"
+    _run(["make", "-s", "-C", str(oracle), "-B", "liboracle.so"])
+    if verbose:
+        print(f"built {oracle / 'liboracle.so'}")
+    return oracle / "liboracle.so"
+
+
+def build_all(verbose: bool = True) -> None:
+    if shutil.which(HIPCC) is None and not Path(HIPCC).exists():
+        raise RuntimeError(f"hipcc not found at {HIPCC}")
+    build_hip(verbose)
+    build_host(verbose)
+    build_oracle(verbose)
+
+
+if __name__ == "__main__":
+    build_all()

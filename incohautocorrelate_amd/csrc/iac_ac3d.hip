@@ -1,0 +1,303 @@
+// iac_ac3d.hip -- 3D AC_uw pair accumulation (sparse photon lists) on gfx950.
+//
+// Work decomposition: each frame's n photon records are cut into 64-record
+// tiles and the upper triangle of its n x n pair matrix into 64 x 64 tiles.
+// A persistent grid walks the concatenated tile list of all frames (uneven
+// frames cost only their own tiles).  Each 256-thread block stages the
+// b-tile in LDS and every thread owns one a-record and 16 b-records.
+//
+// With nearest-neighbour second-level binning the ordered pair (b,a) lands on
+// the point reflection of (a,b)'s voxel with the same value (iac_binning.h),
+// so only unordered pairs are binned and the mirrored voxel receives the same
+// add; the self pair lands on the centre once.  Linear modes bin both orders.
+#include <algorithm>
+#include <cmath>
+
+#include "iac_binning.h"
+#include "iac_internal.h"
+
+#pragma clang fp contract(off)
+
+namespace {
+
+constexpr int kTile = 64;
+constexpr int kBlock = 256;
+constexpr int kBPerThread = kTile / (kBlock / kTile);  // 16
+
+struct Rot9 {
+    float m[9];
+};
+
+struct GlobalSink {
+    unsigned long long *mesh;
+    unsigned long long *dropped;
+    int sab, sc;
+    __device__ __forceinline__ void add(int fs, int ms, int ss, uint32_t v)
+    {
+        if ((unsigned)fs < (unsigned)sab && (unsigned)ms < (unsigned)sab && (unsigned)ss < (unsigned)sc) {
+            if (v) atomicAdd(mesh + (size_t)fs + (size_t)ms * sab + (size_t)ss * sab * sab,
+                             (unsigned long long)v);
+        } else {
+            atomicAdd(dropped, 1ull);
+        }
+    }
+};
+
+__device__ __forceinline__ uint64_t tri_tiles(uint64_t n)
+{
+    const uint64_t nt = (n + kTile - 1) / kTile;
+    return nt * (nt + 1) / 2;
+}
+
+// Row-major upper-triangle tile l -> (ta, tb), ta <= tb.  Row r starts at
+// S(r) = r*nt - r*(r-1)/2.
+__device__ __forceinline__ void decode_tri(uint64_t l, uint32_t nt, uint32_t &ta, uint32_t &tb)
+{
+    const double b = 2.0 * nt + 1.0;
+    double r = floor((b - sqrt(b * b - 8.0 * (double)l)) * 0.5);
+    int ri = (int)r;
+    if (ri < 0) ri = 0;
+    if (ri > (int)nt - 1) ri = (int)nt - 1;
+    auto S = [nt](int rr) -> uint64_t {
+        return (uint64_t)rr * nt - (uint64_t)rr * (uint64_t)(rr - 1) / 2;
+    };
+    while (ri > 0 && S(ri) > l) ri--;
+    while (ri + 1 < (int)nt && S(ri + 1) <= l) ri++;
+    ta = (uint32_t)ri;
+    tb = (uint32_t)(ri + (l - S(ri)));
+}
+
+// Photon records {k, count} of all frames.  A pixel index outside the k-map
+// is counted in invalid[0] and becomes a zero-valued record at k = 0.
+__global__ void gather_records_kernel(const float *__restrict__ kmap, uint32_t npix,
+                                      const uint32_t *__restrict__ pix,
+                                      const uint16_t *__restrict__ cnt, uint64_t n,
+                                      float4 *__restrict__ rec, unsigned long long *invalid)
+{
+    for (uint64_t i = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; i < n;
+         i += (uint64_t)gridDim.x * blockDim.x) {
+        const uint32_t p = pix[i];
+        if (p < npix) {
+            rec[i] = make_float4(kmap[3 * (size_t)p], kmap[3 * (size_t)p + 1], kmap[3 * (size_t)p + 2],
+                                 (float)cnt[i]);
+        } else {
+            rec[i] = make_float4(0.f, 0.f, 0.f, 0.f);
+            atomicAdd(invalid, 1ull);
+        }
+    }
+}
+
+// Exclusive scan of per-frame triangle tile counts into tile_off[0..nframes].
+__global__ void __launch_bounds__(1024) tile_scan_kernel(const uint64_t *__restrict__ frame_off,
+                                                         uint32_t nframes,
+                                                         uint64_t *__restrict__ tile_off)
+{
+    __shared__ uint64_t part[1024];
+    const uint32_t t = threadIdx.x;
+    const uint32_t per = (nframes + 1023) / 1024;
+    const uint32_t f0 = min(nframes, t * per), f1 = min(nframes, f0 + per);
+    uint64_t s = 0;
+    for (uint32_t f = f0; f < f1; f++) s += tri_tiles(frame_off[f + 1] - frame_off[f]);
+    part[t] = s;
+    __syncthreads();
+    for (uint32_t off = 1; off < 1024; off <<= 1) {
+        uint64_t v = t >= off ? part[t - off] : 0;
+        __syncthreads();
+        part[t] += v;
+        __syncthreads();
+    }
+    uint64_t run = t ? part[t - 1] : 0;
+    for (uint32_t f = f0; f < f1; f++) {
+        tile_off[f] = run;
+        run += tri_tiles(frame_off[f + 1] - frame_off[f]);
+    }
+    if (t == 1023) tile_off[nframes] = part[1023];
+}
+
+// kSym: nearest-neighbour second level (bin unordered pairs, add mirror).
+template <bool kSym>
+__global__ void __launch_bounds__(kBlock)
+    ac3d_tiles_kernel(const float4 *__restrict__ rec, const uint64_t *__restrict__ frame_off,
+                      const uint64_t *__restrict__ tile_off, uint32_t nframes, uint64_t single_n,
+                      const float *__restrict__ rots, Rot9 single_rot, iac::BinParams P, unsigned long long *__restrict__ mesh,
+                      unsigned long long *__restrict__ dropped)
+{
+    __shared__ float4 sb[kTile];
+    const bool multi = frame_off != nullptr;
+    const uint64_t total = multi ? tile_off[nframes] : tri_tiles(single_n);
+    const int lane_a = threadIdx.x & (kTile - 1);
+    const int bgrp = threadIdx.x / kTile;
+    GlobalSink sink{mesh, dropped, P.sab, P.sc};
+    const int S = P.sab;
+    const size_t last = (size_t)S * S * P.sc - 1;  // mirror(idx) = last - idx (cubic mesh)
+
+    for (uint64_t t = blockIdx.x; t < total; t += gridDim.x) {
+        uint32_t f = 0;
+        uint64_t base = 0, n = single_n, l = t;
+        if (multi) {
+            uint32_t lo = 0, hi = nframes;
+            while (hi - lo > 1) {
+                const uint32_t mid = (lo + hi) >> 1;
+                if (tile_off[mid] <= t) lo = mid; else hi = mid;
+            }
+            f = lo;
+            base = frame_off[f];
+            n = frame_off[f + 1] - base;
+            l = t - tile_off[f];
+        }
+        const uint32_t nt = (uint32_t)((n + kTile - 1) / kTile);
+        uint32_t ta, tb;
+        decode_tri(l, nt, ta, tb);
+        float R[9];
+        const float *src = (multi && rots) ? rots + 9 * (size_t)f : nullptr;
+#pragma unroll
+        for (int i = 0; i < 9; i++) R[i] = src ? src[i] : single_rot.m[i];
+        iac::BinParams Q = P;
+        Q.rot_identity = (R[0] == 1.0f && R[1] == 0.0f && R[2] == 0.0f && R[3] == 0.0f &&
+                          R[4] == 1.0f && R[5] == 0.0f && R[6] == 0.0f && R[7] == 0.0f &&
+                          R[8] == 1.0f);
+
+        __syncthreads();
+        if (threadIdx.x < kTile) {
+            const uint64_t b = (uint64_t)tb * kTile + threadIdx.x;
+            sb[threadIdx.x] = b < n ? rec[base + b] : make_float4(0.f, 0.f, 0.f, 0.f);
+        }
+        __syncthreads();
+
+        const uint64_t a = (uint64_t)ta * kTile + lane_a;
+        if (a < n) {
+            const float4 ra = rec[base + a];
+            const int j0 = bgrp * kBPerThread;
+            for (int jj = 0; jj < kBPerThread; jj++) {
+                const int bl = j0 + jj;
+                const uint64_t b = (uint64_t)tb * kTile + bl;
+                if (b >= n) break;
+                if (ta == tb && b < a) continue;
+                const float4 rb = sb[bl];
+                if (kSym) {
+                    int dx, dy, dz;
+                    if (!iac::bin_nn(ra.x, ra.y, ra.z, rb.x, rb.y, rb.z, Q, R, dx, dy, dz)) continue;
+                    const uint32_t v = iac::pair_val(ra.w * rb.w);
+                    const int fs = dx + Q.c0, ms = dy + Q.c1, ss = dz + Q.c2;
+                    if ((unsigned)fs < (unsigned)S && (unsigned)ms < (unsigned)S &&
+                        (unsigned)ss < (unsigned)Q.sc) {
+                        if (v) {
+                            const size_t idx = (size_t)fs + (size_t)ms * S + (size_t)ss * S * S;
+                            atomicAdd(mesh + idx, (unsigned long long)v);
+                            if (a != b) atomicAdd(mesh + (last - idx), (unsigned long long)v);
+                        }
+                    } else {
+                        atomicAdd(dropped, a != b ? 2ull : 1ull);
+                    }
+                } else {
+                    const float v = ra.w * rb.w;
+                    iac::add_q_entry(sink, ra.x, ra.y, ra.z, rb.x, rb.y, rb.z, v, Q, R);
+                    if (a != b) iac::add_q_entry(sink, rb.x, rb.y, rb.z, ra.x, ra.y, ra.z, v, Q, R);
+                }
+            }
+        }
+    }
+}
+
+int make_params(const iac_mesh_shape *s, int interp1, int interp2, int double_map,
+                iac::BinParams *P)
+{
+    if (!s) return iac::fail(IAC_ERR_INVALID_ARG, "null mesh shape");
+    if ((interp1 != IAC_NEAREST && interp1 != IAC_LINEAR) ||
+        (interp2 != IAC_NEAREST && interp2 != IAC_LINEAR))
+        return iac::fail(IAC_ERR_INVALID_ARG, "interpolation mode must be 0 or 1");
+    if (s->size_ab % 2 == 0 || s->size_c % 2 == 0 || s->size_ab < 5)
+        return iac::fail(IAC_ERR_EVEN_MESH, "Meshsize must be odd for each dimension");
+    if (s->size_c != s->size_ab)
+        return iac::fail(IAC_ERR_SHAPE, "AC_uw mesh must be cubic (CreateBigMeshForDetector)");
+    P->cut_sq = iac::cut_sq(s->max_q);
+    P->vox = (float)s->voxel_per_dq;
+    P->c0 = (int)s->center[0];
+    P->c1 = (int)s->center[1];
+    P->c2 = (int)s->center[2];
+    P->sab = (int)s->size_ab;
+    P->sc = (int)s->size_c;
+    P->interp1 = interp1;
+    P->interp2 = interp2;
+    P->double_map = double_map ? 1 : 0;
+    P->rot_identity = 1;
+    return IAC_OK;
+}
+
+bool symmetric_mode(const iac::BinParams &P)
+{
+    return P.interp2 == IAC_NEAREST && (!P.double_map || P.interp1 == IAC_NEAREST);
+}
+
+int launch_tiles(iac_ctx *ctx, const float4 *rec, const uint64_t *frame_off, const uint64_t *tile_off,
+                 uint32_t nframes, uint64_t single_n, const float *rots, const Rot9 &single_rot,
+                 const iac::BinParams &P, uint64_t *mesh)
+{
+    uint64_t blocks = (uint64_t)ctx->num_cus * 8;
+    if (!frame_off) {
+        const uint64_t nt = (single_n + kTile - 1) / kTile;
+        blocks = std::min<uint64_t>(blocks, std::max<uint64_t>(1, nt * (nt + 1) / 2));
+    }
+    auto *m = reinterpret_cast<unsigned long long *>(mesh);
+    IAC_TRY(iac::timer_begin(ctx));
+    if (symmetric_mode(P))
+        hipLaunchKernelGGL(ac3d_tiles_kernel<true>, dim3((unsigned)blocks), dim3(kBlock), 0, ctx->stream,
+                           rec, frame_off, tile_off, nframes, single_n, rots, single_rot, P, m,
+                           ctx->d_counters);
+    else
+        hipLaunchKernelGGL(ac3d_tiles_kernel<false>, dim3((unsigned)blocks), dim3(kBlock), 0,
+                           ctx->stream, rec, frame_off, tile_off, nframes, single_n, rots, single_rot,
+                           P, m, ctx->d_counters);
+    IAC_HIP(hipGetLastError());
+    IAC_TRY(iac::timer_end(ctx));
+    return IAC_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+int iac_ac3d_sparse(iac_ctx *ctx, const float *d_hitlist4, uint32_t n, const float rot[9],
+                    const iac_mesh_shape *shape, int interp1, int interp2, int double_map,
+                    uint64_t *d_mesh)
+{
+    IAC_TRY(iac::bind(ctx));
+    if (!d_mesh || !rot) return iac::fail(IAC_ERR_INVALID_ARG, "null mesh or rotation");
+    iac::BinParams P;
+    IAC_TRY(make_params(shape, interp1, interp2, double_map, &P));
+    if (n == 0) return IAC_OK;
+    if (!d_hitlist4) return iac::fail(IAC_ERR_PRECONDITION, "no sparse hit list");
+    Rot9 R;
+    for (int i = 0; i < 9; i++) R.m[i] = rot[i];
+    return launch_tiles(ctx, reinterpret_cast<const float4 *>(d_hitlist4), nullptr, nullptr, 0, n,
+                        nullptr, R, P, d_mesh);
+}
+
+int iac_ac3d_sparse_batch(iac_ctx *ctx, const float *d_kmap, uint32_t npix, const uint32_t *d_pix,
+                          const uint16_t *d_cnt, const uint64_t *d_frame_off, uint32_t nframes,
+                          uint64_t n_total, const float *d_rots, const iac_mesh_shape *shape,
+                          int interp1, int interp2, int double_map, uint64_t *d_mesh)
+{
+    IAC_TRY(iac::bind(ctx));
+    if (!d_mesh || !d_frame_off || !d_kmap || npix == 0)
+        return iac::fail(IAC_ERR_INVALID_ARG, "null mesh, offsets or k-map");
+    iac::BinParams P;
+    IAC_TRY(make_params(shape, interp1, interp2, double_map, &P));
+    if (nframes == 0 || n_total == 0) return IAC_OK;
+    if (!d_pix || !d_cnt) return iac::fail(IAC_ERR_PRECONDITION, "no photon lists");
+    void *rec = nullptr, *toff = nullptr;
+    IAC_TRY(iac::scratch(ctx, 0, n_total * sizeof(float4), &rec));
+    IAC_TRY(iac::scratch(ctx, 1, (size_t)(nframes + 1) * sizeof(uint64_t), &toff));
+    const unsigned gblocks = (unsigned)std::min<uint64_t>((n_total + 255) / 256, 65535);
+    hipLaunchKernelGGL(gather_records_kernel, dim3(gblocks), dim3(256), 0, ctx->stream, d_kmap, npix,
+                       d_pix, d_cnt, n_total, static_cast<float4 *>(rec), ctx->d_counters + 1);
+    IAC_HIP(hipGetLastError());
+    hipLaunchKernelGGL(tile_scan_kernel, dim3(1), dim3(1024), 0, ctx->stream, d_frame_off, nframes,
+                       static_cast<uint64_t *>(toff));
+    IAC_HIP(hipGetLastError());
+    const Rot9 ident{{1, 0, 0, 0, 1, 0, 0, 0, 1}};
+    return launch_tiles(ctx, static_cast<const float4 *>(rec), d_frame_off,
+                        static_cast<const uint64_t *>(toff), nframes, 0, d_rots, ident, P, d_mesh);
+}
+
+}  // extern "C"

@@ -1,0 +1,61 @@
+// iac_internal.h -- context, error plumbing and launch helpers shared by the
+// HIP translation units of libiac_hip.so.  Not part of the C ABI.
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include <cstddef>
+#include <cstdint>
+#include <string>
+#include <vector>
+
+#include "../../include/iac_hip.h"
+
+struct iac_ctx {
+    int device = 0;
+    int num_cus = 0;
+    hipStream_t own_stream = nullptr;
+    hipStream_t stream = nullptr;
+    // device counters: [0] dropped updates, [1..7] reserved
+    unsigned long long *d_counters = nullptr;
+    // growable scratch arenas (device); each call may reuse them
+    void *scratch[4] = {nullptr, nullptr, nullptr, nullptr};
+    size_t scratch_bytes[4] = {0, 0, 0, 0};
+    // measurement: event pairs around the dominant kernel of each call
+    bool timing = false;
+    std::vector<hipEvent_t> ev_start, ev_stop;
+    size_t ev_used = 0;
+};
+
+namespace iac {
+
+// Thread-local detail of the last failure (iac_last_error()).
+void set_error(const std::string &msg);
+int fail(int code, const std::string &msg);
+int hip_fail(hipError_t e, const char *what, const char *file, int line);
+
+// Makes ctx->device current on the calling thread.
+int bind(iac_ctx *ctx);
+// Ensures scratch arena `slot` holds at least `bytes`; returns its pointer in *out.
+int scratch(iac_ctx *ctx, int slot, size_t bytes, void **out);
+
+// Brackets the dominant kernel of a call when timing is enabled.
+int timer_begin(iac_ctx *ctx);
+int timer_end(iac_ctx *ctx);
+
+// Smallest float s >= 0 with sqrtf(s) > max_q (correctly rounded sqrtf).
+// `sqrtf(s) > max_q` is then exactly `s >= cut_sq(max_q)` for s >= 0.
+float cut_sq(float max_q);
+
+}  // namespace iac
+
+#define IAC_HIP(call)                                                              \
+    do {                                                                           \
+        hipError_t e_ = (call);                                                    \
+        if (e_ != hipSuccess) return iac::hip_fail(e_, #call, __FILE__, __LINE__); \
+    } while (0)
+
+#define IAC_TRY(call)            \
+    do {                         \
+        int r_ = (call);         \
+        if (r_ != IAC_OK) return r_; \
+    } while (0)

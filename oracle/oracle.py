@@ -1,0 +1,232 @@
+"""ctypes view of liboracle.so -- the CPU oracle (TEST INFRASTRUCTURE ONLY).
+
+Only tests/, __graft_entry__.smoke() and bench.py's CPU-baseline leg use this
+module, and only as the checker.  See iac_oracle.h for what each function
+restates and where in the reference it comes from.
+"""
+from __future__ import annotations
+
+import ctypes as C
+from pathlib import Path
+
+import numpy as np
+
+_HERE = Path(__file__).resolve().parent
+LIB_PATH = _HERE / "liboracle.so"
+
+
+class MeshShape(C.Structure):
+    _fields_ = [
+        ("k_a", C.c_uint32), ("k_b", C.c_uint32), ("k_c", C.c_uint32),
+        ("size_ab", C.c_uint32), ("size_c", C.c_uint32),
+        ("center", C.c_uint32 * 3),
+        ("dq_per_voxel", C.c_float),
+        ("voxel_per_dq", C.c_uint32),
+        ("max_q", C.c_float),
+    ]
+
+    def as_dict(self) -> dict:
+        return {
+            "k_a": self.k_a, "k_b": self.k_b, "k_c": self.k_c,
+            "size_ab": self.size_ab, "size_c": self.size_c,
+            "center": tuple(self.center), "dq_per_voxel": self.dq_per_voxel,
+            "voxel_per_dq": self.voxel_per_dq, "max_q": self.max_q,
+        }
+
+    @classmethod
+    def from_any(cls, s) -> "MeshShape":
+        d = s.as_dict() if hasattr(s, "as_dict") else dict(s)
+        o = cls()
+        for k, v in d.items():
+            if k == "center":
+                o.center[:] = list(v)
+            else:
+                setattr(o, k, v)
+        return o
+
+    @property
+    def nvox(self) -> int:
+        return self.size_ab * self.size_ab * self.size_c
+
+
+P = C.c_void_p
+F = C.POINTER(C.c_float)
+D = C.POINTER(C.c_double)
+U64P = C.POINTER(C.c_uint64)
+U32 = C.c_uint32
+I32 = C.c_int
+SP = C.POINTER(MeshShape)
+
+_SIG = {
+    "ora_float_to_int_f": (C.c_int64, [C.c_float]),
+    "ora_float_to_int_d": (C.c_int64, [C.c_double]),
+    "ora_int_to_float_u64": (C.c_float, [C.c_uint64]),
+    "ora_pair_val": (C.c_uint32, [C.c_float]),
+    "ora_generate_pixel_map": (None, [I32, I32, C.c_float, F, F, F, F]),
+    "ora_calc_kmap": (None, [F, U32, F, F, F, F]),
+    "ora_mesh_shape_big": (I32, [F, I32, C.c_float, SP]),
+    "ora_mesh_shape_small_cofq": (I32, [F, I32, C.c_float, SP]),
+    "ora_sparse_hit_list": (U32, [F, P, F, U32, C.c_float, C.c_float, F]),
+    "ora_hit_list_from_photons": (None, [F, P, P, U32, F]),
+    "ora_ac3d_sparse": (None, [F, U32, F, SP, I32, I32, I32, U64P, U64P, I32]),
+    "ora_ac3d_sparse_frames": (None, [F, P, P, U64P, U32, P, SP, I32, I32, I32, U64P, U64P, I32]),
+    "ora_ac3d_bruteforce": (None, [F, U32, F, SP, I32, I32, I32, U64P, U64P]),
+    "ora_cq_small_multiplicator": (C.c_double, [F, U32]),
+    "ora_cq_small": (None, [F, F, U32, SP, I32, C.c_double, U64P, U64P, I32]),
+    "ora_merge_prepare": (C.c_double, [F, F, U32, D, C.c_size_t, F]),
+    "ora_merge_cq": (None, [D, SP, F, U32, I32, C.c_double, U64P, U64P, I32]),
+    "ora_ac1d_shape": (None, [F, U32, C.c_float, F, F]),
+    "ora_ac1d_cq_multiplicator": (C.c_double, [F, U32]),
+    "ora_ac1d_cq": (None, [F, F, U32, U32, C.c_float, C.c_float, I32, C.c_double, U64P, U64P, I32]),
+    "ora_ac1d_acuw": (None, [F, U32, U32, C.c_float, C.c_float, I32, I32, D, U64P]),
+    "ora_ac1d_calc_ac": (None, [D, D, U32, D]),
+    "ora_merge_ac_cq": (None, [U64P, D, C.c_size_t, D]),
+    "ora_simulate_fp64": (None, [F, U32, U32, U32, D, U32, D, U32, U32, C.c_double, I32, D, D, I32,
+                                 D, I32]),
+    "ora_philox4x32_10": (None, [C.POINTER(C.c_uint32), C.POINTER(C.c_uint32), C.POINTER(C.c_uint32)]),
+    "ora_uniform2": (None, [C.c_uint64, U32, U32, D]),
+    "ora_exp": (C.c_double, [C.c_double]),
+    "ora_log": (C.c_double, [C.c_double]),
+    "ora_cos": (C.c_double, [C.c_double]),
+    "ora_poisson_sample": (C.c_uint32, [C.c_double, D]),
+    "ora_rescale_poisson": (None, [F, U32, C.c_double, I32, C.c_uint64, U32]),
+}
+
+_lib = None
+
+
+def lib() -> C.CDLL:
+    global _lib
+    if _lib is None:
+        if not LIB_PATH.exists():
+            raise ImportError(f"{LIB_PATH} missing: run `make -C oracle` or __graft_entry__.build()")
+        L = C.CDLL(str(LIB_PATH))
+        for k, (r, a) in _SIG.items():
+            fn = getattr(L, k)
+            fn.restype = r
+            fn.argtypes = a
+        _lib = L
+    return _lib
+
+
+def _f(a):
+    return np.ascontiguousarray(a, dtype=np.float32)
+
+
+def fp(a):
+    return a.ctypes.data_as(F)
+
+
+def dp(a):
+    return a.ctypes.data_as(D)
+
+
+def up(a):
+    return a.ctypes.data_as(U64P)
+
+
+IDENTITY = np.eye(3, dtype=np.float32).reshape(9)
+
+
+# ---- geometry ----------------------------------------------------------------
+def generate_pixel_map(size_a, size_b, pixel_size, center, vec_a, vec_b):
+    out = np.empty((size_a * size_b, 3), np.float32)
+    lib().ora_generate_pixel_map(size_a, size_b, pixel_size, fp(_f(center)), fp(_f(vec_a)),
+                                 fp(_f(vec_b)), fp(out))
+    return out
+
+
+def calc_kmap(pixmap):
+    pm = _f(pixmap).reshape(-1, 3)
+    km = np.empty_like(pm)
+    mx, mn, mq = (np.empty(3, np.float32) for _ in range(3))
+    lib().ora_calc_kmap(fp(pm), pm.shape[0], fp(km), fp(mx), fp(mn), fp(mq))
+    return km, mx, mn, mq
+
+
+def mesh_shape_big(max_q, edge, zoom=1.0):
+    s = MeshShape()
+    r = lib().ora_mesh_shape_big(fp(_f(max_q)), edge, zoom, C.byref(s))
+    if r != 0:
+        raise ValueError("mesh size must be odd")
+    return s
+
+
+def mesh_shape_small_cofq(max_q, perp, zoom=1.0):
+    s = MeshShape()
+    r = lib().ora_mesh_shape_small_cofq(fp(_f(max_q)), perp, zoom, C.byref(s))
+    if r != 0:
+        raise ValueError("mesh size must be odd")
+    return s
+
+
+def detector(n, pixel_um=75.0, dist_um=500000.0):
+    """Flat detector of the benchmark configs: (pixmap, kmap, max_q)."""
+    pm = generate_pixel_map(n, n, pixel_um, [0, 0, dist_um], [1, 0, 0], [0, 1, 0])
+    km, _, _, mq = calc_kmap(pm)
+    return pm, km, mq
+
+
+# ---- sparse hit list -----------------------------------------------------------
+def sparse_hit_list(intensity, kmap, threshold=0.0, step=1.0, mask=None):
+    I = _f(intensity).reshape(-1)
+    km = _f(kmap).reshape(-1, 3)
+    out = np.empty((I.size, 4), np.float32)
+    m = None
+    if mask is not None:
+        m = np.ascontiguousarray(mask, dtype=np.int32)
+    n = lib().ora_sparse_hit_list(fp(I), m.ctypes.data if m is not None else None, fp(km), I.size,
+                                  threshold, step, fp(out))
+    return out[:n].copy()
+
+
+def hit_list_from_photons(kmap, pix, cnt):
+    pix = np.ascontiguousarray(pix, np.uint32)
+    cnt = np.ascontiguousarray(cnt, np.uint16)
+    out = np.empty((pix.size, 4), np.float32)
+    lib().ora_hit_list_from_photons(fp(_f(kmap)), pix.ctypes.data, cnt.ctypes.data, pix.size, fp(out))
+    return out
+
+
+# ---- 3D AC_uw ------------------------------------------------------------------
+def ac3d_sparse(hl, shape, rot=None, interp1=0, interp2=0, double_map=True, mesh=None, nthreads=0):
+    s = MeshShape.from_any(shape)
+    hl = _f(hl).reshape(-1, 4)
+    if mesh is None:
+        mesh = np.zeros(s.nvox, np.uint64)
+    dropped = np.zeros(1, np.uint64)
+    R = _f(IDENTITY if rot is None else rot).reshape(9)
+    lib().ora_ac3d_sparse(fp(hl), hl.shape[0], fp(R), C.byref(s), interp1, interp2, int(double_map),
+                          up(mesh), up(dropped), nthreads)
+    return mesh, int(dropped[0])
+
+
+def ac3d_bruteforce(hl, shape, rot=None, interp1=0, interp2=0, double_map=True):
+    s = MeshShape.from_any(shape)
+    hl = _f(hl).reshape(-1, 4)
+    mesh = np.zeros(s.nvox, np.uint64)
+    dropped = np.zeros(1, np.uint64)
+    R = _f(IDENTITY if rot is None else rot).reshape(9)
+    lib().ora_ac3d_bruteforce(fp(hl), hl.shape[0], fp(R), C.byref(s), interp1, interp2,
+                              int(double_map), up(mesh), up(dropped))
+    return mesh, int(dropped[0])
+
+
+def ac3d_frames(kmap, pix, cnt, frame_off, shape, rots=None, interp1=0, interp2=0, double_map=True,
+                mesh=None, nthreads=0):
+    s = MeshShape.from_any(shape)
+    pix = np.ascontiguousarray(pix, np.uint32)
+    cnt = np.ascontiguousarray(cnt, np.uint16)
+    off = np.ascontiguousarray(frame_off, np.uint64)
+    if mesh is None:
+        mesh = np.zeros(s.nvox, np.uint64)
+    dropped = np.zeros(1, np.uint64)
+    r = None if rots is None else _f(rots).reshape(-1, 9)
+    lib().ora_ac3d_sparse_frames(fp(_f(kmap)), pix.ctypes.data, cnt.ctypes.data, up(off), off.size - 1,
+                                 r.ctypes.data if r is not None else None, C.byref(s), interp1,
+                                 interp2, int(double_map), up(mesh), up(dropped), nthreads)
+    return mesh, int(dropped[0])
+
+
+def pair_val(p: float) -> int:
+    return int(lib().ora_pair_val(float(p)))
