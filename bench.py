@@ -44,6 +44,8 @@ def parse():
                     help="target CPU work of the bounded CPU-baseline sample")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--check", action="store_true", help="bit-exact check of a frame subset")
+    ap.add_argument("--kernel", type=int, default=0, help="0 auto, 1 global-atomic tiles, 2 LDS boxes")
+    ap.add_argument("--no-tune", action="store_true", help="skip the cell-size / frame-chunk sweep")
     return ap.parse_args()
 
 
@@ -84,6 +86,11 @@ def main():
     updates = int(d_mesh.sum().item()) // 1000
     del d_one
     log(f"[bench] ordered pair-updates per step: {updates:.4e} ({updates / args.frames:.4e}/frame)")
+
+    ctx.set_option(iac.OPT_AC3D_KERNEL, args.kernel)
+    tuned = None
+    if args.kernel != 1 and not args.no_tune:
+        tuned = autotune(ctx, iac, d_kmap, d_pix, d_cnt, data, shape, d_mesh, dev)
 
     def step():
         d_mesh.zero_()
@@ -141,6 +148,8 @@ def main():
             "photons": n_total,
             "pair_updates_per_step": updates,
             "seed": args.seed,
+            "kernel": "tile (global atomics)" if args.kernel == 1 else "LDS cell boxes",
+            "tuned": tuned,
         },
         "roofline": {
             "bound": "hbm",
@@ -149,7 +158,7 @@ def main():
             "unit": "GB/s",
             "frac": achieved / HBM_PEAK_GBS,
             "traffic": None,
-            "kernel": "ac3d_tiles_kernel<true>",
+            "kernel": "ac3d_tiles_kernel<true>" if args.kernel == 1 else "pairs_lds_kernel",
             "kernel_avg_ms": k_avg_ms,
             "bytes_per_update": BYTES_PER_UPDATE,
         },
@@ -157,6 +166,37 @@ def main():
     }
     print(json.dumps(out), flush=True)
     ctx.close()
+
+
+def autotune(ctx, iac, d_kmap, d_pix, d_cnt, data, shape, d_mesh, dev):
+    """Pick the LDS kernel's cell side (voxels) and frames per block on a
+    1,000-frame prefix by measured kernel time."""
+    import torch
+
+    off = data["frame_off"]
+    nf = min(1000, len(off) - 1)
+    n = int(off[nf])
+    d_off = torch.from_numpy(off[: nf + 1].view(np.int64)).to(dev)
+    best = None
+    ctx.enable_timing(True)
+    for cv in (3, 4, 6):
+        for fc in (64, 128, 256):
+            ctx.set_option(iac.OPT_CELL_VOX, cv)
+            ctx.set_option(iac.OPT_FRAME_CHUNK, fc)
+            ctx.ac3d_sparse_batch(d_kmap, d_pix[:n], d_cnt[:n], d_off, n, shape, d_mesh)  # plan
+            ctx.kernel_time(reset=True)
+            for _ in range(2):
+                ctx.ac3d_sparse_batch(d_kmap, d_pix[:n], d_cnt[:n], d_off, n, shape, d_mesh)
+            ms, k = ctx.kernel_time(reset=True)
+            t = ms / max(k, 1)
+            log(f"[tune] cell_vox={cv} frame_chunk={fc}: {t:.3f} ms / {nf} frames")
+            if best is None or t < best[0]:
+                best = (t, cv, fc)
+    ctx.enable_timing(False)
+    ctx.set_option(iac.OPT_CELL_VOX, best[1])
+    ctx.set_option(iac.OPT_FRAME_CHUNK, best[2])
+    log(f"[tune] chose cell_vox={best[1]} frame_chunk={best[2]}")
+    return {"cell_vox": best[1], "frame_chunk": best[2]}
 
 
 def check_subset(iac, W, kmap, data, shape, dev):

@@ -84,6 +84,11 @@ int iac_memset_zero(iac_ctx *ctx, void *d_ptr, size_t bytes);
  * launches since the last reset, and resets when `reset` is nonzero. */
 int iac_ctx_enable_timing(iac_ctx *ctx, int enable);
 int iac_ctx_timing(iac_ctx *ctx, double *ms, uint64_t *launches, int reset);
+/* Context options (tuning and kernel selection for measurement and tests). */
+#define IAC_OPT_AC3D_KERNEL 1 /* 0 automatic, 1 global-atomic tile kernel, 2 LDS cell-box kernel */
+#define IAC_OPT_CELL_VOX 2    /* LDS kernel: cell side in voxels, 0 = default (4) */
+#define IAC_OPT_FRAME_CHUNK 3 /* LDS kernel: frames per block, 0 = default (128) */
+int iac_ctx_set_option(iac_ctx *ctx, int option, int64_t value);
 /* Updates whose voxel fell outside the mesh since the context was created.
  * The reference writes such updates out of bounds; this path drops them. */
 int iac_dropped_updates(iac_ctx *ctx, uint64_t *count);

@@ -21,6 +21,11 @@ __all__ = [
 NEAREST = 0
 LINEAR = 1
 
+# iac_ctx_set_option keys (include/iac_hip.h)
+OPT_AC3D_KERNEL = 1
+OPT_CELL_VOX = 2
+OPT_FRAME_CHUNK = 3
+
 
 def _f32(a, n=None) -> np.ndarray:
     arr = np.ascontiguousarray(np.asarray(a, dtype=np.float32))
@@ -112,6 +117,9 @@ class Context:
 
     def sync(self) -> None:
         check(lib().iac_ctx_sync(self._h), "iac_ctx_sync")
+
+    def set_option(self, option: int, value: int) -> None:
+        check(lib().iac_ctx_set_option(self._h, int(option), int(value)), "iac_ctx_set_option")
 
     def enable_timing(self, on: bool = True) -> None:
         check(lib().iac_ctx_enable_timing(self._h, int(bool(on))), "iac_ctx_enable_timing")

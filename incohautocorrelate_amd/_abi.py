@@ -58,6 +58,7 @@ SIGNATURES: dict[str, tuple] = {
     "iac_memcpy_d2h": (I32, [P, P, P, C.c_size_t]),
     "iac_memset_zero": (I32, [P, P, C.c_size_t]),
     "iac_ctx_enable_timing": (I32, [P, I32]),
+    "iac_ctx_set_option": (I32, [P, I32, C.c_int64]),
     "iac_ctx_timing": (I32, [P, C.POINTER(C.c_double), C.POINTER(U64), I32]),
     "iac_dropped_updates": (I32, [P, C.POINTER(U64)]),
     "iac_generate_pixel_map": (I32, [I32, I32, C.c_float, F3, F3, F3, F3]),

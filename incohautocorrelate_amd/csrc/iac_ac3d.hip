@@ -292,6 +292,15 @@ int iac_ac3d_sparse_batch(iac_ctx *ctx, const float *d_kmap, uint32_t npix, cons
     hipLaunchKernelGGL(gather_records_kernel, dim3(gblocks), dim3(256), 0, ctx->stream, d_kmap, npix,
                        d_pix, d_cnt, n_total, static_cast<float4 *>(rec), ctx->d_counters + 1);
     IAC_HIP(hipGetLastError());
+    // the LDS cell-box pipeline covers nearest-neighbour second-level binning of
+    // frames that share the identity rotation; everything else takes the tile kernel
+    const bool lds_ok = symmetric_mode(P) && d_rots == nullptr && n_total < (1ull << 32);
+    if (ctx->ac3d_kernel == 2 && !lds_ok)
+        return iac::fail(IAC_ERR_PRECONDITION,
+                         "the LDS kernel needs nearest-neighbour binning, identity rotation and < 2^32 photons");
+    if (lds_ok && ctx->ac3d_kernel != 1)
+        return iac::ac3d_batch_lds(ctx, d_kmap, npix, static_cast<const float4 *>(rec), d_frame_off, nframes,
+                                   n_total, shape, P, d_mesh, ctx->cell_vox, ctx->frame_chunk);
     hipLaunchKernelGGL(tile_scan_kernel, dim3(1), dim3(1024), 0, ctx->stream, d_frame_off, nframes,
                        static_cast<uint64_t *>(toff));
     IAC_HIP(hipGetLastError());

@@ -159,6 +159,7 @@ int iac_ctx_destroy(iac_ctx *ctx)
     if (ctx->stream) (void)hipStreamSynchronize(ctx->stream);
     for (int i = 0; i < 4; i++)
         if (ctx->scratch[i]) (void)hipFree(ctx->scratch[i]);
+    iac::free_ac3d_plan(ctx);
     for (auto e : ctx->ev_start) (void)hipEventDestroy(e);
     for (auto e : ctx->ev_stop) (void)hipEventDestroy(e);
     if (ctx->d_counters) (void)hipFree(ctx->d_counters);
@@ -259,6 +260,27 @@ int iac_ctx_timing(iac_ctx *ctx, double *ms, uint64_t *launches, int reset)
     *launches = ctx->ev_used;
     if (reset) ctx->ev_used = 0;
     return IAC_OK;
+}
+
+int iac_ctx_set_option(iac_ctx *ctx, int option, int64_t value)
+{
+    IAC_TRY(iac::bind(ctx));
+    switch (option) {
+    case IAC_OPT_AC3D_KERNEL:
+        if (value < 0 || value > 2) return fail(IAC_ERR_INVALID_ARG, "IAC_OPT_AC3D_KERNEL is 0, 1 or 2");
+        ctx->ac3d_kernel = (int)value;
+        return IAC_OK;
+    case IAC_OPT_CELL_VOX:
+        if (value < 0 || value > 64) return fail(IAC_ERR_INVALID_ARG, "IAC_OPT_CELL_VOX is 0..64");
+        ctx->cell_vox = (int)value;
+        return IAC_OK;
+    case IAC_OPT_FRAME_CHUNK:
+        if (value < 0 || value > (1 << 20)) return fail(IAC_ERR_INVALID_ARG, "IAC_OPT_FRAME_CHUNK is 0..2^20");
+        ctx->frame_chunk = (uint32_t)value;
+        return IAC_OK;
+    default:
+        return fail(IAC_ERR_INVALID_ARG, "unknown option");
+    }
 }
 
 int iac_dropped_updates(iac_ctx *ctx, uint64_t *count)

@@ -10,6 +10,8 @@
 
 #include "../../include/iac_hip.h"
 
+struct iac_ac3d_plan;  // cell grid and task table of the LDS pair kernel
+
 struct iac_ctx {
     int device = 0;
     int num_cus = 0;
@@ -24,6 +26,11 @@ struct iac_ctx {
     bool timing = false;
     std::vector<hipEvent_t> ev_start, ev_stop;
     size_t ev_used = 0;
+    iac_ac3d_plan *ac3d_plan = nullptr;
+    // options (IAC_OPT_*)
+    int ac3d_kernel = 0;
+    int cell_vox = 0;
+    uint32_t frame_chunk = 0;
 };
 
 namespace iac {
@@ -41,6 +48,15 @@ int scratch(iac_ctx *ctx, int slot, size_t bytes, void **out);
 // Brackets the dominant kernel of a call when timing is enabled.
 int timer_begin(iac_ctx *ctx);
 int timer_end(iac_ctx *ctx);
+
+void free_ac3d_plan(iac_ctx *ctx);
+
+struct BinParams;
+// LDS cell-box pipeline for nearest-neighbour binning with identity rotation.
+int ac3d_batch_lds(iac_ctx *ctx, const float *d_kmap, uint32_t npix, const float4 *rec,
+                   const uint64_t *d_frame_off, uint32_t nframes, uint64_t n_total,
+                   const iac_mesh_shape *shape, const BinParams &P, uint64_t *d_mesh, int cell_vox,
+                   uint32_t fchunk);
 
 // Smallest float s >= 0 with sqrtf(s) > max_q (correctly rounded sqrtf).
 // `sqrtf(s) > max_q` is then exactly `s >= cut_sq(max_q)` for s >= 0.

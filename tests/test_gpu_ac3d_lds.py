@@ -1,0 +1,116 @@
+"""GPU parity of the LDS cell-box pipeline (nearest-neighbour binning, identity
+rotation) against the oracle and against the global-atomic tile kernel."""
+import numpy as np
+import pytest
+
+import oracle as O
+from gpu_helpers import gpu_batch, mesh_np
+from incohautocorrelate_amd import workloads as W
+
+pytestmark = pytest.mark.gpu
+
+
+@pytest.fixture
+def lds(ctx):
+    import incohautocorrelate_amd as iac
+
+    ctx.set_option(iac.OPT_AC3D_KERNEL, 2)
+    yield ctx
+    ctx.set_option(iac.OPT_AC3D_KERNEL, 0)
+    ctx.set_option(iac.OPT_CELL_VOX, 0)
+    ctx.set_option(iac.OPT_FRAME_CHUNK, 0)
+
+
+def test_lds_cfg1_full(lds, dev):
+    import incohautocorrelate_amd as iac
+
+    _, km, mq = W.detector(128)
+    shape = iac.mesh_shape_big(mq, 63)
+    d = W.dense_poisson_frames(128 * 128, 100, 0.01, seed=1)
+    ref, _ = O.ac3d_frames(km, d["pix"], d["cnt"], d["frame_off"], shape)
+    got = mesh_np(gpu_batch(lds, dev, km, d["pix"], d["cnt"], d["frame_off"], shape))
+    assert np.array_equal(got, ref)
+
+
+def test_lds_cfg2_subset(lds, dev):
+    import incohautocorrelate_amd as iac
+
+    _, km, mq = W.detector(1024)
+    shape = iac.mesh_shape_big(mq, 255)
+    d = W.poisson_photon_frames(1024 * 1024, 24, 0.002, seed=2)
+    ref, _ = O.ac3d_frames(km, d["pix"], d["cnt"], d["frame_off"], shape)
+    got = mesh_np(gpu_batch(lds, dev, km, d["pix"], d["cnt"], d["frame_off"], shape))
+    assert np.array_equal(got, ref)
+
+
+@pytest.mark.parametrize("cell_vox,chunk", [(2, 1), (3, 7), (4, 128), (8, 64), (16, 300)])
+def test_lds_tuning_invariance(lds, dev, cell_vox, chunk):
+    """Every cell size and frame chunk gives the same bits (the tile kernel's)."""
+    import incohautocorrelate_amd as iac
+
+    _, km, mq = W.detector(1024)
+    shape = iac.mesh_shape_big(mq, 255)
+    d = W.poisson_photon_frames(1024 * 1024, 200, 0.002, seed=3)
+    lds.set_option(iac.OPT_AC3D_KERNEL, 1)
+    tile = mesh_np(gpu_batch(lds, dev, km, d["pix"], d["cnt"], d["frame_off"], shape))
+    lds.set_option(iac.OPT_AC3D_KERNEL, 2)
+    lds.set_option(iac.OPT_CELL_VOX, cell_vox)
+    lds.set_option(iac.OPT_FRAME_CHUNK, chunk)
+    got = mesh_np(gpu_batch(lds, dev, km, d["pix"], d["cnt"], d["frame_off"], shape))
+    assert np.array_equal(got, tile)
+
+
+def test_lds_uneven_batch_and_edges(lds, dev):
+    """Lognormal frame sizes with empty and one-photon frames, large counts."""
+    import incohautocorrelate_amd as iac
+
+    _, km, mq = W.detector(256)
+    shape = iac.mesh_shape_big(mq, 127)
+    d = W.lognormal_photon_frames(256 * 256, 60, 400, 1.0, 10, 5000, seed=8)
+    off = np.sort(np.concatenate([d["frame_off"], [0, d["frame_off"][5], d["frame_off"][9] + 1]]))
+    off = off.astype(np.uint64)
+    cnt = d["cnt"].copy()
+    rng = np.random.default_rng(1)
+    cnt[rng.choice(cnt.size, 50, replace=False)] = rng.choice([3, 146, 20000, 65535], 50)
+    ref, _ = O.ac3d_frames(km, d["pix"], cnt, off, shape)
+    got = mesh_np(gpu_batch(lds, dev, km, d["pix"], cnt, off, shape))
+    assert np.array_equal(got, ref)
+
+
+def test_lds_single_cell_hot_spot(lds, dev):
+    """All photons of every frame inside one small patch: one task carries
+    all the work and the centre voxel is hammered."""
+    import incohautocorrelate_amd as iac
+
+    n = 512
+    _, km, mq = W.detector(n)
+    shape = iac.mesh_shape_big(mq, 101)
+    rng = np.random.default_rng(4)
+    pix, off = [], [0]
+    for _ in range(30):
+        rows = rng.integers(250, 262, 300)
+        cols = rng.integers(250, 262, 300)
+        p = np.unique(rows * n + cols)
+        pix.append(p)
+        off.append(off[-1] + p.size)
+    pix = np.concatenate(pix).astype(np.uint32)
+    cnt = rng.integers(1, 4, pix.size).astype(np.uint16)
+    off = np.array(off, np.uint64)
+    ref, _ = O.ac3d_frames(km, pix, cnt, off, shape)
+    got = mesh_np(gpu_batch(lds, dev, km, pix, cnt, off, shape))
+    assert np.array_equal(got, ref)
+
+
+def test_lds_accumulates_into_existing_mesh(lds, dev):
+    import incohautocorrelate_amd as iac
+
+    _, km, mq = W.detector(128)
+    shape = iac.mesh_shape_big(mq, 63)
+    d = W.dense_poisson_frames(128 * 128, 40, 0.01, seed=6)
+    h = d["frame_off"][20]
+    first = (d["pix"][:h], d["cnt"][:h], d["frame_off"][:21])
+    second = (d["pix"][h:], d["cnt"][h:], d["frame_off"][20:] - h)
+    m = gpu_batch(lds, dev, km, *first, shape)
+    m = gpu_batch(lds, dev, km, *second, shape, mesh=m)
+    ref, _ = O.ac3d_frames(km, d["pix"], d["cnt"], d["frame_off"], shape)
+    assert np.array_equal(mesh_np(m), ref)
