@@ -131,6 +131,85 @@ int iac_ac3d_sparse_batch(iac_ctx *ctx, const float *d_kmap, uint32_t npix,
                           const float *d_rots, const iac_mesh_shape *shape, int interp1,
                           int interp2, int double_map, uint64_t *d_mesh);
 
+/* The same frame loop over ready-made hit-list records {kx,ky,kz,value}
+ * (d_rec4, e.g. from iac_sparse_hit_list), frame f owning records
+ * [d_frame_off[f], d_frame_off[f+1]). */
+int iac_ac3d_records_batch(iac_ctx *ctx, const float *d_kmap, uint32_t npix, const float *d_rec4,
+                           const uint64_t *d_frame_off, uint32_t nframes, uint64_t n_total,
+                           const float *d_rots, const iac_mesh_shape *shape, int interp1, int interp2,
+                           int double_map, uint64_t *d_mesh);
+
+/* ---- sparsification ------------------------------------------------------------- */
+/* Detector::CreateSparseHitList(Threshold, PhotonSamplingStep) after the pixel-mask
+ * multiply, IAC/Detector.cpp:926-958, IAC/RunIAC.cpp:530-536, for nframes dense frames
+ * ([nframes][npix] floats): every pixel with I*mask >= threshold and > 0 becomes a
+ * record {k, ceilf((I*mask - threshold)/step)} (step <= 0 keeps I*mask), in pixel
+ * order.  d_mask is [npix] ints or NULL (all ones); d_rec4 needs room for
+ * nframes*npix records; d_frame_off gets nframes+1 offsets; *n_total the count. */
+int iac_sparse_hit_list(iac_ctx *ctx, const float *d_kmap, uint32_t npix, const float *d_frames,
+                        uint32_t nframes, const int *d_mask, float threshold, float step,
+                        float *d_rec4, uint64_t *d_frame_off, uint64_t *n_total);
+
+/* ---- AC1D: angular-averaged correlation ---------------------------------------- */
+/* AC1D::Initialize(Det, ArraySize, QZoom), IAC/AC1D.cpp:53-58 (host). */
+int iac_ac1d_shape(const float max_q[3], uint32_t size, float q_zoom, float *max_q_out, float *dq_out);
+/* AC1D::Calculate_CQ multiplicator, IAC/AC1D.cpp:78-90 (host). */
+int iac_ac1d_cq_multiplicator(const float *intensity, uint32_t npix, double *mult);
+/* AC1D::Calculate_CQ / AutoCorr_CQ_AV, IAC/AC1D.cpp:61-195, IAC/kernel.cl:656-764: every
+ * ordered pixel pair adds (long)(Val*mult) (nearest) or its linear split to the uint64
+ * radial histogram d_hist (size bins, accumulating); the reference's double C(q)
+ * is d_hist / mult.  size <= 6144. */
+int iac_ac1d_cq(iac_ctx *ctx, const float *d_kmap, uint32_t npix, const float *d_intensity,
+                uint32_t size, float max_q, float dq, int interp, double mult, uint64_t *d_hist);
+/* AC1D::Calculate_AC_UW_MR, CPU branch of Calculate_AC_UW_Mapped, IAC/AC1D.cpp:256-293:
+ * photon pairs j <= k of every frame add 2*(c_j*c_k) (or its linear split) to the
+ * double histogram d_hist (accumulating).  self_pair_factor 2 weights self pairs
+ * like the reference; 1 counts them once. */
+int iac_ac1d_acuw(iac_ctx *ctx, const float *d_kmap, uint32_t npix, const uint32_t *d_pix,
+                  const uint16_t *d_cnt, const uint64_t *d_frame_off, uint32_t nframes,
+                  uint64_t n_total, uint32_t size, float max_q, float dq, int interp,
+                  int self_pair_factor, double *d_hist);
+/* AC1D::CalcAC, IAC/AC1D.cpp:482-493: ac = ac_uw / cq, NaN -> 0. */
+int iac_ac1d_calc_ac(iac_ctx *ctx, const double *d_ac_uw, const double *d_cq, uint32_t size,
+                     double *d_ac);
+
+/* ---- C(q): detector-pair normalisation mesh ------------------------------------ */
+/* Multiplicator of Detector::AutoCorrelate_CofQ_SmallMesh, IAC/Detector.cpp:1215-1227:
+ * the smallest 10^k >= 1e-10 with <I>^2 * 10^k >= 1, times 1e5 (host). */
+int iac_cq_small_multiplicator(const float *intensity, uint32_t npix, double *mult);
+/* Detector::AutoCorrelate_CofQ_SmallMesh / AutoCorr_CQ_small, IAC/Detector.cpp:1184-1315,
+ * IAC/kernel.cl:27-184: every ordered pixel pair of d_intensity (npix floats)
+ * adds (uint64)((double)(I_a*I_b) * mult) (pairs below 1e-37 skipped) to the
+ * uint64 small mesh d_mesh (size_ab^2*size_c, axes k_a/k_b/k_c), accumulating. */
+int iac_cq_small(iac_ctx *ctx, const float *d_kmap, uint32_t npix, const float *d_intensity,
+                 const iac_mesh_shape *small_shape, int interp, double mult, uint64_t *d_mesh);
+/* The same binning for nframes dense frames d_frames ([nframes][npix]); frames
+ * [h_group_off[g], h_group_off[g+1]) accumulate into mesh g of d_meshes
+ * (ngroups consecutive meshes) -- e.g. one group per crystal orientation. */
+int iac_cq_dense_frames(iac_ctx *ctx, const float *d_kmap, uint32_t npix, const float *d_frames,
+                        uint32_t nframes, const uint32_t *h_group_off, uint32_t ngroups,
+                        const iac_mesh_shape *shape, int interp, double mult, uint64_t *d_meshes);
+/* d_out[i] = (double)d_in[i] / mult (IAC/Detector.cpp:1304-1308, 1525-1534). */
+int iac_u64_to_f64(iac_ctx *ctx, const uint64_t *d_in, size_t n, double mult, double *d_out);
+/* Detector::Merge_smallCofQ host preparation, IAC/Detector.cpp:1339-1427: rw10 gets
+ * the 9 rotation entries and the weight <I>^2 (quantised to the Multiplicator)
+ * per event; *mult_out gets the Multiplicator (host). */
+int iac_merge_prepare(const float *rot9, const float *mean_intensity, uint32_t nev,
+                      const double *small_cq, size_t nsmall, float *rw10, double *mult_out);
+/* Merge_CQ, IAC/kernel.cl:189-344: every small-mesh voxel (double C(q), device)
+ * rotated by each event's matrix and weighted, into the cubic uint64 big mesh
+ * d_big (size_ab^3), accumulating. */
+int iac_merge_cq(iac_ctx *ctx, const double *d_small_cq, const iac_mesh_shape *small_shape,
+                 const float *d_rw10, uint32_t nev, int interp, double mult, uint64_t *d_big);
+/* Merge_CQ with one small mesh per event: d_small_meshes holds nev consecutive
+ * double meshes (e.g. one dense C(q) per crystal orientation); event i rotates
+ * mesh i.  Otherwise as iac_merge_cq. */
+int iac_merge_cq_multi(iac_ctx *ctx, const double *d_small_meshes, const iac_mesh_shape *small_shape,
+                       const float *d_rw10, uint32_t nev, int interp, double mult, uint64_t *d_big);
+/* RunIAC::Merge_ACandCQ, IAC/RunIAC.cpp:120-158: out = IntToFloat(ac) / cq, 0 where
+ * cq <= 0 or the quotient is NaN. */
+int iac_merge_ac_cq(iac_ctx *ctx, const uint64_t *d_ac, const double *d_cq, size_t n, double *d_out);
+
 #ifdef __cplusplus
 }
 #endif

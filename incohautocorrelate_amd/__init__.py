@@ -71,6 +71,41 @@ def mesh_shape_small_cofq(max_q, perp_size: int, q_zoom: float = 1.0) -> MeshSha
     return s
 
 
+def cq_small_multiplicator(intensity: np.ndarray) -> float:
+    """Detector::AutoCorrelate_CofQ_SmallMesh multiplicator (Detector.cpp:1215-1227)."""
+    I = _f32(intensity)
+    m = C.c_double()
+    check(lib().iac_cq_small_multiplicator(fptr(I), I.size, C.byref(m)), "iac_cq_small_multiplicator")
+    return m.value
+
+
+def merge_prepare(rot9: np.ndarray, mean_intensity: np.ndarray, small_cq: np.ndarray):
+    """Detector::Merge_smallCofQ host preparation (Detector.cpp:1339-1427): (rw10, mult)."""
+    R = _f32(rot9).reshape(-1, 9)
+    mi = _f32(mean_intensity)
+    cq = np.ascontiguousarray(small_cq, dtype=np.float64)
+    rw = np.empty((R.shape[0], 10), np.float32)
+    m = C.c_double()
+    check(lib().iac_merge_prepare(fptr(R), fptr(mi), R.shape[0], cq.ctypes.data_as(C.POINTER(C.c_double)),
+                                  cq.size, fptr(rw), C.byref(m)), "iac_merge_prepare")
+    return rw, m.value
+
+
+def ac1d_shape(max_q, size: int, q_zoom: float = 1.0) -> tuple[float, float]:
+    """AC1D::Initialize (AC1D.cpp:53-58): (Max_Q, dq_per_Step)."""
+    mq = _f32(max_q, 3)
+    a, b = np.zeros(1, np.float32), np.zeros(1, np.float32)
+    check(lib().iac_ac1d_shape(fptr(mq), int(size), float(q_zoom), fptr(a), fptr(b)), "iac_ac1d_shape")
+    return float(a[0]), float(b[0])
+
+
+def ac1d_cq_multiplicator(intensity: np.ndarray) -> float:
+    I = _f32(intensity)
+    m = C.c_double()
+    check(lib().iac_ac1d_cq_multiplicator(fptr(I), I.size, C.byref(m)), "iac_ac1d_cq_multiplicator")
+    return m.value
+
+
 def _ptr(t) -> int | None:
     if t is None:
         return None
@@ -156,3 +191,63 @@ class Context:
                                           C.byref(shape), interp1, interp2, int(bool(double_map)),
                                           _ptr(mesh)),
               "iac_ac3d_sparse_batch")
+
+    def ac3d_records_batch(self, kmap, rec4, frame_off, n_total: int, shape: MeshShape, mesh, rots=None,
+                           interp1: int = NEAREST, interp2: int = NEAREST, double_map: bool = True) -> None:
+        nframes = frame_off.shape[0] - 1
+        check(lib().iac_ac3d_records_batch(self._h, _ptr(kmap), kmap.shape[0], _ptr(rec4), _ptr(frame_off),
+                                           nframes, int(n_total), _ptr(rots), C.byref(shape), interp1, interp2,
+                                           int(bool(double_map)), _ptr(mesh)), "iac_ac3d_records_batch")
+
+    def sparse_hit_list(self, kmap, frames, mask, threshold: float, step: float, rec4, frame_off) -> int:
+        """CreateSparseHitList over dense frames [nframes, npix]; returns the record count."""
+        n = C.c_uint64()
+        check(lib().iac_sparse_hit_list(self._h, _ptr(kmap), kmap.shape[0], _ptr(frames), frames.shape[0],
+                                        _ptr(mask), float(threshold), float(step), _ptr(rec4), _ptr(frame_off),
+                                        C.byref(n)), "iac_sparse_hit_list")
+        return int(n.value)
+
+    # ---- AC1D --------------------------------------------------------------------
+    def ac1d_cq(self, kmap, intensity, size: int, max_q: float, dq: float, interp: int, mult: float, hist):
+        check(lib().iac_ac1d_cq(self._h, _ptr(kmap), kmap.shape[0], _ptr(intensity), int(size), float(max_q),
+                                float(dq), interp, float(mult), _ptr(hist)), "iac_ac1d_cq")
+
+    def ac1d_acuw(self, kmap, pix, cnt, frame_off, n_total: int, size: int, max_q: float, dq: float,
+                  interp: int, hist, self_pair_factor: int = 2):
+        check(lib().iac_ac1d_acuw(self._h, _ptr(kmap), kmap.shape[0], _ptr(pix), _ptr(cnt), _ptr(frame_off),
+                                  frame_off.shape[0] - 1, int(n_total), int(size), float(max_q), float(dq), interp,
+                                  int(self_pair_factor), _ptr(hist)), "iac_ac1d_acuw")
+
+    def ac1d_calc_ac(self, ac_uw, cq, out):
+        check(lib().iac_ac1d_calc_ac(self._h, _ptr(ac_uw), _ptr(cq), ac_uw.numel(), _ptr(out)), "iac_ac1d_calc_ac")
+
+    # ---- C(q) ------------------------------------------------------------------
+    def cq_small(self, kmap, intensity, shape: MeshShape, mesh, interp: int, mult: float) -> None:
+        """AutoCorr_CQ_small over one intensity image into the uint64 small mesh."""
+        check(lib().iac_cq_small(self._h, _ptr(kmap), kmap.shape[0], _ptr(intensity), C.byref(shape),
+                                 interp, float(mult), _ptr(mesh)), "iac_cq_small")
+
+    def cq_dense_frames(self, kmap, frames, group_off, shape: MeshShape, meshes, interp: int,
+                        mult: float) -> None:
+        """Dense frames [nframes, npix]; frames of group g accumulate into meshes[g]."""
+        goff = np.ascontiguousarray(group_off, dtype=np.uint32)
+        check(lib().iac_cq_dense_frames(self._h, _ptr(kmap), kmap.shape[0], _ptr(frames), frames.shape[0],
+                                        goff.ctypes.data_as(C.POINTER(C.c_uint32)), goff.size - 1,
+                                        C.byref(shape), interp, float(mult), _ptr(meshes)),
+              "iac_cq_dense_frames")
+
+    def u64_to_f64(self, src, mult: float, out) -> None:
+        check(lib().iac_u64_to_f64(self._h, _ptr(src), src.numel(), float(mult), _ptr(out)), "iac_u64_to_f64")
+
+    def merge_cq(self, small_cq, shape: MeshShape, rw10, interp: int, mult: float, big) -> None:
+        """Merge_CQ: rotate and weight the small C(q) into the cubic big mesh."""
+        check(lib().iac_merge_cq(self._h, _ptr(small_cq), C.byref(shape), _ptr(rw10), rw10.shape[0], interp,
+                                 float(mult), _ptr(big)), "iac_merge_cq")
+
+    def merge_cq_multi(self, small_meshes, shape: MeshShape, rw10, interp: int, mult: float, big) -> None:
+        """Merge_CQ with one small mesh per event (meshes[i] rotated by event i)."""
+        check(lib().iac_merge_cq_multi(self._h, _ptr(small_meshes), C.byref(shape), _ptr(rw10), rw10.shape[0],
+                                       interp, float(mult), _ptr(big)), "iac_merge_cq_multi")
+
+    def merge_ac_cq(self, ac, cq, out) -> None:
+        check(lib().iac_merge_ac_cq(self._h, _ptr(ac), _ptr(cq), ac.numel(), _ptr(out)), "iac_merge_ac_cq")

@@ -68,6 +68,23 @@ SIGNATURES: dict[str, tuple] = {
     "iac_ac3d_sparse": (I32, [P, P, U32, F3, C.POINTER(MeshShape), I32, I32, I32, P]),
     "iac_ac3d_sparse_batch": (I32, [P, P, U32, P, P, P, U32, U64, P, C.POINTER(MeshShape),
                                     I32, I32, I32, P]),
+    "iac_ac3d_records_batch": (I32, [P, P, U32, P, P, U32, U64, P, C.POINTER(MeshShape), I32, I32, I32, P]),
+    "iac_sparse_hit_list": (I32, [P, P, U32, P, U32, P, C.c_float, C.c_float, P, P, C.POINTER(U64)]),
+    "iac_ac1d_shape": (I32, [F3, U32, C.c_float, F3, F3]),
+    "iac_ac1d_cq_multiplicator": (I32, [F3, U32, C.POINTER(C.c_double)]),
+    "iac_ac1d_cq": (I32, [P, P, U32, P, U32, C.c_float, C.c_float, I32, C.c_double, P]),
+    "iac_ac1d_acuw": (I32, [P, P, U32, P, P, P, U32, U64, U32, C.c_float, C.c_float, I32, I32, P]),
+    "iac_ac1d_calc_ac": (I32, [P, P, P, U32, P]),
+    "iac_cq_small_multiplicator": (I32, [F3, U32, C.POINTER(C.c_double)]),
+    "iac_cq_small": (I32, [P, P, U32, P, C.POINTER(MeshShape), I32, C.c_double, P]),
+    "iac_cq_dense_frames": (I32, [P, P, U32, P, U32, C.POINTER(U32), U32, C.POINTER(MeshShape), I32,
+                                  C.c_double, P]),
+    "iac_u64_to_f64": (I32, [P, P, C.c_size_t, C.c_double, P]),
+    "iac_merge_prepare": (I32, [F3, F3, U32, C.POINTER(C.c_double), C.c_size_t, F3,
+                                C.POINTER(C.c_double)]),
+    "iac_merge_cq": (I32, [P, P, C.POINTER(MeshShape), P, U32, I32, C.c_double, P]),
+    "iac_merge_cq_multi": (I32, [P, P, C.POINTER(MeshShape), P, U32, I32, C.c_double, P]),
+    "iac_merge_ac_cq": (I32, [P, P, P, C.c_size_t, P]),
 }
 
 

@@ -1,0 +1,364 @@
+// iac_ac1d.hip -- AC1D: angular-averaged (radial) pair histograms on gfx950.
+//
+//   iac_ac1d_cq:   AutoCorr_CQ_AV (IAC/kernel.cl:656-764), every ordered pixel
+//                  pair of one intensity image, uint64 fixed point per bin.
+//   iac_ac1d_acuw: Calculate_AC_UW_Mapped's CPU branch (IAC/AC1D.cpp:256-293),
+//                  photon pairs j <= k of each frame weighted 2 (self pairs
+//                  too, as the reference does), accumulated in double.
+//   iac_ac1d_calc_ac: AC1D::CalcAC (IAC/AC1D.cpp:482-493).
+//
+// |q| and its bin are the same for (a,b) and (b,a), so the C(q) kernel bins
+// each unordered pair once with twice its value.  Every block keeps a private
+// histogram of all bins in LDS across many 64x64 pair tiles (persistent grid)
+// and flushes it once.  Integer photon counts in nearest-neighbour mode make
+// every AC_uw value an integer-valued float: those runs accumulate in uint64
+// and convert at the end, which reproduces the reference's double sums bit
+// for bit (all partial sums are integers below 2^53).
+#include <algorithm>
+#include <cmath>
+
+#include "iac_internal.h"
+
+#pragma clang fp contract(off)
+
+namespace {
+
+constexpr int kTile = 64;
+constexpr int kBlock = 256;
+constexpr int kBPerThread = kTile / (kBlock / kTile);  // 16
+constexpr int kMaxBins = 6144;                         // 48 KiB of uint64 / double in LDS
+
+__device__ __forceinline__ uint64_t tri_tiles(uint64_t n)
+{
+    const uint64_t nt = (n + kTile - 1) / kTile;
+    return nt * (nt + 1) / 2;
+}
+
+__device__ __forceinline__ void decode_tri(uint64_t l, uint32_t nt, uint32_t &ta, uint32_t &tb)
+{
+    const double b = 2.0 * nt + 1.0;
+    int ri = (int)floor((b - sqrt(b * b - 8.0 * (double)l)) * 0.5);
+    ri = max(0, min(ri, (int)nt - 1));
+    auto S = [nt](int rr) -> uint64_t { return (uint64_t)rr * nt - (uint64_t)rr * (uint64_t)(rr - 1) / 2; };
+    while (ri > 0 && S(ri) > l) ri--;
+    while (ri + 1 < (int)nt && S(ri + 1) <= l) ri++;
+    ta = (uint32_t)ri;
+    tb = (uint32_t)(ri + (l - S(ri)));
+}
+
+// |k_a - k_b| as the reference computes it: a float expression of repeated
+// differences, then a correctly rounded float sqrt.
+__device__ __forceinline__ float qnorm(const float4 &a, const float4 &b)
+{
+    return sqrtf((a.x - b.x) * (a.x - b.x) + (a.y - b.y) * (a.y - b.y) + (a.z - b.z) * (a.z - b.z));
+}
+
+// AutoCorr_CQ_AV over one image: records are {k, I} of every pixel.
+__global__ void __launch_bounds__(kBlock)
+    ac1d_cq_kernel(const float4 *__restrict__ rec, uint32_t n, uint32_t size, float max_q, float dq, int interp,
+                   double mult, unsigned long long *__restrict__ hist, unsigned long long *__restrict__ dropped)
+{
+    __shared__ unsigned long long h[kMaxBins];
+    __shared__ float4 sb[kTile];
+    for (uint32_t i = threadIdx.x; i < size; i += kBlock) h[i] = 0ull;
+    unsigned long long drop = 0;
+    const uint64_t total = tri_tiles(n);
+    const uint32_t nt = (n + kTile - 1) / kTile;
+    const int lane_a = threadIdx.x & (kTile - 1), bgrp = threadIdx.x / kTile;
+    for (uint64_t t = blockIdx.x; t < total; t += gridDim.x) {
+        uint32_t ta, tb;
+        decode_tri(t, nt, ta, tb);
+        __syncthreads();
+        if (threadIdx.x < kTile) {
+            const uint32_t b = tb * kTile + threadIdx.x;
+            sb[threadIdx.x] = b < n ? rec[b] : make_float4(0.f, 0.f, 0.f, 0.f);
+        }
+        __syncthreads();
+        const uint32_t a = ta * kTile + lane_a;
+        if (a >= n) continue;
+        const float4 ra = rec[a];
+        for (int jj = 0; jj < kBPerThread; jj++) {
+            const int bl = bgrp * kBPerThread + jj;
+            const uint32_t b = tb * kTile + bl;
+            if (b >= n) break;
+            if (ta == tb && b < a) continue;
+            const float4 rb = sb[bl];
+            const double val = (double)(ra.w * rb.w);
+            if (val < (double)1e-37f) continue;
+            float q = qnorm(ra, rb);
+            if (q > max_q) continue;
+            q = q / dq;
+            const unsigned long long rep = (a == b) ? 1ull : 2ull;
+            if (interp == 0) {
+                const uint32_t sc = (uint32_t)floor((double)q + 0.5);
+                const unsigned long long v = (unsigned long long)(long long)(val * mult);
+                if (sc < size) { if (v) atomicAdd(&h[sc], v * rep); } else drop += rep;
+            } else {
+                const uint32_t sc1 = (uint32_t)floorf(q), sc2 = sc1 + 1;
+                const double sep = (double)(q - floorf(q));
+                const unsigned long long v1 = (unsigned long long)(long long)((val * (1 - sep)) * mult);
+                const unsigned long long v2 = (unsigned long long)(long long)((val * sep) * mult);
+                if (sc1 < size) { if (v1) atomicAdd(&h[sc1], v1 * rep); } else drop += rep;
+                if (sc2 < size) { if (v2) atomicAdd(&h[sc2], v2 * rep); } else drop += rep;
+            }
+        }
+    }
+    __syncthreads();
+    for (uint32_t i = threadIdx.x; i < size; i += kBlock)
+        if (h[i]) atomicAdd(hist + i, h[i]);
+    if (drop) atomicAdd(dropped, drop);
+}
+
+// Exclusive scan of per-frame triangle tile counts (diagonal included).
+__global__ void __launch_bounds__(1024) tile_scan_kernel(const uint64_t *__restrict__ frame_off, uint32_t nframes,
+                                                         uint64_t *__restrict__ tile_off)
+{
+    __shared__ uint64_t part[1024];
+    const uint32_t t = threadIdx.x;
+    const uint32_t per = (nframes + 1023) / 1024;
+    const uint32_t f0 = min(nframes, t * per), f1 = min(nframes, f0 + per);
+    uint64_t s = 0;
+    for (uint32_t f = f0; f < f1; f++) s += tri_tiles(frame_off[f + 1] - frame_off[f]);
+    part[t] = s;
+    __syncthreads();
+    for (uint32_t o = 1; o < 1024; o <<= 1) {
+        const uint64_t v = t >= o ? part[t - o] : 0;
+        __syncthreads();
+        part[t] += v;
+        __syncthreads();
+    }
+    uint64_t run = t ? part[t - 1] : 0;
+    for (uint32_t f = f0; f < f1; f++) {
+        tile_off[f] = run;
+        run += tri_tiles(frame_off[f + 1] - frame_off[f]);
+    }
+    if (t == 1023) tile_off[nframes] = part[1023];
+}
+
+__global__ void gather_kernel(const float *__restrict__ kmap, uint32_t npix, const uint32_t *__restrict__ pix,
+                              const uint16_t *__restrict__ cnt, uint64_t n, float4 *__restrict__ rec,
+                              unsigned long long *invalid)
+{
+    for (uint64_t i = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; i < n; i += (uint64_t)gridDim.x * blockDim.x) {
+        const uint32_t p = pix[i];
+        if (p < npix) {
+            rec[i] = make_float4(kmap[3 * (size_t)p], kmap[3 * (size_t)p + 1], kmap[3 * (size_t)p + 2], (float)cnt[i]);
+        } else {
+            rec[i] = make_float4(0.f, 0.f, 0.f, 0.f);
+            atomicAdd(invalid, 1ull);
+        }
+    }
+}
+
+__global__ void gather_image_kernel(const float *__restrict__ kmap, const float *__restrict__ I, uint32_t npix,
+                                    float4 *__restrict__ rec)
+{
+    for (uint32_t p = blockIdx.x * blockDim.x + threadIdx.x; p < npix; p += gridDim.x * blockDim.x)
+        rec[p] = make_float4(kmap[3 * (size_t)p], kmap[3 * (size_t)p + 1], kmap[3 * (size_t)p + 2], I[p]);
+}
+
+// AC_uw: photon pairs j <= k of every frame.  kExact: integer-valued weights
+// accumulate in uint64 (nearest neighbour); otherwise double.
+template <bool kExact>
+__global__ void __launch_bounds__(kBlock)
+    ac1d_acuw_kernel(const float4 *__restrict__ rec, const uint64_t *__restrict__ frame_off,
+                     const uint64_t *__restrict__ tile_off, uint32_t nframes, uint32_t size, float max_q, float dq,
+                     int interp, float self_factor, unsigned long long *__restrict__ hist_u,
+                     double *__restrict__ hist_d, unsigned long long *__restrict__ dropped)
+{
+    __shared__ unsigned long long hu[kExact ? kMaxBins : 1];
+    __shared__ double hd[kExact ? 1 : kMaxBins];
+    __shared__ float4 sb[kTile];
+    for (uint32_t i = threadIdx.x; i < size; i += kBlock) {
+        if (kExact) hu[i] = 0ull; else hd[i] = 0.0;
+    }
+    unsigned long long drop = 0;
+    const uint64_t total = tile_off[nframes];
+    const int lane_a = threadIdx.x & (kTile - 1), bgrp = threadIdx.x / kTile;
+    for (uint64_t t = blockIdx.x; t < total; t += gridDim.x) {
+        uint32_t lo = 0, hi = nframes;
+        while (hi - lo > 1) {
+            const uint32_t mid = (lo + hi) >> 1;
+            if (tile_off[mid] <= t) lo = mid; else hi = mid;
+        }
+        const uint64_t base = frame_off[lo];
+        const uint32_t n = (uint32_t)(frame_off[lo + 1] - base);
+        const uint32_t nt = (n + kTile - 1) / kTile;
+        uint32_t ta, tb;
+        decode_tri(t - tile_off[lo], nt, ta, tb);
+        __syncthreads();
+        if (threadIdx.x < kTile) {
+            const uint32_t b = tb * kTile + threadIdx.x;
+            sb[threadIdx.x] = b < n ? rec[base + b] : make_float4(0.f, 0.f, 0.f, 0.f);
+        }
+        __syncthreads();
+        const uint32_t a = ta * kTile + lane_a;
+        if (a >= n) continue;
+        const float4 ra = rec[base + a];
+        for (int jj = 0; jj < kBPerThread; jj++) {
+            const int bl = bgrp * kBPerThread + jj;
+            const uint32_t b = tb * kTile + bl;
+            if (b >= n) break;
+            if (ta == tb && b < a) continue;
+            const float4 rb = sb[bl];
+            float q = qnorm(ra, rb);
+            if (q > max_q) continue;
+            q = q / dq;
+            const float f2 = (a == b) ? self_factor : 2.0f;
+            if (interp == 0) {
+                const uint32_t sc = (uint32_t)floor((double)q + 0.5);
+                const float v = f2 * (ra.w * rb.w);
+                if (sc < size) {
+                    if (kExact) atomicAdd(&hu[sc], (unsigned long long)v);
+                    else atomicAdd(&hd[sc], (double)v);
+                } else {
+                    drop++;
+                }
+            } else {
+                const uint32_t sc1 = (uint32_t)floorf(q), sc2 = sc1 + 1;
+                const float sep = q - (float)sc1;
+                const float v1 = f2 * ((ra.w * rb.w) * (1 - sep));
+                const float v2 = f2 * ((ra.w * rb.w) * sep);
+                if (sc1 < size) atomicAdd(&hd[sc1], (double)v1); else drop++;
+                if (sc2 < size) atomicAdd(&hd[sc2], (double)v2); else drop++;
+            }
+        }
+    }
+    __syncthreads();
+    for (uint32_t i = threadIdx.x; i < size; i += kBlock) {
+        if (kExact) {
+            if (hu[i]) atomicAdd(hist_u + i, hu[i]);
+        } else {
+            if (hd[i] != 0.0) atomicAdd(hist_d + i, hd[i]);
+        }
+    }
+    if (drop) atomicAdd(dropped, drop);
+}
+
+__global__ void add_u64_as_f64_kernel(const unsigned long long *__restrict__ in, uint32_t n, double *__restrict__ out)
+{
+    for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x)
+        out[i] += (double)in[i];
+}
+
+__global__ void zero_u64_kernel(unsigned long long *p, uint32_t n)
+{
+    for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) p[i] = 0ull;
+}
+
+__global__ void calc_ac_kernel(const double *__restrict__ ac_uw, const double *__restrict__ cq, uint32_t n,
+                               double *__restrict__ ac)
+{
+    for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
+        double v = ac_uw[i] / cq[i];
+        if (isnan(v)) v = 0;
+        ac[i] = v;
+    }
+}
+
+unsigned persistent_blocks(iac_ctx *ctx) { return (unsigned)std::max(1, ctx->num_cus * 2); }
+
+}  // namespace
+
+extern "C" {
+
+int iac_ac1d_shape(const float max_q[3], uint32_t size, float q_zoom, float *max_q_out, float *dq_out)
+{
+    if (!max_q || !max_q_out || !dq_out || size < 3) return iac::fail(IAC_ERR_INVALID_ARG, "bad AC1D shape request");
+    const float m = std::sqrt(max_q[0] * max_q[0] + max_q[1] * max_q[1] + max_q[2] * max_q[2]) / q_zoom;
+    *max_q_out = m;
+    *dq_out = m / ((float)(size - 2));
+    return IAC_OK;
+}
+
+int iac_ac1d_cq_multiplicator(const float *intensity, uint32_t npix, double *mult)
+{
+    if (!intensity || !mult || npix == 0) return iac::fail(IAC_ERR_INVALID_ARG, "null input");
+    double t = 0;
+    for (uint32_t i = 0; i < npix; i++) t += intensity[i];
+    const float mean = (float)(t / npix);
+    double m = 1e-10;
+    for (; 1 > mean * mean * m;) m *= 10;
+    m = m * 1e5;
+    if (m > 1e18) m = 1e18;
+    *mult = m;
+    return IAC_OK;
+}
+
+int iac_ac1d_cq(iac_ctx *ctx, const float *d_kmap, uint32_t npix, const float *d_intensity, uint32_t size,
+                float max_q, float dq, int interp, double mult, uint64_t *d_hist)
+{
+    IAC_TRY(iac::bind(ctx));
+    if (!d_kmap || !d_intensity || !d_hist || npix == 0) return iac::fail(IAC_ERR_INVALID_ARG, "null input");
+    if (size == 0 || size > (uint32_t)kMaxBins) return iac::fail(IAC_ERR_LIMIT, "AC1D size must be 1..6144 bins");
+    if (interp != IAC_NEAREST && interp != IAC_LINEAR) return iac::fail(IAC_ERR_INVALID_ARG, "bad interpolation");
+    void *rec = nullptr;
+    IAC_TRY(iac::scratch(ctx, 0, (size_t)npix * sizeof(float4), &rec));
+    hipLaunchKernelGGL(gather_image_kernel, dim3(std::min<uint32_t>(4096, (npix + 255) / 256)), dim3(256), 0,
+                       ctx->stream, d_kmap, d_intensity, npix, static_cast<float4 *>(rec));
+    IAC_HIP(hipGetLastError());
+    IAC_TRY(iac::timer_begin(ctx));
+    hipLaunchKernelGGL(ac1d_cq_kernel, dim3(persistent_blocks(ctx)), dim3(kBlock), 0, ctx->stream,
+                       static_cast<const float4 *>(rec), npix, size, max_q, dq, interp, mult,
+                       reinterpret_cast<unsigned long long *>(d_hist), ctx->d_counters);
+    IAC_HIP(hipGetLastError());
+    IAC_TRY(iac::timer_end(ctx));
+    return IAC_OK;
+}
+
+int iac_ac1d_acuw(iac_ctx *ctx, const float *d_kmap, uint32_t npix, const uint32_t *d_pix, const uint16_t *d_cnt,
+                  const uint64_t *d_frame_off, uint32_t nframes, uint64_t n_total, uint32_t size, float max_q,
+                  float dq, int interp, int self_pair_factor, double *d_hist)
+{
+    IAC_TRY(iac::bind(ctx));
+    if (!d_kmap || !d_frame_off || !d_hist || npix == 0) return iac::fail(IAC_ERR_INVALID_ARG, "null input");
+    if (size == 0 || size > (uint32_t)kMaxBins) return iac::fail(IAC_ERR_LIMIT, "AC1D size must be 1..6144 bins");
+    if (interp != IAC_NEAREST && interp != IAC_LINEAR) return iac::fail(IAC_ERR_INVALID_ARG, "bad interpolation");
+    if (self_pair_factor != 1 && self_pair_factor != 2)
+        return iac::fail(IAC_ERR_INVALID_ARG, "self_pair_factor is 1 or 2");
+    if (nframes == 0 || n_total == 0) return IAC_OK;
+    if (!d_pix || !d_cnt) return iac::fail(IAC_ERR_PRECONDITION, "no photon lists");
+    void *s0 = nullptr, *s1 = nullptr;
+    IAC_TRY(iac::scratch(ctx, 0, n_total * sizeof(float4), &s0));
+    IAC_TRY(iac::scratch(ctx, 1, (size_t)(nframes + 1) * sizeof(uint64_t) + (size_t)size * 8, &s1));
+    auto *rec = static_cast<float4 *>(s0);
+    auto *toff = static_cast<uint64_t *>(s1);
+    auto *hu = reinterpret_cast<unsigned long long *>(static_cast<char *>(s1) + (size_t)(nframes + 1) * 8);
+    const unsigned gb = (unsigned)std::min<uint64_t>(65535, (n_total + 255) / 256);
+    hipLaunchKernelGGL(gather_kernel, dim3(gb), dim3(256), 0, ctx->stream, d_kmap, npix, d_pix, d_cnt, n_total, rec,
+                       ctx->d_counters + 1);
+    hipLaunchKernelGGL(tile_scan_kernel, dim3(1), dim3(1024), 0, ctx->stream, d_frame_off, nframes, toff);
+    IAC_HIP(hipGetLastError());
+    const bool exact = interp == IAC_NEAREST;  // uint16 counts: every weight is an integer-valued float
+    const float sf = (float)self_pair_factor;
+    IAC_TRY(iac::timer_begin(ctx));
+    if (exact) {
+        hipLaunchKernelGGL(zero_u64_kernel, dim3(1), dim3(256), 0, ctx->stream, hu, size);
+        hipLaunchKernelGGL(ac1d_acuw_kernel<true>, dim3(persistent_blocks(ctx)), dim3(kBlock), 0, ctx->stream, rec,
+                           d_frame_off, toff, nframes, size, max_q, dq, interp, sf, hu, d_hist, ctx->d_counters);
+    } else {
+        hipLaunchKernelGGL(ac1d_acuw_kernel<false>, dim3(persistent_blocks(ctx)), dim3(kBlock), 0, ctx->stream, rec,
+                           d_frame_off, toff, nframes, size, max_q, dq, interp, sf, hu, d_hist, ctx->d_counters);
+    }
+    IAC_HIP(hipGetLastError());
+    IAC_TRY(iac::timer_end(ctx));
+    if (exact) {
+        hipLaunchKernelGGL(add_u64_as_f64_kernel, dim3(std::max(1u, (size + 255) / 256)), dim3(256), 0, ctx->stream,
+                           hu, size, d_hist);
+        IAC_HIP(hipGetLastError());
+    }
+    return IAC_OK;
+}
+
+int iac_ac1d_calc_ac(iac_ctx *ctx, const double *d_ac_uw, const double *d_cq, uint32_t size, double *d_ac)
+{
+    IAC_TRY(iac::bind(ctx));
+    if (!d_ac_uw || !d_cq || !d_ac) return iac::fail(IAC_ERR_INVALID_ARG, "null pointer");
+    if (size == 0) return IAC_OK;
+    hipLaunchKernelGGL(calc_ac_kernel, dim3(std::max(1u, (size + 255) / 256)), dim3(256), 0, ctx->stream, d_ac_uw,
+                       d_cq, size, d_ac);
+    IAC_HIP(hipGetLastError());
+    return IAC_OK;
+}
+
+}  // extern "C"

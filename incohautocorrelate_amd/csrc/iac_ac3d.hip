@@ -13,8 +13,11 @@
 #include <algorithm>
 #include <cmath>
 
+#include <cstring>
+
 #include "iac_binning.h"
 #include "iac_internal.h"
+#include "iac_pairs.h"
 
 #pragma clang fp contract(off)
 
@@ -253,6 +256,46 @@ int launch_tiles(iac_ctx *ctx, const float4 *rec, const uint64_t *frame_off, con
     return IAC_OK;
 }
 
+// Frames of records through the LDS cell-box engine (one shared rotation, here
+// the identity) or, for per-frame rotations, the tile kernel.
+int batch_records(iac_ctx *ctx, const float *d_kmap, uint32_t npix, const float4 *rec,
+                  const uint64_t *d_frame_off, uint32_t nframes, uint64_t n_total, const float *d_rots,
+                  const iac::BinParams &P, uint64_t *d_mesh)
+{
+    const bool engine_ok = d_rots == nullptr && n_total < (1ull << 32);
+    if (ctx->ac3d_kernel == 2 && !engine_ok)
+        return iac::fail(IAC_ERR_PRECONDITION,
+                         "the LDS engine needs one rotation shared by all frames and < 2^32 photons");
+    if (engine_ok && ctx->ac3d_kernel != 1) {
+        iac::PairRun R;
+        std::memset(&R.spec, 0, sizeof(R.spec));
+        R.spec.kind = iac::kKindAc3d;
+        for (int i = 0; i < 9; i++) R.spec.R[i] = (i % 4) == 0 ? 1.0f : 0.0f;
+        R.spec.P = P;
+        R.spec.P.rot_identity = 1;
+        R.ordered = !symmetric_mode(P);
+        R.d_kmap = d_kmap;
+        R.npix = npix;
+        R.rec = rec;
+        R.d_frame_off = d_frame_off;
+        R.nframes = nframes;
+        R.n_total = n_total;
+        const uint32_t goff[2] = {0, nframes};
+        R.h_group_off = goff;
+        R.ngroups = 1;
+        R.d_mesh = d_mesh;
+        return iac::pairs_run(ctx, R);
+    }
+    void *toff = nullptr;
+    IAC_TRY(iac::scratch(ctx, 1, (size_t)(nframes + 1) * sizeof(uint64_t), &toff));
+    hipLaunchKernelGGL(tile_scan_kernel, dim3(1), dim3(1024), 0, ctx->stream, d_frame_off, nframes,
+                       static_cast<uint64_t *>(toff));
+    IAC_HIP(hipGetLastError());
+    const Rot9 ident{{1, 0, 0, 0, 1, 0, 0, 0, 1}};
+    return launch_tiles(ctx, rec, d_frame_off, static_cast<const uint64_t *>(toff), nframes, 0, d_rots, ident, P,
+                        d_mesh);
+}
+
 }  // namespace
 
 extern "C" {
@@ -285,28 +328,30 @@ int iac_ac3d_sparse_batch(iac_ctx *ctx, const float *d_kmap, uint32_t npix, cons
     IAC_TRY(make_params(shape, interp1, interp2, double_map, &P));
     if (nframes == 0 || n_total == 0) return IAC_OK;
     if (!d_pix || !d_cnt) return iac::fail(IAC_ERR_PRECONDITION, "no photon lists");
-    void *rec = nullptr, *toff = nullptr;
+    void *rec = nullptr;
     IAC_TRY(iac::scratch(ctx, 0, n_total * sizeof(float4), &rec));
-    IAC_TRY(iac::scratch(ctx, 1, (size_t)(nframes + 1) * sizeof(uint64_t), &toff));
     const unsigned gblocks = (unsigned)std::min<uint64_t>((n_total + 255) / 256, 65535);
     hipLaunchKernelGGL(gather_records_kernel, dim3(gblocks), dim3(256), 0, ctx->stream, d_kmap, npix,
                        d_pix, d_cnt, n_total, static_cast<float4 *>(rec), ctx->d_counters + 1);
     IAC_HIP(hipGetLastError());
-    // the LDS cell-box pipeline covers nearest-neighbour second-level binning of
-    // frames that share the identity rotation; everything else takes the tile kernel
-    const bool lds_ok = symmetric_mode(P) && d_rots == nullptr && n_total < (1ull << 32);
-    if (ctx->ac3d_kernel == 2 && !lds_ok)
-        return iac::fail(IAC_ERR_PRECONDITION,
-                         "the LDS kernel needs nearest-neighbour binning, identity rotation and < 2^32 photons");
-    if (lds_ok && ctx->ac3d_kernel != 1)
-        return iac::ac3d_batch_lds(ctx, d_kmap, npix, static_cast<const float4 *>(rec), d_frame_off, nframes,
-                                   n_total, shape, P, d_mesh, ctx->cell_vox, ctx->frame_chunk);
-    hipLaunchKernelGGL(tile_scan_kernel, dim3(1), dim3(1024), 0, ctx->stream, d_frame_off, nframes,
-                       static_cast<uint64_t *>(toff));
-    IAC_HIP(hipGetLastError());
-    const Rot9 ident{{1, 0, 0, 0, 1, 0, 0, 0, 1}};
-    return launch_tiles(ctx, static_cast<const float4 *>(rec), d_frame_off,
-                        static_cast<const uint64_t *>(toff), nframes, 0, d_rots, ident, P, d_mesh);
+    return batch_records(ctx, d_kmap, npix, static_cast<const float4 *>(rec), d_frame_off, nframes, n_total,
+                         d_rots, P, d_mesh);
+}
+
+int iac_ac3d_records_batch(iac_ctx *ctx, const float *d_kmap, uint32_t npix, const float *d_rec4,
+                           const uint64_t *d_frame_off, uint32_t nframes, uint64_t n_total,
+                           const float *d_rots, const iac_mesh_shape *shape, int interp1, int interp2,
+                           int double_map, uint64_t *d_mesh)
+{
+    IAC_TRY(iac::bind(ctx));
+    if (!d_mesh || !d_frame_off || !d_kmap || npix == 0)
+        return iac::fail(IAC_ERR_INVALID_ARG, "null mesh, offsets or k-map");
+    iac::BinParams P;
+    IAC_TRY(make_params(shape, interp1, interp2, double_map, &P));
+    if (nframes == 0 || n_total == 0) return IAC_OK;
+    if (!d_rec4) return iac::fail(IAC_ERR_PRECONDITION, "no hit-list records");
+    return batch_records(ctx, d_kmap, npix, reinterpret_cast<const float4 *>(d_rec4), d_frame_off, nframes,
+                         n_total, d_rots, P, d_mesh);
 }
 
 }  // extern "C"

@@ -159,7 +159,7 @@ int iac_ctx_destroy(iac_ctx *ctx)
     if (ctx->stream) (void)hipStreamSynchronize(ctx->stream);
     for (int i = 0; i < 4; i++)
         if (ctx->scratch[i]) (void)hipFree(ctx->scratch[i]);
-    iac::free_ac3d_plan(ctx);
+    iac::free_pair_plans(ctx);
     for (auto e : ctx->ev_start) (void)hipEventDestroy(e);
     for (auto e : ctx->ev_stop) (void)hipEventDestroy(e);
     if (ctx->d_counters) (void)hipFree(ctx->d_counters);

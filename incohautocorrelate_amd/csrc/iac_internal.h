@@ -10,7 +10,7 @@
 
 #include "../../include/iac_hip.h"
 
-struct iac_ac3d_plan;  // cell grid and task table of the LDS pair kernel
+struct iac_pair_plan;  // cell grid and task table of the LDS pair engine
 
 struct iac_ctx {
     int device = 0;
@@ -26,7 +26,7 @@ struct iac_ctx {
     bool timing = false;
     std::vector<hipEvent_t> ev_start, ev_stop;
     size_t ev_used = 0;
-    iac_ac3d_plan *ac3d_plan = nullptr;
+    std::vector<iac_pair_plan *> pair_plans;  // small cache, oldest first
     // options (IAC_OPT_*)
     int ac3d_kernel = 0;
     int cell_vox = 0;
@@ -49,14 +49,7 @@ int scratch(iac_ctx *ctx, int slot, size_t bytes, void **out);
 int timer_begin(iac_ctx *ctx);
 int timer_end(iac_ctx *ctx);
 
-void free_ac3d_plan(iac_ctx *ctx);
-
-struct BinParams;
-// LDS cell-box pipeline for nearest-neighbour binning with identity rotation.
-int ac3d_batch_lds(iac_ctx *ctx, const float *d_kmap, uint32_t npix, const float4 *rec,
-                   const uint64_t *d_frame_off, uint32_t nframes, uint64_t n_total,
-                   const iac_mesh_shape *shape, const BinParams &P, uint64_t *d_mesh, int cell_vox,
-                   uint32_t fchunk);
+void free_pair_plans(iac_ctx *ctx);
 
 // Smallest float s >= 0 with sqrtf(s) > max_q (correctly rounded sqrtf).
 // `sqrtf(s) > max_q` is then exactly `s >= cut_sq(max_q)` for s >= 0.

@@ -1,0 +1,42 @@
+// iac_pairs.h -- interface of the LDS cell-box pair engine (iac_pairs.hip).
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include "iac_binning.h"
+
+struct iac_ctx;
+
+namespace iac {
+
+enum { kKindAc3d = 0, kKindCq = 1 };
+
+// What one pair contributes.  Trivially copyable; zero it before filling (it
+// is compared bytewise as a plan-cache key).
+struct PairSpec {
+    int kind;        // kKindAc3d: ACMesh binning, kKindCq: AutoCorr_CQ_small binning
+    int ka, kb, kc;  // C(q): k components feeding fs, ms, ss (MeshShape k_A, k_B, k_C)
+    int interp;      // C(q): interpolation (0 nearest, 1 linear)
+    int reserved;
+    double mult;     // C(q): fixed-point multiplicator, value (uint64)((double)(Ia*Ib) * mult)
+    float R[9];      // AC_uw: rotation shared by all frames (row major)
+    BinParams P;     // cut, voxel scale, centres, sizes, AC_uw interpolation and double map
+};
+
+struct PairRun {
+    PairSpec spec;
+    bool ordered;                 // linear modes: every ordered pair, no mirror pass
+    const float *d_kmap;          // [npix][3], defines the cell grid
+    uint32_t npix;
+    const float4 *rec;            // records of all frames
+    const uint64_t *d_frame_off;  // [nframes + 1]
+    uint32_t nframes;
+    uint64_t n_total;
+    const uint32_t *h_group_off;  // host [ngroups + 1]: frames of group g are [off[g], off[g+1])
+    uint32_t ngroups;
+    uint64_t *d_mesh;             // ngroups meshes of size_ab^2 * size_c uint64, accumulated into
+};
+
+int pairs_run(iac_ctx *ctx, const PairRun &run);
+
+}  // namespace iac

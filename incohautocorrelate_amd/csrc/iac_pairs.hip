@@ -1,0 +1,816 @@
+// iac_pairs.hip -- the LDS cell-box pair engine shared by the 3D AC_uw path
+// (Detector::AutoCorrelateSparseList) and the dense C(q) path
+// (AutoCorr_CQ_small, IAC/kernel.cl:27-184).
+//
+// Input: photon or pixel records {kx, ky, kz, value} of nframes frames; frames
+// form groups (frame range -> one output mesh each).  One call:
+//   1. plan (cached): the k-map's extents define a cell grid over its two
+//      widest k axes (cell side `cell_vox` voxels); per-cell k extents bound
+//      the voxel box reached by the pairs of every cell offset delta.
+//   2. cell_sort: per frame, counting sort of its records by cell, dropping
+//      zero-valued records (they add nothing in any mode); in symmetric mode
+//      the frame's self-pair value is summed for its group.
+//   3. pairs: one block per (delta, chunk of frames of one group).  The block
+//      privatises delta's voxel box in LDS across the chunk and bins
+//        symmetric modes: each unordered pair of distinct records once
+//          (deltas of the positive half plane; within-cell pairs for 0),
+//        ordered modes: every ordered pair (all deltas; self pairs in 0),
+//      flushing the box with one atomic per nonzero voxel.  A pair outside
+//      the box (the box is only a bound) adds to global memory directly.
+//   4. symmetric modes: mesh[v] += H[v] + H[mirror v], centre += self.
+// Nearest-neighbour binning is odd in q (iac_binning.h): the (b,a) pair lands
+// on the point reflection of (a,b) with the same value, so step 4 restores
+// the ordered-pair mesh exactly.  Linear modes are not odd and run ordered.
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+#include <vector>
+
+#include "iac_binning.h"
+#include "iac_internal.h"
+#include "iac_pairs.h"
+
+#pragma clang fp contract(off)
+
+namespace iac {
+namespace {
+
+constexpr int kBlock = 256;
+constexpr int kItemsPerThread = 4;
+constexpr int kGroup = kBlock * kItemsPerThread;  // items per scan group
+constexpr int kBoxMax = 4096;                      // LDS box capacity (uint64 voxels, 32 KiB)
+constexpr int kMaxCells = 4096;                    // NC_u * NC_v
+
+struct CellGeom {
+    int au, av;    // k axes spanned by the cell grid
+    float u0, v0;  // grid origin
+    float inv_w;   // 1 / cell width (k units)
+    int ncu, ncv;  // cells per axis
+};
+
+struct Task {
+    int du, dv;         // cell offset delta (u, v)
+    int iu0, iv0;       // first cell i whose partner i - delta is on the grid
+    int nu, nv;         // extent of that cell range
+    int bx0, by0, bz0;  // LDS box origin (voxel indices)
+    int bx, by, bz;     // LDS box extent; bx == 0 -> no box, every add goes to global
+};
+
+struct Chunk {
+    uint32_t f0, f1, group;
+};
+
+__device__ __forceinline__ uint32_t fkey(float f)
+{
+    const uint32_t b = __float_as_uint(f);
+    return (b & 0x80000000u) ? ~b : (b | 0x80000000u);
+}
+
+inline float unkey(uint32_t k)
+{
+    uint32_t b = (k & 0x80000000u) ? (k & 0x7fffffffu) : ~k;
+    float f;
+    std::memcpy(&f, &b, 4);
+    return f;
+}
+
+__device__ __forceinline__ int cell_of(const CellGeom &g, float ku, float kv)
+{
+    int cu = (int)floorf((ku - g.u0) * g.inv_w);
+    int cv = (int)floorf((kv - g.v0) * g.inv_w);
+    cu = min(max(cu, 0), g.ncu - 1);
+    cv = min(max(cv, 0), g.ncv - 1);
+    return cv * g.ncu + cu;
+}
+
+__device__ __forceinline__ float comp(const float4 &r, int a) { return a == 0 ? r.x : (a == 1 ? r.y : r.z); }
+
+// ---- binning policies -------------------------------------------------------
+
+// C(q) fixed point of AutoCorr_CQ_small (kernel.cl:93-101): (ulong)((double)(Ia*Ib) * M),
+// pairs below 1e-37f skipped.
+__device__ __forceinline__ bool cq_value(const PairSpec &S, float p, double &val)
+{
+    val = (double)p;
+    return !(val < (double)1e-37f);
+}
+
+// Symmetric (nearest-neighbour) binning of one pair: voxel + value.
+template <int KIND>
+__device__ __forceinline__ bool bin_sym(const PairSpec &S, const float4 &a, const float4 &b, int &fs,
+                                        int &ms, int &ss, uint64_t &v)
+{
+    if (KIND == kKindAc3d) {
+        int dx, dy, dz;
+        if (!bin_nn(a.x, a.y, a.z, b.x, b.y, b.z, S.P, S.R, dx, dy, dz)) return false;
+        v = pair_val(a.w * b.w);
+        fs = dx + S.P.c0;
+        ms = dy + S.P.c1;
+        ss = dz + S.P.c2;
+        return true;
+    } else {
+        double val;
+        if (!cq_value(S, a.w * b.w, val)) return false;
+        float q[3] = {a.x - b.x, a.y - b.y, a.z - b.z};
+        const float s = (q[0] * q[0] + q[1] * q[1]) + q[2] * q[2];
+        if (!(s < S.P.cut_sq)) return false;
+        q[0] = q[0] * S.P.vox;
+        q[1] = q[1] * S.P.vox;
+        q[2] = q[2] * S.P.vox;
+        fs = (int)(roundf(q[S.ka]) + (float)S.P.c0);
+        ms = (int)(roundf(q[S.kb]) + (float)S.P.c1);
+        ss = (int)(roundf(q[S.kc]) + (float)S.P.c2);
+        v = (uint64_t)(val * S.mult);
+        return true;
+    }
+}
+
+template <int KIND>
+__device__ __forceinline__ uint64_t self_value(const PairSpec &S, float c)
+{
+    if (KIND == kKindAc3d) return pair_val(c * c);
+    double val;
+    return cq_value(S, c * c, val) ? (uint64_t)(val * S.mult) : 0ull;
+}
+
+// Ordered (linear-mode) binning of one pair into a sink.
+template <int KIND, class Sink>
+__device__ __forceinline__ void bin_ordered(const PairSpec &S, const float4 &a, const float4 &b, Sink &sink)
+{
+    if (KIND == kKindAc3d) {
+        add_q_entry(sink, a.x, a.y, a.z, b.x, b.y, b.z, a.w * b.w, S.P, S.R);
+    } else {
+        double val;
+        if (!cq_value(S, a.w * b.w, val)) return;
+        float q[3] = {a.x - b.x, a.y - b.y, a.z - b.z};
+        const float s = (q[0] * q[0] + q[1] * q[1]) + q[2] * q[2];
+        if (!(s < S.P.cut_sq)) return;
+        q[0] = q[0] * S.P.vox;
+        q[1] = q[1] * S.P.vox;
+        q[2] = q[2] * S.P.vox;
+        if (S.interp == 0) {
+            sink.add((int)(roundf(q[S.ka]) + (float)S.P.c0), (int)(roundf(q[S.kb]) + (float)S.P.c1),
+                     (int)(roundf(q[S.kc]) + (float)S.P.c2), (uint64_t)(val * S.mult));
+            return;
+        }
+        // kernel.cl:136-181: `1 - (q - floor(q))` in float, weights in double
+        const float qa = q[S.ka], qb = q[S.kb], qc = q[S.kc];
+        const double sfs = (double)(1.0f - (qa - floorf(qa)));
+        const double sms = (double)(1.0f - (qb - floorf(qb)));
+        const double sss = (double)(1.0f - (qc - floorf(qc)));
+        const int fs = (int)(floorf(qa) + (float)S.P.c0);
+        const int ms = (int)(floorf(qb) + (float)S.P.c1);
+        const int ss = (int)(floorf(qc) + (float)S.P.c2);
+        const double vm = val * S.mult;
+        sink.add(fs + 0, ms + 0, ss + 0, (uint64_t)(vm * ((sfs * sms) * sss)));
+        sink.add(fs + 0, ms + 0, ss + 1, (uint64_t)(vm * ((sfs * sms) * (1 - sss))));
+        sink.add(fs + 1, ms + 0, ss + 0, (uint64_t)(vm * (((1 - sfs) * sms) * sss)));
+        sink.add(fs + 1, ms + 0, ss + 1, (uint64_t)(vm * (((1 - sfs) * sms) * (1 - sss))));
+        sink.add(fs + 0, ms + 1, ss + 0, (uint64_t)(vm * ((sfs * (1 - sms)) * sss)));
+        sink.add(fs + 0, ms + 1, ss + 1, (uint64_t)(vm * ((sfs * (1 - sms)) * (1 - sss))));
+        sink.add(fs + 1, ms + 1, ss + 0, (uint64_t)(vm * (((1 - sfs) * (1 - sms)) * sss)));
+        sink.add(fs + 1, ms + 1, ss + 1, (uint64_t)(vm * (((1 - sfs) * (1 - sms)) * (1 - sss))));
+    }
+}
+
+// ---- small kernels -------------------------------------------------------------
+
+__global__ void fill_u32_kernel(uint32_t *p, uint32_t v, size_t n)
+{
+    for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x)
+        p[i] = v;
+}
+
+__global__ void fill_u64_kernel(unsigned long long *p, size_t n)
+{
+    for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x)
+        p[i] = 0ull;
+}
+
+// per-cell extents start empty: min keys at the top, max keys at the bottom
+__global__ void init_cell_ext_kernel(uint32_t *ext, int ncell)
+{
+    const int c = blockIdx.x * blockDim.x + threadIdx.x;
+    if (c < ncell)
+        for (int d = 0; d < 3; d++) {
+            ext[6 * c + d] = 0xffffffffu;
+            ext[6 * c + 3 + d] = 0u;
+        }
+}
+
+// min/max of each k component: ext[0..2] min keys, ext[3..5] max keys
+__global__ void kext_kernel(const float *__restrict__ kmap, uint32_t npix, uint32_t *ext)
+{
+    uint32_t mn[3] = {0xffffffffu, 0xffffffffu, 0xffffffffu}, mx[3] = {0u, 0u, 0u};
+    for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < npix; i += gridDim.x * blockDim.x)
+        for (int d = 0; d < 3; d++) {
+            const uint32_t k = fkey(kmap[3 * (size_t)i + d]);
+            mn[d] = min(mn[d], k);
+            mx[d] = max(mx[d], k);
+        }
+    for (int d = 0; d < 3; d++) {
+        atomicMin(ext + d, mn[d]);
+        atomicMax(ext + 3 + d, mx[d]);
+    }
+}
+
+// per-cell k extents: ext[6c + d] min key of component d, ext[6c + 3 + d] max key
+__global__ void cell_ext_kernel(const float *__restrict__ kmap, uint32_t npix, CellGeom g, uint32_t *ext)
+{
+    for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < npix; i += gridDim.x * blockDim.x) {
+        const float k[3] = {kmap[3 * (size_t)i], kmap[3 * (size_t)i + 1], kmap[3 * (size_t)i + 2]};
+        const int c = cell_of(g, k[g.au], k[g.av]);
+        for (int d = 0; d < 3; d++) {
+            atomicMin(ext + 6 * c + d, fkey(k[d]));
+            atomicMax(ext + 6 * c + 3 + d, fkey(k[d]));
+        }
+    }
+}
+
+// One block per frame: counting sort of the frame's nonzero records by cell.
+// cs[f*(ncell+1) + c] = first sorted position of cell c relative to off[f];
+// cs[f*(ncell+1) + ncell] = number of kept records.  In symmetric mode the
+// frame's self-pair values are added to self[group of f].
+template <int KIND, bool ORD>
+__global__ void __launch_bounds__(kBlock)
+    cell_sort_kernel(const float4 *__restrict__ rec, const uint64_t *__restrict__ off, CellGeom g,
+                     float4 *__restrict__ srec, uint32_t *__restrict__ cs, PairSpec S,
+                     const uint32_t *__restrict__ group_off, uint32_t ngroups,
+                     unsigned long long *__restrict__ self)
+{
+    __shared__ uint32_t cnt[kMaxCells];
+    __shared__ uint32_t part[kBlock];
+    __shared__ unsigned long long sred[kBlock / 64];
+    const uint32_t f = blockIdx.x;
+    const int ncell = g.ncu * g.ncv;
+    const uint64_t base = off[f];
+    const uint32_t n = (uint32_t)(off[f + 1] - base);
+    for (int c = threadIdx.x; c < ncell; c += kBlock) cnt[c] = 0;
+    __syncthreads();
+    unsigned long long sv = 0;
+    for (uint32_t i = threadIdx.x; i < n; i += kBlock) {
+        const float4 r = rec[base + i];
+        if (r.w == 0.0f) continue;
+        atomicAdd(&cnt[cell_of(g, comp(r, g.au), comp(r, g.av))], 1u);
+        if (!ORD) sv += self_value<KIND>(S, r.w);
+    }
+    if (!ORD) {
+        for (int o = 32; o > 0; o >>= 1) sv += __shfl_down(sv, o, 64);
+        if ((threadIdx.x & 63) == 0) sred[threadIdx.x >> 6] = sv;
+    }
+    __syncthreads();
+    if (!ORD && threadIdx.x == 0) {
+        unsigned long long t = 0;
+        for (int w = 0; w < kBlock / 64; w++) t += sred[w];
+        if (t) {
+            uint32_t lo = 0, hi = ngroups;  // group of f: largest gi with group_off[gi] <= f
+            while (hi - lo > 1) {
+                const uint32_t mid = (lo + hi) >> 1;
+                if (group_off[mid] <= f) lo = mid; else hi = mid;
+            }
+            atomicAdd(self + lo, t);
+        }
+    }
+    // exclusive scan of cnt[0..ncell)
+    const int per = (ncell + kBlock - 1) / kBlock;
+    const int c0 = min(ncell, (int)threadIdx.x * per), c1 = min(ncell, c0 + per);
+    uint32_t s = 0;
+    for (int c = c0; c < c1; c++) s += cnt[c];
+    part[threadIdx.x] = s;
+    __syncthreads();
+    for (int o = 1; o < kBlock; o <<= 1) {
+        const uint32_t v = threadIdx.x >= (unsigned)o ? part[threadIdx.x - o] : 0u;
+        __syncthreads();
+        part[threadIdx.x] += v;
+        __syncthreads();
+    }
+    uint32_t run = threadIdx.x ? part[threadIdx.x - 1] : 0u;
+    uint32_t *csf = cs + (size_t)f * (ncell + 1);
+    for (int c = c0; c < c1; c++) {
+        const uint32_t v = cnt[c];
+        csf[c] = run;
+        cnt[c] = run;  // becomes the scatter cursor
+        run += v;
+    }
+    if (threadIdx.x == kBlock - 1) csf[ncell] = part[kBlock - 1];
+    __syncthreads();
+    for (uint32_t i = threadIdx.x; i < n; i += kBlock) {
+        const float4 r = rec[base + i];
+        if (r.w == 0.0f) continue;
+        const uint32_t pos = atomicAdd(&cnt[cell_of(g, comp(r, g.au), comp(r, g.av))], 1u);
+        srec[base + pos] = r;
+    }
+}
+
+// mesh[g][v] += H[g][v] + H[g][last - v]; the centre also receives self[g].
+__global__ void symmetrize_kernel(const unsigned long long *__restrict__ H, unsigned long long *__restrict__ mesh,
+                                  size_t nvox, uint32_t ngroups, size_t centre,
+                                  const unsigned long long *__restrict__ self)
+{
+    const size_t last = nvox - 1, total = nvox * ngroups;
+    for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < total; i += (size_t)gridDim.x * blockDim.x) {
+        const size_t gi = i / nvox, v = i - gi * nvox;
+        const unsigned long long *h = H + gi * nvox;
+        unsigned long long add = h[v] + h[last - v];
+        if (v == centre) add += self[gi];
+        if (add) mesh[i] += add;
+    }
+}
+
+struct BoxSink {
+    unsigned long long *box;  // LDS
+    unsigned long long *out;  // global mesh (group's H or mesh)
+    unsigned long long *dropped;
+    int bx0, by0, bz0, bx, by, bz;
+    int S, SC;
+    unsigned long long drop_weight;
+    __device__ __forceinline__ void add(int fs, int ms, int ss, uint64_t v)
+    {
+        if (!v) return;
+        const unsigned lx = (unsigned)(fs - bx0), ly = (unsigned)(ms - by0), lz = (unsigned)(ss - bz0);
+        if (lx < (unsigned)bx && ly < (unsigned)by && lz < (unsigned)bz) {
+            atomicAdd(box + lx + (size_t)bx * (ly + (size_t)by * lz), (unsigned long long)v);
+        } else if ((unsigned)fs < (unsigned)S && (unsigned)ms < (unsigned)S && (unsigned)ss < (unsigned)SC) {
+            atomicAdd(out + (size_t)fs + (size_t)ms * S + (size_t)ss * S * S, (unsigned long long)v);
+        } else {
+            atomicAdd(dropped, drop_weight);
+        }
+    }
+};
+
+template <int KIND, bool ORD>
+__global__ void __launch_bounds__(kBlock)
+    pairs_kernel(const float4 *__restrict__ srec, const uint64_t *__restrict__ off,
+                 const uint32_t *__restrict__ cs, const Chunk *__restrict__ chunks,
+                 const Task *__restrict__ tasks, uint32_t ntasks, CellGeom g, PairSpec S,
+                 unsigned long long *__restrict__ target, size_t nvox,
+                 unsigned long long *__restrict__ dropped)
+{
+    __shared__ unsigned long long box[kBoxMax];
+    __shared__ uint4 item[kGroup];        // {a base, na, b base, nb}
+    __shared__ uint32_t pre[kGroup + 1];  // exclusive prefix of pair counts
+    __shared__ uint32_t wsum[kBlock];
+
+    const Task T = tasks[blockIdx.x % ntasks];
+    const Chunk CH = chunks[blockIdx.x / ntasks];
+    const int ncell = g.ncu * g.ncv;
+    const bool zero_delta = (T.du == 0 && T.dv == 0);
+    const bool tri = !ORD && zero_delta;  // distinct unordered pairs within a cell
+
+    BoxSink sink{box, target + (size_t)CH.group * nvox, dropped, T.bx0, T.by0, T.bz0, T.bx, T.by, T.bz,
+                 S.P.sab, S.P.sc, ORD ? 1ull : 2ull};
+    const int vbox = T.bx * T.by * T.bz;
+    for (int v = threadIdx.x; v < vbox; v += kBlock) box[v] = 0ull;
+    __syncthreads();
+
+    const uint32_t cells = (uint32_t)T.nu * (uint32_t)T.nv;
+    const uint64_t nitems = (uint64_t)(CH.f1 - CH.f0) * cells;
+    for (uint64_t ub = 0; ub < nitems; ub += kGroup) {
+        // phase 1: item descriptors and pair counts (4 contiguous items per thread)
+        uint32_t mycnt[kItemsPerThread];
+        uint32_t tsum = 0;
+#pragma unroll
+        for (int k = 0; k < kItemsPerThread; k++) {
+            const uint32_t li = threadIdx.x * kItemsPerThread + k;
+            const uint64_t u = ub + li;
+            uint4 d = make_uint4(0, 0, 0, 0);
+            uint32_t c = 0;
+            if (u < nitems) {
+                const uint32_t f = CH.f0 + (uint32_t)(u / cells);
+                const uint32_t rem = (uint32_t)(u % cells);
+                const int iv = T.iv0 + (int)(rem / T.nu), iu = T.iu0 + (int)(rem % T.nu);
+                const int ci = iv * g.ncu + iu;
+                const int cj = (iv - T.dv) * g.ncu + (iu - T.du);
+                const uint32_t *csf = cs + (size_t)f * (ncell + 1);
+                const uint32_t base = (uint32_t)off[f];
+                const uint32_t a0 = csf[ci], na = csf[ci + 1] - a0;
+                const uint32_t b0 = csf[cj], nb = csf[cj + 1] - b0;
+                d = make_uint4(base + a0, na, base + b0, nb);
+                c = tri ? na * (na - 1) / 2 : na * nb;
+            }
+            item[li] = d;
+            mycnt[k] = c;
+            tsum += c;
+        }
+        wsum[threadIdx.x] = tsum;
+        __syncthreads();
+        for (int o = 1; o < kBlock; o <<= 1) {
+            const uint32_t v = threadIdx.x >= (unsigned)o ? wsum[threadIdx.x - o] : 0u;
+            __syncthreads();
+            wsum[threadIdx.x] += v;
+            __syncthreads();
+        }
+        {
+            uint32_t run = threadIdx.x ? wsum[threadIdx.x - 1] : 0u;
+#pragma unroll
+            for (int k = 0; k < kItemsPerThread; k++) {
+                pre[threadIdx.x * kItemsPerThread + k] = run;
+                run += mycnt[k];
+            }
+            if (threadIdx.x == kBlock - 1) pre[kGroup] = run;
+        }
+        __syncthreads();
+
+        // phase 2: thread t bins the contiguous pair range [p0, p1)
+        const uint32_t total = pre[kGroup];
+        const uint32_t p0 = (uint32_t)(((uint64_t)total * threadIdx.x) / kBlock);
+        const uint32_t p1 = (uint32_t)(((uint64_t)total * (threadIdx.x + 1)) / kBlock);
+        if (p0 < p1) {
+            int lo = 0, hi = kGroup;  // the item holding p0: largest u with pre[u] <= p0
+            while (hi - lo > 1) {
+                const int mid = (lo + hi) >> 1;
+                if (pre[mid] <= p0) lo = mid; else hi = mid;
+            }
+            int u = lo;
+            uint4 d = item[u];
+            uint32_t r = p0 - pre[u];
+            uint32_t ia, ib;
+            if (tri) {
+                ia = 0;
+                while (r >= d.y - 1 - ia) {
+                    r -= d.y - 1 - ia;
+                    ia++;
+                }
+                ib = ia + 1 + r;
+            } else {
+                ia = r / d.w;
+                ib = r % d.w;
+            }
+            float4 ra = srec[d.x + ia];
+            for (uint32_t p = p0;;) {
+                const float4 rb = srec[d.z + ib];
+                if (ORD) {
+                    bin_ordered<KIND>(S, ra, rb, sink);
+                } else {
+                    int fs, ms, ss;
+                    uint64_t v;
+                    if (bin_sym<KIND>(S, ra, rb, fs, ms, ss, v)) sink.add(fs, ms, ss, v);
+                }
+                if (++p == p1) break;
+                // next pair; one exists because p < p1 <= total
+                ib++;
+                if (ib >= (tri ? d.y : d.w)) {
+                    ia++;
+                    ib = tri ? ia + 1 : 0;
+                    if ((tri ? ib : ia) >= d.y) {  // item exhausted: next item with pairs
+                        do {
+                            u++;
+                        } while (pre[u + 1] == pre[u]);
+                        d = item[u];
+                        ia = 0;
+                        ib = tri ? 1 : 0;
+                    }
+                    ra = srec[d.x + ia];
+                }
+            }
+        }
+        __syncthreads();
+    }
+
+    // flush the LDS box
+    unsigned long long *out = target + (size_t)CH.group * nvox;
+    for (int v = threadIdx.x; v < vbox; v += kBlock) {
+        const unsigned long long x = box[v];
+        if (x) {
+            const int lx = v % T.bx, rest = v / T.bx, ly = rest % T.by, lz = rest / T.by;
+            const size_t gi = (size_t)(T.bx0 + lx) + (size_t)(T.by0 + ly) * S.P.sab +
+                              (size_t)(T.bz0 + lz) * S.P.sab * S.P.sab;
+            atomicAdd(out + gi, x);
+        }
+    }
+}
+
+// ---- host planning ---------------------------------------------------------------
+
+// interval of round(x) / of floor(x)+{0,1} for x in [lo, hi]
+inline void nn_range(double &lo, double &hi)
+{
+    lo = std::floor(lo);
+    hi = std::ceil(hi);
+}
+inline void lin_range(double &lo, double &hi)
+{
+    lo = std::floor(lo);
+    hi = std::ceil(hi) + 1;
+}
+
+// Conservative voxel box (per mesh axis, relative to the centre) of the pairs
+// whose q lies in [qlo, qhi].
+void voxel_box(const PairSpec &S, const double qlo[3], const double qhi[3], double lo[3], double hi[3])
+{
+    const double vox = S.P.vox;
+    double l[3], h[3];
+    for (int d = 0; d < 3; d++) {
+        l[d] = qlo[d] * vox;
+        h[d] = qhi[d] * vox;
+    }
+    if (S.kind == kKindAc3d) {
+        if (S.P.double_map) {
+            for (int d = 0; d < 3; d++) (S.P.interp1 ? lin_range : nn_range)(l[d], h[d]);
+        }
+        if (!S.P.rot_identity) {
+            double rl[3], rh[3];
+            for (int i = 0; i < 3; i++) {
+                rl[i] = rh[i] = 0;
+                for (int j = 0; j < 3; j++) {
+                    const double a = S.R[3 * i + j] * l[j], b = S.R[3 * i + j] * h[j];
+                    rl[i] += std::min(a, b);
+                    rh[i] += std::max(a, b);
+                }
+            }
+            for (int d = 0; d < 3; d++) {
+                l[d] = rl[d];
+                h[d] = rh[d];
+            }
+        }
+        for (int d = 0; d < 3; d++) {
+            (S.P.interp2 ? lin_range : nn_range)(l[d], h[d]);
+            lo[d] = l[d];
+            hi[d] = h[d];
+        }
+    } else {
+        const int ax[3] = {S.ka, S.kb, S.kc};
+        for (int m = 0; m < 3; m++) {
+            double a = l[ax[m]], b = h[ax[m]];
+            (S.interp ? lin_range : nn_range)(a, b);
+            lo[m] = a;
+            hi[m] = b;
+        }
+    }
+    for (int d = 0; d < 3; d++) {  // one voxel of slack for float rounding
+        lo[d] -= 1;
+        hi[d] += 1;
+    }
+}
+
+}  // namespace
+}  // namespace iac
+
+struct iac_pair_plan {
+    bool valid = false;
+    // key
+    const float *kmap = nullptr;
+    uint32_t npix = 0;
+    int cell_vox = 0;
+    iac::PairSpec spec{};
+    iac::CellGeom g{};
+    std::vector<uint32_t> cell_ext;  // 6 keys per cell
+    // product
+    std::vector<iac::Task> tasks;
+    iac::Task *d_tasks = nullptr;
+};
+
+namespace iac {
+namespace {
+
+bool same_spec(const PairSpec &a, const PairSpec &b)
+{
+    return std::memcmp(&a, &b, sizeof(PairSpec)) == 0;
+}
+
+void build_tasks(iac_pair_plan &pl, bool ordered)
+{
+    const CellGeom &g = pl.g;
+    const PairSpec &S = pl.spec;
+    const int nc = g.ncu * g.ncv;
+    std::vector<char> has(nc, 0);
+    std::vector<double> kmn(3 * (size_t)nc), kmx(3 * (size_t)nc);
+    for (int c = 0; c < nc; c++) {
+        const uint32_t *e = &pl.cell_ext[6 * (size_t)c];
+        if (e[0] == 0xffffffffu) continue;  // no pixel in this cell
+        has[c] = 1;
+        for (int d = 0; d < 3; d++) {
+            kmn[3 * c + d] = unkey(e[d]);
+            kmx[3 * c + d] = unkey(e[3 + d]);
+        }
+    }
+    const double maxq = std::sqrt((double)S.P.cut_sq);
+    const int cen[3] = {S.P.c0, S.P.c1, S.P.c2};
+    const int dim[3] = {S.P.sab, S.P.sab, S.P.sc};
+    pl.tasks.clear();
+    for (int dv = ordered ? -(g.ncv - 1) : 0; dv < g.ncv; dv++) {
+        for (int du = -(g.ncu - 1); du < g.ncu; du++) {
+            if (!ordered && dv == 0 && du < 0) continue;  // positive half plane
+            Task t{};
+            t.du = du;
+            t.dv = dv;
+            t.iu0 = std::max(0, du);
+            t.iv0 = std::max(0, dv);
+            t.nu = std::min(g.ncu, g.ncu + du) - t.iu0;
+            t.nv = std::min(g.ncv, g.ncv + dv) - t.iv0;
+            if (t.nu <= 0 || t.nv <= 0) continue;
+            double qlo[3] = {1e30, 1e30, 1e30}, qhi[3] = {-1e30, -1e30, -1e30};
+            bool any = false;
+            for (int iv = t.iv0; iv < t.iv0 + t.nv; iv++)
+                for (int iu = t.iu0; iu < t.iu0 + t.nu; iu++) {
+                    const int ci = iv * g.ncu + iu, cj = (iv - dv) * g.ncu + (iu - du);
+                    if (!has[ci] || !has[cj]) continue;
+                    any = true;
+                    for (int d = 0; d < 3; d++) {
+                        qlo[d] = std::min(qlo[d], kmn[3 * ci + d] - kmx[3 * cj + d]);
+                        qhi[d] = std::max(qhi[d], kmx[3 * ci + d] - kmn[3 * cj + d]);
+                    }
+                }
+            if (!any) continue;
+            double dist2 = 0;  // all pairs of the task are cut if the q box lies beyond Max_Q
+            for (int d = 0; d < 3; d++) {
+                const double m = qlo[d] > 0 ? qlo[d] : (qhi[d] < 0 ? -qhi[d] : 0.0);
+                dist2 += m * m;
+            }
+            if (std::sqrt(dist2) > maxq * (1.0 + 1e-5) + 1e-7) continue;
+            double lo[3], hi[3];
+            voxel_box(S, qlo, qhi, lo, hi);
+            int ilo[3], ihi[3];
+            for (int d = 0; d < 3; d++) {
+                ilo[d] = (int)std::max(0.0, lo[d] + cen[d]);
+                ihi[d] = (int)std::min((double)dim[d] - 1, hi[d] + cen[d]);
+            }
+            const long vol = (long)(ihi[0] - ilo[0] + 1) * (ihi[1] - ilo[1] + 1) * (ihi[2] - ilo[2] + 1);
+            if (ihi[0] >= ilo[0] && ihi[1] >= ilo[1] && ihi[2] >= ilo[2] && vol <= kBoxMax) {
+                t.bx0 = ilo[0];
+                t.by0 = ilo[1];
+                t.bz0 = ilo[2];
+                t.bx = ihi[0] - ilo[0] + 1;
+                t.by = ihi[1] - ilo[1] + 1;
+                t.bz = ihi[2] - ilo[2] + 1;
+            }
+            pl.tasks.push_back(t);
+        }
+    }
+}
+
+iac_pair_plan *find_plan(iac_ctx *ctx, const float *kmap, uint32_t npix, int cell_vox, const PairSpec &S)
+{
+    for (auto *p : ctx->pair_plans)
+        if (p->valid && p->kmap == kmap && p->npix == npix && p->cell_vox == cell_vox && same_spec(p->spec, S))
+            return p;
+    return nullptr;
+}
+
+}  // namespace
+
+void free_pair_plans(iac_ctx *ctx)
+{
+    for (auto *p : ctx->pair_plans) {
+        if (p->d_tasks) (void)hipFree(p->d_tasks);
+        delete p;
+    }
+    ctx->pair_plans.clear();
+}
+
+int pairs_run(iac_ctx *ctx, const PairRun &R)
+{
+    hipStream_t st = ctx->stream;
+    const PairSpec &S = R.spec;
+    const bool ordered = R.ordered;
+    const int cell_vox = ctx->cell_vox > 0 ? ctx->cell_vox : (S.kind == kKindAc3d ? 6 : 4);
+    const uint32_t fchunk = ctx->frame_chunk > 0 ? ctx->frame_chunk : 64;
+    if (R.n_total >= (1ull << 32)) return fail(IAC_ERR_LIMIT, "pair engine: at most 2^32 - 1 records per call");
+
+    // ---- plan: cell grid from the k extents, per-cell extents, tasks (cached)
+    void *ext = nullptr;
+    IAC_TRY(scratch(ctx, 3, (6 + 6 * (size_t)kMaxCells) * sizeof(uint32_t), &ext));
+    uint32_t *d_ext = static_cast<uint32_t *>(ext);
+    hipLaunchKernelGGL(fill_u32_kernel, dim3(1), dim3(64), 0, st, d_ext, 0xffffffffu, (size_t)3);
+    hipLaunchKernelGGL(fill_u32_kernel, dim3(1), dim3(64), 0, st, d_ext + 3, 0u, (size_t)3);
+    const unsigned eb = std::max(1u, std::min<unsigned>(1024, (R.npix + 255) / 256));
+    hipLaunchKernelGGL(kext_kernel, dim3(eb), dim3(256), 0, st, R.d_kmap, R.npix, d_ext);
+    IAC_HIP(hipGetLastError());
+    uint32_t hext[6];
+    IAC_HIP(hipMemcpyAsync(hext, d_ext, sizeof(hext), hipMemcpyDeviceToHost, st));
+    IAC_HIP(hipStreamSynchronize(st));
+    float kmin[3], ext3[3];
+    for (int d = 0; d < 3; d++) {
+        kmin[d] = unkey(hext[d]);
+        ext3[d] = unkey(hext[3 + d]) - kmin[d];
+    }
+    CellGeom g{};
+    int order[3] = {0, 1, 2};
+    std::sort(order, order + 3, [&](int a, int b) { return ext3[a] > ext3[b]; });
+    g.au = std::min(order[0], order[1]);
+    g.av = std::max(order[0], order[1]);
+    double w = (double)cell_vox / (double)S.P.vox;
+    auto ncells = [&](int a) { return std::max(1, (int)std::ceil(ext3[a] / w * (1 + 1e-6))); };
+    while ((long)ncells(g.au) * ncells(g.av) > kMaxCells) w *= 1.25;
+    g.inv_w = (float)(1.0 / w);
+    g.u0 = kmin[g.au];
+    g.v0 = kmin[g.av];
+    g.ncu = ncells(g.au);
+    g.ncv = ncells(g.av);
+    const int nc = g.ncu * g.ncv;
+    hipLaunchKernelGGL(init_cell_ext_kernel, dim3((nc + 255) / 256), dim3(256), 0, st, d_ext + 6, nc);
+    hipLaunchKernelGGL(cell_ext_kernel, dim3(eb), dim3(256), 0, st, R.d_kmap, R.npix, g, d_ext + 6);
+    IAC_HIP(hipGetLastError());
+    std::vector<uint32_t> cext(6 * (size_t)nc);
+    IAC_HIP(hipMemcpyAsync(cext.data(), d_ext + 6, cext.size() * 4, hipMemcpyDeviceToHost, st));
+    IAC_HIP(hipStreamSynchronize(st));
+    iac_pair_plan *pl = find_plan(ctx, R.d_kmap, R.npix, cell_vox, S);
+    if (pl && (std::memcmp(&pl->g, &g, sizeof(g)) != 0 || pl->cell_ext != cext)) pl = nullptr;
+    if (!pl) {
+        // reuse the oldest slot beyond four cached plans
+        if (ctx->pair_plans.size() >= 4) {
+            iac_pair_plan *old = ctx->pair_plans.front();
+            if (old->d_tasks) IAC_HIP(hipFree(old->d_tasks));
+            delete old;
+            ctx->pair_plans.erase(ctx->pair_plans.begin());
+        }
+        pl = new iac_pair_plan();
+        ctx->pair_plans.push_back(pl);
+        pl->kmap = R.d_kmap;
+        pl->npix = R.npix;
+        pl->cell_vox = cell_vox;
+        pl->spec = S;
+        pl->g = g;
+        pl->cell_ext.swap(cext);
+        build_tasks(*pl, ordered);
+        if (!pl->tasks.empty()) {
+            IAC_HIP(hipMalloc(&pl->d_tasks, pl->tasks.size() * sizeof(Task)));
+            IAC_HIP(hipMemcpy(pl->d_tasks, pl->tasks.data(), pl->tasks.size() * sizeof(Task),
+                              hipMemcpyHostToDevice));
+        }
+        pl->valid = true;
+    }
+
+    // ---- chunks: runs of at most fchunk frames inside one group
+    std::vector<Chunk> chunks;
+    for (uint32_t gi = 0; gi < R.ngroups; gi++)
+        for (uint32_t f = R.h_group_off[gi]; f < R.h_group_off[gi + 1]; f += fchunk)
+            chunks.push_back(Chunk{f, std::min(R.h_group_off[gi + 1], f + fchunk), gi});
+    const size_t nvox = (size_t)S.P.sab * S.P.sab * S.P.sc;
+
+    // ---- scratch: sorted records + cell offsets (slot 1), H + group offsets + self + chunks (slot 2)
+    void *s1 = nullptr, *s2 = nullptr;
+    const size_t cs_bytes = (size_t)R.nframes * (nc + 1) * sizeof(uint32_t);
+    IAC_TRY(scratch(ctx, 1, R.n_total * sizeof(float4) + cs_bytes, &s1));
+    float4 *srec = static_cast<float4 *>(s1);
+    uint32_t *cs = reinterpret_cast<uint32_t *>(static_cast<char *>(s1) + R.n_total * sizeof(float4));
+    const size_t h_bytes = ordered ? 0 : nvox * R.ngroups * sizeof(unsigned long long);
+    const size_t goff_bytes = ((R.ngroups + 1) * sizeof(uint32_t) + 255) / 256 * 256;
+    const size_t self_bytes = (R.ngroups * sizeof(unsigned long long) + 255) / 256 * 256;
+    const size_t chunk_bytes = chunks.size() * sizeof(Chunk);
+    IAC_TRY(scratch(ctx, 2, h_bytes + goff_bytes + self_bytes + chunk_bytes, &s2));
+    char *p2 = static_cast<char *>(s2);
+    auto *H = reinterpret_cast<unsigned long long *>(p2);
+    auto *d_goff = reinterpret_cast<uint32_t *>(p2 + h_bytes);
+    auto *d_self = reinterpret_cast<unsigned long long *>(p2 + h_bytes + goff_bytes);
+    auto *d_chunks = reinterpret_cast<Chunk *>(p2 + h_bytes + goff_bytes + self_bytes);
+    IAC_HIP(hipMemcpyAsync(d_goff, R.h_group_off, (R.ngroups + 1) * sizeof(uint32_t), hipMemcpyHostToDevice, st));
+    if (!chunks.empty())
+        IAC_HIP(hipMemcpyAsync(d_chunks, chunks.data(), chunk_bytes, hipMemcpyHostToDevice, st));
+    IAC_HIP(hipStreamSynchronize(st));  // the host arrays above are released on return
+    const unsigned zb = (unsigned)std::max<size_t>(1, std::min<size_t>(4096, (nvox * R.ngroups + 255) / 256));
+    if (!ordered) {
+        hipLaunchKernelGGL(fill_u64_kernel, dim3(zb), dim3(256), 0, st, H, nvox * R.ngroups);
+        hipLaunchKernelGGL(fill_u64_kernel, dim3(1), dim3(256), 0, st, d_self, (size_t)R.ngroups);
+    }
+
+    if (S.kind == kKindAc3d) {
+        if (ordered)
+            hipLaunchKernelGGL((cell_sort_kernel<kKindAc3d, true>), dim3(R.nframes), dim3(kBlock), 0, st, R.rec,
+                               R.d_frame_off, g, srec, cs, S, d_goff, R.ngroups, d_self);
+        else
+            hipLaunchKernelGGL((cell_sort_kernel<kKindAc3d, false>), dim3(R.nframes), dim3(kBlock), 0, st, R.rec,
+                               R.d_frame_off, g, srec, cs, S, d_goff, R.ngroups, d_self);
+    } else {
+        if (ordered)
+            hipLaunchKernelGGL((cell_sort_kernel<kKindCq, true>), dim3(R.nframes), dim3(kBlock), 0, st, R.rec,
+                               R.d_frame_off, g, srec, cs, S, d_goff, R.ngroups, d_self);
+        else
+            hipLaunchKernelGGL((cell_sort_kernel<kKindCq, false>), dim3(R.nframes), dim3(kBlock), 0, st, R.rec,
+                               R.d_frame_off, g, srec, cs, S, d_goff, R.ngroups, d_self);
+    }
+    IAC_HIP(hipGetLastError());
+
+    const uint32_t ntasks = (uint32_t)pl->tasks.size();
+    auto *target = ordered ? reinterpret_cast<unsigned long long *>(R.d_mesh) : H;
+    if (ntasks && !chunks.empty()) {
+        const dim3 grid(ntasks * (uint32_t)chunks.size());
+        IAC_TRY(timer_begin(ctx));
+        if (S.kind == kKindAc3d) {
+            if (ordered)
+                hipLaunchKernelGGL((pairs_kernel<kKindAc3d, true>), grid, dim3(kBlock), 0, st, srec, R.d_frame_off,
+                                   cs, d_chunks, pl->d_tasks, ntasks, g, S, target, nvox, ctx->d_counters);
+            else
+                hipLaunchKernelGGL((pairs_kernel<kKindAc3d, false>), grid, dim3(kBlock), 0, st, srec, R.d_frame_off,
+                                   cs, d_chunks, pl->d_tasks, ntasks, g, S, target, nvox, ctx->d_counters);
+        } else {
+            if (ordered)
+                hipLaunchKernelGGL((pairs_kernel<kKindCq, true>), grid, dim3(kBlock), 0, st, srec, R.d_frame_off,
+                                   cs, d_chunks, pl->d_tasks, ntasks, g, S, target, nvox, ctx->d_counters);
+            else
+                hipLaunchKernelGGL((pairs_kernel<kKindCq, false>), grid, dim3(kBlock), 0, st, srec, R.d_frame_off,
+                                   cs, d_chunks, pl->d_tasks, ntasks, g, S, target, nvox, ctx->d_counters);
+        }
+        IAC_HIP(hipGetLastError());
+        IAC_TRY(timer_end(ctx));
+    }
+    if (!ordered) {
+        const size_t centre = (size_t)S.P.c0 + (size_t)S.P.c1 * S.P.sab + (size_t)S.P.c2 * S.P.sab * S.P.sab;
+        hipLaunchKernelGGL(symmetrize_kernel, dim3(zb), dim3(256), 0, st, H,
+                           reinterpret_cast<unsigned long long *>(R.d_mesh), nvox, R.ngroups, centre, d_self);
+        IAC_HIP(hipGetLastError());
+    }
+    return IAC_OK;
+}
+
+}  // namespace iac

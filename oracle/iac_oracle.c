@@ -561,6 +561,59 @@ void ora_merge_cq(const double *small_cq, const ora_mesh_shape *ss_, const float
     }
 }
 
+void ora_cq_small_f64(const float *intensity, const float *kmap, uint32_t npix,
+                      const ora_mesh_shape *s, double *mesh, int nthreads)
+{
+#ifdef _OPENMP
+    int nt = nthreads > 0 ? nthreads : omp_get_max_threads();
+#endif
+    const float vox = (float)s->voxel_per_dq;
+    const int ai = (int)s->k_a, bi = (int)s->k_b, ci = (int)s->k_c;
+    const int sab = (int)s->size_ab, sc = (int)s->size_c;
+    const int cab = (sab - 1) / 2, cc = (sc - 1) / 2;
+#pragma omp parallel for schedule(dynamic, 16) num_threads(nt)
+    for (uint32_t ind = 0; ind < npix; ind++) {
+        for (uint32_t i = 0; i < npix; i++) {
+            const double val = (double)intensity[ind] * (double)intensity[i];
+            if (val == 0.0) continue;
+            float q[3];
+            for (int d = 0; d < 3; d++) q[d] = kmap[3 * (size_t)ind + d] - kmap[3 * (size_t)i + d];
+            if (sqrtf(q[0] * q[0] + q[1] * q[1] + q[2] * q[2]) > s->max_q) continue;
+            for (int d = 0; d < 3; d++) q[d] = q[d] * vox;
+            const int fs = (int)(roundf(q[ai]) + (float)cab);
+            const int ms = (int)(roundf(q[bi]) + (float)cab);
+            const int ss = (int)(roundf(q[ci]) + (float)cc);
+            if (fs < 0 || ms < 0 || ss < 0 || fs >= sab || ms >= sab || ss >= sc) continue;
+#pragma omp atomic
+            mesh[(size_t)fs + (size_t)ms * sab + (size_t)ss * sab * sab] += val;
+        }
+    }
+}
+
+void ora_merge_f64(const double *small_cq, const ora_mesh_shape *s, const float *R, double *big)
+{
+    const int sab = (int)s->size_ab, sc = (int)s->size_c;
+    const int cab = (sab - 1) / 2, cc = (sc - 1) / 2;
+    for (size_t ind = 0; ind < (size_t)sab * sab * sc; ind++) {
+        const double v = small_cq[ind];
+        if (v == 0.0) continue;
+        const int ss_in = (int)(ind / ((size_t)sab * sab));
+        const int ms_in = (int)((ind - (size_t)ss_in * sab * sab) / sab);
+        const int fs_in = (int)(ind - (size_t)ss_in * sab * sab - (size_t)ms_in * sab);
+        float q_in[3];
+        q_in[s->k_a] = (float)(fs_in - cab);
+        q_in[s->k_b] = (float)(ms_in - cab);
+        q_in[s->k_c] = (float)(ss_in - cc);
+        const float q0 = R[0] * q_in[0] + R[1] * q_in[1] + R[2] * q_in[2];
+        const float q1 = R[3] * q_in[0] + R[4] * q_in[1] + R[5] * q_in[2];
+        const float q2 = R[6] * q_in[0] + R[7] * q_in[1] + R[8] * q_in[2];
+        const int fs = (int)(roundf(q0) + (float)cab), ms = (int)(roundf(q1) + (float)cab),
+                  ss = (int)(roundf(q2) + (float)cab);
+        if (fs < 0 || ms < 0 || ss < 0 || fs >= sab || ms >= sab || ss >= sab) continue;
+        big[(size_t)fs + (size_t)ms * sab + (size_t)ss * sab * sab] += v * (double)R[9];
+    }
+}
+
 /* ------------------------------------------------------------------------- */
 /* AC1D */
 

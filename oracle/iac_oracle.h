@@ -103,6 +103,16 @@ void ora_merge_cq(const double *small_cq, const ora_mesh_shape *small_shape, con
                   uint32_t nev, int interp, double mult, uint64_t *big, uint64_t *dropped,
                   int nthreads);
 
+/* FP64 oracle of the dense float path: the C(q) small binning of every ordered
+ * pixel pair (nearest neighbour, AutoCorr_CQ_small index rule) with the value
+ * (double)I_a * (double)I_b accumulated in double -- no fixed point. */
+void ora_cq_small_f64(const float *intensity, const float *kmap, uint32_t npix,
+                      const ora_mesh_shape *s, double *mesh, int nthreads);
+/* FP64 merge: small-mesh voxel v rotated by R (nearest neighbour, Merge_CQ index
+ * rule) adds w * small[v] to the cubic big mesh in double. */
+void ora_merge_f64(const double *small_cq, const ora_mesh_shape *small_shape, const float *rw10,
+                   double *big);
+
 /* ---------------- AC1D: AC1D.cpp ---------------- */
 /* AC1D::Initialize(Det, Size, QZoom), AC1D.cpp:53-58 */
 void ora_ac1d_shape(const float max_q[3], uint32_t size, float q_zoom, float *max_q_out,

@@ -81,6 +81,8 @@ _SIG = {
     "ora_ac1d_acuw": (None, [F, U32, U32, C.c_float, C.c_float, I32, I32, D, U64P]),
     "ora_ac1d_calc_ac": (None, [D, D, U32, D]),
     "ora_merge_ac_cq": (None, [U64P, D, C.c_size_t, D]),
+    "ora_cq_small_f64": (None, [F, F, U32, SP, D, I32]),
+    "ora_merge_f64": (None, [D, SP, F, D]),
     "ora_simulate_fp64": (None, [F, U32, U32, U32, D, U32, D, U32, U32, C.c_double, I32, D, D, I32,
                                  D, I32]),
     "ora_philox4x32_10": (None, [C.POINTER(C.c_uint32), C.POINTER(C.c_uint32), C.POINTER(C.c_uint32)]),
@@ -230,3 +232,101 @@ def ac3d_frames(kmap, pix, cnt, frame_off, shape, rots=None, interp1=0, interp2=
 
 def pair_val(p: float) -> int:
     return int(lib().ora_pair_val(float(p)))
+
+
+# ---- C(q) ------------------------------------------------------------------------
+def cq_small_multiplicator(intensity):
+    return float(lib().ora_cq_small_multiplicator(fp(_f(intensity)), np.asarray(intensity).size))
+
+
+def cq_small(intensity, kmap, shape, interp=0, mult=1.0, nthreads=0, mesh=None):
+    s = MeshShape.from_any(shape)
+    I = _f(intensity).reshape(-1)
+    if mesh is None:
+        mesh = np.zeros(s.nvox, np.uint64)
+    dropped = np.zeros(1, np.uint64)
+    lib().ora_cq_small(fp(I), fp(_f(kmap)), I.size, C.byref(s), interp, mult, up(mesh), up(dropped),
+                       nthreads)
+    return mesh, int(dropped[0])
+
+
+def merge_prepare(rot9, mean_intensity, small_cq):
+    R = _f(rot9).reshape(-1, 9)
+    mi = _f(mean_intensity)
+    cq = np.ascontiguousarray(small_cq, np.float64)
+    rw = np.empty((R.shape[0], 10), np.float32)
+    m = lib().ora_merge_prepare(fp(R), fp(mi), R.shape[0], dp(cq), cq.size, fp(rw))
+    return rw, float(m)
+
+
+def merge_cq(small_cq, small_shape, rw10, interp=0, mult=1.0, nthreads=0):
+    s = MeshShape.from_any(small_shape)
+    cq = np.ascontiguousarray(small_cq, np.float64)
+    rw = _f(rw10).reshape(-1, 10)
+    big = np.zeros(s.size_ab ** 3, np.uint64)
+    dropped = np.zeros(1, np.uint64)
+    lib().ora_merge_cq(dp(cq), C.byref(s), fp(rw), rw.shape[0], interp, mult, up(big), up(dropped), nthreads)
+    return big, int(dropped[0])
+
+
+def merge_ac_cq(ac, cq):
+    ac = np.ascontiguousarray(ac, np.uint64)
+    cq = np.ascontiguousarray(cq, np.float64)
+    out = np.empty(ac.size, np.float64)
+    lib().ora_merge_ac_cq(up(ac), dp(cq), ac.size, dp(out))
+    return out
+
+
+# ---- AC1D ------------------------------------------------------------------------
+def ac1d_shape(max_q, size, zoom=1.0):
+    mq, dq = np.zeros(1, np.float32), np.zeros(1, np.float32)
+    lib().ora_ac1d_shape(fp(_f(max_q)), size, zoom, fp(mq), fp(dq))
+    return float(mq[0]), float(dq[0])
+
+
+def ac1d_cq_multiplicator(intensity):
+    return float(lib().ora_ac1d_cq_multiplicator(fp(_f(intensity)), np.asarray(intensity).size))
+
+
+def ac1d_cq(intensity, kmap, size, max_q, dq, interp=1, mult=1.0, nthreads=0):
+    I = _f(intensity).reshape(-1)
+    hist = np.zeros(size, np.uint64)
+    dropped = np.zeros(1, np.uint64)
+    lib().ora_ac1d_cq(fp(I), fp(_f(kmap)), I.size, size, max_q, dq, interp, mult, up(hist), up(dropped),
+                      nthreads)
+    return hist, int(dropped[0])
+
+
+def ac1d_acuw(hl, size, max_q, dqdx, interp=1, self_pair_factor=2, ac=None):
+    hl = _f(hl).reshape(-1, 4)
+    if ac is None:
+        ac = np.zeros(size, np.float64)
+    dropped = np.zeros(1, np.uint64)
+    lib().ora_ac1d_acuw(fp(hl), hl.shape[0], size, max_q, dqdx, interp, self_pair_factor, dp(ac), up(dropped))
+    return ac, int(dropped[0])
+
+
+def ac1d_calc_ac(ac_uw, cq):
+    a = np.ascontiguousarray(ac_uw, np.float64)
+    c = np.ascontiguousarray(cq, np.float64)
+    out = np.empty(a.size, np.float64)
+    lib().ora_ac1d_calc_ac(dp(a), dp(c), a.size, dp(out))
+    return out
+
+
+def cq_small_f64(intensity, kmap, shape, nthreads=0, mesh=None):
+    s = MeshShape.from_any(shape)
+    I = _f(intensity).reshape(-1)
+    if mesh is None:
+        mesh = np.zeros(s.nvox, np.float64)
+    lib().ora_cq_small_f64(fp(I), fp(_f(kmap)), I.size, C.byref(s), dp(mesh), nthreads)
+    return mesh
+
+
+def merge_f64(small_cq, small_shape, rw10, big=None):
+    s = MeshShape.from_any(small_shape)
+    cq = np.ascontiguousarray(small_cq, np.float64)
+    if big is None:
+        big = np.zeros(s.size_ab ** 3, np.float64)
+    lib().ora_merge_f64(dp(cq), C.byref(s), fp(_f(rw10).reshape(10)), dp(big))
+    return big
