@@ -86,8 +86,9 @@ int iac_ctx_enable_timing(iac_ctx *ctx, int enable);
 int iac_ctx_timing(iac_ctx *ctx, double *ms, uint64_t *launches, int reset);
 /* Context options (tuning and kernel selection for measurement and tests). */
 #define IAC_OPT_AC3D_KERNEL 1 /* 0 automatic, 1 global-atomic tile kernel, 2 LDS cell-box kernel */
-#define IAC_OPT_CELL_VOX 2    /* LDS kernel: cell side in voxels, 0 = default (4) */
-#define IAC_OPT_FRAME_CHUNK 3 /* LDS kernel: frames per block, 0 = default (128) */
+#define IAC_OPT_CELL_VOX 2    /* LDS kernel: cell side in voxels, 0 = default (6 for AC_uw, 4 for C(q)) */
+#define IAC_OPT_FRAME_CHUNK 3 /* LDS kernel: frames per block, 0 = default (64) */
+#define IAC_OPT_SIM_CHUNK_ROWS 4 /* simulator: detector rows (pixel x subpixel) per GEMM, 0 = auto (1 GiB of FP32 field) */
 int iac_ctx_set_option(iac_ctx *ctx, int option, int64_t value);
 /* Updates whose voxel fell outside the mesh since the context was created.
  * The reference writes such updates out of bounds; this path drops them. */
@@ -209,6 +210,50 @@ int iac_merge_cq_multi(iac_ctx *ctx, const double *d_small_meshes, const iac_mes
 /* RunIAC::Merge_ACandCQ, IAC/RunIAC.cpp:120-158: out = IntToFloat(ac) / cq, 0 where
  * cq <= 0 or the quotient is NaN. */
 int iac_merge_ac_cq(iac_ctx *ctx, const uint64_t *d_ac, const double *d_cq, size_t n, double *d_out);
+
+/* ---------------- simulator: IAC/Simulator.cpp:662-944, IAC/kernel.cl:768-859 ---------------- */
+/* psi(p, f) = sum_e exp(i (k_p . r_e + phi_{e,f})), k_p = 2 pi / lambda * p / |p|, over the
+ * (2 SuSa + 1)^2 subpixels p + v v_step + u u_step of each pixel.  Phases are
+ * [nframes * modes][nem] (mode-minor: row f * modes + m); the output [nframes][npix]
+ * sums, over the modes of a frame, the subpixel mean of |psi| (IAC_SIM_AMPLITUDE, what
+ * SimulateCrystal returns, kernel.cl:844) or |psi|^2 (IAC_SIM_INTENSITY). */
+#define IAC_SIM_FP32_DIRECT 0 /* the reference kernel's FP32 arithmetic, iac_sim_direct_fp32 */
+#define IAC_SIM_BF16X3 1      /* complex GEMM, bf16 hi + lo split, 3 MFMA products, FP32 accumulate */
+#define IAC_SIM_FP16X3 2      /* the same with fp16 hi + lo */
+#define IAC_SIM_BF16 3        /* one bf16 product (about 2^-9 per element; below the 1e-3 bar) */
+#define IAC_SIM_FP16 4        /* one fp16 product */
+#define IAC_SIM_AMPLITUDE 0
+#define IAC_SIM_INTENSITY 1
+typedef struct iac_sim_plan iac_sim_plan;
+/* The emitter-geometry half of the GEMM: X[p s][e] = exp(i k_{p s} . r_e) with the phase
+ * argument in FP64, stored as 16-bit hi (+ lo) of [cos | sin] (npix (2SuSa+1)^2 x 2 nem x
+ * 2 or 4 bytes of device memory, kept until iac_sim_plan_destroy).  d_pixmap: float
+ * [npix][3] (Detector::PixelMap), d_emitters: double [nem][3]; u_step / v_step as
+ * SimulatePart computes them (Simulator.cpp:742-760); mode IAC_SIM_BF16X3..IAC_SIM_FP16.
+ * Replaces the per-frame upload and the k / argument work of SimulateCrystal. */
+int iac_sim_plan_create(iac_ctx *ctx, const float *d_pixmap, uint32_t npix, int subsampling, const double u_step[3],
+                        const double v_step[3], double wavelength, const double *d_emitters, uint32_t nem, int mode,
+                        iac_sim_plan **out);
+int iac_sim_plan_destroy(iac_sim_plan *plan);
+/* The SimulatePart frame loop (Simulator.cpp:694-862) for nframes frames at once:
+ * d_phases double [nframes * modes][nem] -> d_out float [nframes][npix]. */
+int iac_sim_run(iac_ctx *ctx, iac_sim_plan *plan, const double *d_phases, uint32_t nframes, uint32_t modes,
+                int out_kind, float *d_out);
+/* SimulateCrystal as the reference computes it, in FP32 (float positions and phases,
+ * k recomputed per emitter), for nframes frames: d_emitters float [nem][3], d_phases
+ * float [nframes * modes][nem] -> d_out float [nframes][npix].  The plain baseline. */
+int iac_sim_direct_fp32(iac_ctx *ctx, const float *d_pixmap, uint32_t npix, int subsampling, const float u_step[3],
+                        const float v_step[3], float wavelength, const float *d_emitters, uint32_t nem,
+                        const float *d_phases, uint32_t nframes, uint32_t modes, int out_kind, float *d_out);
+/* SimulatePart's post-processing (Simulator.cpp:868-888) per frame, in place: scale to
+ * n_expected photons (double sum, float factor), Poisson draw (ScalarPoissonSampling,
+ * ArrayOperators.h:25-55) if do_poisson, white noise (2u - 1) adu_noise if adu_noise != 0
+ * (ArrayOperators.h:219-226), times value_per_photon.  The random numbers come from
+ * Philox4x32-10 keyed by seed with counter (pixel, frame0 + f, tag, stream): reproducible
+ * and independent of launch order (the reference's mt19937_64 seeded from
+ * random_device is neither). */
+int iac_sim_postprocess(iac_ctx *ctx, float *d_frames, uint32_t npix, uint32_t nframes, double n_expected,
+                        int do_poisson, float adu_noise, float value_per_photon, uint64_t seed, uint32_t frame0);
 
 #ifdef __cplusplus
 }

@@ -8,6 +8,7 @@ by the tests and the benchmark: device buffers are torch tensors on the GPU
 from __future__ import annotations
 
 import ctypes as C
+import weakref
 
 import numpy as np
 
@@ -15,7 +16,7 @@ from ._abi import IACError, MeshShape, check, fptr, lib
 
 __all__ = [
     "IACError", "MeshShape", "lib", "generate_pixel_map", "calc_kmap", "mesh_shape_big",
-    "mesh_shape_small_cofq", "Context", "NEAREST", "LINEAR",
+    "mesh_shape_small_cofq", "Context", "SimPlan", "NEAREST", "LINEAR",
 ]
 
 NEAREST = 0
@@ -25,6 +26,9 @@ LINEAR = 1
 OPT_AC3D_KERNEL = 1
 OPT_CELL_VOX = 2
 OPT_FRAME_CHUNK = 3
+OPT_SIM_CHUNK_ROWS = 4
+SIM_FP32_DIRECT, SIM_BF16X3, SIM_FP16X3, SIM_BF16, SIM_FP16 = 0, 1, 2, 3, 4
+SIM_AMPLITUDE, SIM_INTENSITY = 0, 1
 
 
 def _f32(a, n=None) -> np.ndarray:
@@ -121,6 +125,7 @@ class Context:
 
     def __init__(self, device: int = 0, torch_stream: bool = True):
         self._h = C.c_void_p()
+        self._plans: weakref.WeakSet = weakref.WeakSet()
         check(lib().iac_ctx_create(int(device), C.byref(self._h)), "iac_ctx_create")
         self.device = device
         if torch_stream:
@@ -135,6 +140,8 @@ class Context:
 
     def close(self) -> None:
         if self._h:
+            for p in list(self._plans):  # plans hold device memory of this context
+                p.destroy()
             lib().iac_ctx_destroy(self._h)
             self._h = C.c_void_p()
 
@@ -251,3 +258,64 @@ class Context:
 
     def merge_ac_cq(self, ac, cq, out) -> None:
         check(lib().iac_merge_ac_cq(self._h, _ptr(ac), _ptr(cq), ac.numel(), _ptr(out)), "iac_merge_ac_cq")
+
+    # ---- simulator (Simulator::SimulatePart, IAC/Simulator.cpp:662-944) ----
+    def sim_plan(self, pixmap, emitters, wavelength: float, mode: int = SIM_BF16X3, subsampling: int = 0,
+                 u_step=(0.0, 0.0, 0.0), v_step=(0.0, 0.0, 0.0)) -> "SimPlan":
+        """Phase matrix of a detector (float [npix][3] device tensor) and an
+        emitter set (double [nem][3] device tensor) for the GEMM simulator."""
+        us = (C.c_double * 3)(*u_step)
+        vs = (C.c_double * 3)(*v_step)
+        h = C.c_void_p()
+        npix = pixmap.numel() // 3
+        nem = emitters.numel() // 3
+        check(lib().iac_sim_plan_create(self._h, _ptr(pixmap), npix, subsampling, us, vs, wavelength,
+                                        _ptr(emitters), nem, mode, C.byref(h)), "iac_sim_plan_create")
+        plan = SimPlan(self, h, npix, nem)
+        self._plans.add(plan)
+        return plan
+
+    def sim_run(self, plan: "SimPlan", phases, out, modes: int = 1, out_kind: int = SIM_AMPLITUDE) -> None:
+        """phases: double [nframes * modes][nem]; out: float [nframes][npix]."""
+        nframes = phases.numel() // (plan.nem * modes)
+        check(lib().iac_sim_run(self._h, plan.handle, _ptr(phases), nframes, modes, out_kind, _ptr(out)),
+              "iac_sim_run")
+
+    def sim_direct_fp32(self, pixmap, emitters, phases, wavelength: float, out, modes: int = 1,
+                        out_kind: int = SIM_AMPLITUDE, subsampling: int = 0, u_step=(0.0, 0.0, 0.0),
+                        v_step=(0.0, 0.0, 0.0)) -> None:
+        """The reference SimulateCrystal arithmetic in FP32: float emitters [nem][3],
+        float phases [nframes * modes][nem] -> out float [nframes][npix]."""
+        us = (C.c_float * 3)(*u_step)
+        vs = (C.c_float * 3)(*v_step)
+        npix = pixmap.numel() // 3
+        nem = emitters.numel() // 3
+        nframes = phases.numel() // (nem * modes)
+        check(lib().iac_sim_direct_fp32(self._h, _ptr(pixmap), npix, subsampling, us, vs, wavelength,
+                                        _ptr(emitters), nem, _ptr(phases), nframes, modes, out_kind, _ptr(out)),
+              "iac_sim_direct_fp32")
+
+    def sim_postprocess(self, frames, n_expected: float, poisson: bool = True, adu_noise: float = 0.0,
+                        value_per_photon: float = 1.0, seed: int = 0, frame0: int = 0) -> None:
+        """Rescale / Poisson / noise / value-per-photon of float [nframes][npix] in place."""
+        nframes, npix = frames.shape
+        check(lib().iac_sim_postprocess(self._h, _ptr(frames), npix, nframes, n_expected, int(poisson),
+                                        adu_noise, value_per_photon, seed, frame0), "iac_sim_postprocess")
+
+
+class SimPlan:
+    """Owns an iac_sim_plan (device phase matrix); destroy() or garbage collection frees it."""
+
+    def __init__(self, ctx: Context, handle: C.c_void_p, npix: int, nem: int):
+        self.ctx, self.handle, self.npix, self.nem = ctx, handle, npix, nem
+
+    def destroy(self) -> None:
+        if self.handle:
+            lib().iac_sim_plan_destroy(self.handle)
+            self.handle = None
+
+    def __del__(self):
+        try:
+            self.destroy()
+        except Exception:
+            pass

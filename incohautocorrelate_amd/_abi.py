@@ -85,6 +85,13 @@ SIGNATURES: dict[str, tuple] = {
     "iac_merge_cq": (I32, [P, P, C.POINTER(MeshShape), P, U32, I32, C.c_double, P]),
     "iac_merge_cq_multi": (I32, [P, P, C.POINTER(MeshShape), P, U32, I32, C.c_double, P]),
     "iac_merge_ac_cq": (I32, [P, P, P, C.c_size_t, P]),
+    "iac_sim_plan_create": (I32, [P, P, U32, I32, C.POINTER(C.c_double), C.POINTER(C.c_double), C.c_double, P, U32,
+                                  I32, C.POINTER(P)]),
+    "iac_sim_plan_destroy": (I32, [P]),
+    "iac_sim_run": (I32, [P, P, P, U32, U32, I32, P]),
+    "iac_sim_direct_fp32": (I32, [P, P, U32, I32, C.POINTER(C.c_float), C.POINTER(C.c_float), C.c_float, P, U32, P,
+                                  U32, U32, I32, P]),
+    "iac_sim_postprocess": (I32, [P, P, U32, U32, C.c_double, I32, C.c_float, C.c_float, C.c_uint64, U32]),
 }
 
 

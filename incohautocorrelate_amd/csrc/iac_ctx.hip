@@ -160,6 +160,7 @@ int iac_ctx_destroy(iac_ctx *ctx)
     for (int i = 0; i < 4; i++)
         if (ctx->scratch[i]) (void)hipFree(ctx->scratch[i]);
     iac::free_pair_plans(ctx);
+    iac::free_sim_resources(ctx);
     for (auto e : ctx->ev_start) (void)hipEventDestroy(e);
     for (auto e : ctx->ev_stop) (void)hipEventDestroy(e);
     if (ctx->d_counters) (void)hipFree(ctx->d_counters);
@@ -277,6 +278,10 @@ int iac_ctx_set_option(iac_ctx *ctx, int option, int64_t value)
     case IAC_OPT_FRAME_CHUNK:
         if (value < 0 || value > (1 << 20)) return fail(IAC_ERR_INVALID_ARG, "IAC_OPT_FRAME_CHUNK is 0..2^20");
         ctx->frame_chunk = (uint32_t)value;
+        return IAC_OK;
+    case IAC_OPT_SIM_CHUNK_ROWS:
+        if (value < 0 || value > (1ll << 31)) return fail(IAC_ERR_INVALID_ARG, "IAC_OPT_SIM_CHUNK_ROWS is 0..2^31");
+        ctx->sim_chunk_rows = (uint64_t)value;
         return IAC_OK;
     default:
         return fail(IAC_ERR_INVALID_ARG, "unknown option");

@@ -27,10 +27,12 @@ struct iac_ctx {
     std::vector<hipEvent_t> ev_start, ev_stop;
     size_t ev_used = 0;
     std::vector<iac_pair_plan *> pair_plans;  // small cache, oldest first
+    void *rocblas = nullptr;                   // rocblas_handle of the simulator GEMM, made on first use
     // options (IAC_OPT_*)
     int ac3d_kernel = 0;
     int cell_vox = 0;
     uint32_t frame_chunk = 0;
+    uint64_t sim_chunk_rows = 0;
 };
 
 namespace iac {
@@ -50,6 +52,7 @@ int timer_begin(iac_ctx *ctx);
 int timer_end(iac_ctx *ctx);
 
 void free_pair_plans(iac_ctx *ctx);
+void free_sim_resources(iac_ctx *ctx);
 
 // Smallest float s >= 0 with sqrtf(s) > max_q (correctly rounded sqrtf).
 // `sqrtf(s) > max_q` is then exactly `s >= cut_sq(max_q)` for s >= 0.

@@ -815,9 +815,9 @@ void ora_philox4x32_10(const uint32_t ctr[4], const uint32_t key[2], uint32_t ou
     out[3] = c3;
 }
 
-void ora_uniform2(uint64_t seed, uint32_t frame, uint32_t pixel, double u[2])
+void ora_uniform2(uint64_t seed, uint32_t frame, uint32_t pixel, uint32_t stream, double u[2])
 {
-    uint32_t ctr[4] = {pixel, frame, 0x49414350u /* stream tag */, 0};
+    uint32_t ctr[4] = {pixel, frame, 0x49414350u /* tag */, stream};
     uint32_t key[2] = {(uint32_t)seed, (uint32_t)(seed >> 32)};
     uint32_t r[4];
     ora_philox4x32_10(ctr, key, r);
@@ -923,21 +923,37 @@ uint32_t ora_poisson_sample(double mean, const double u[2])
     return (uint32_t)ret;
 }
 
-/* Simulator::SimulatePart post-processing, IAC/Simulator.cpp:870-888: the
- * integrated intensity is summed serially in double, the float array is
- * scaled by (float)(N_exp / ΣI) (ParMultiplyScalar(float*, float, int)), then
- * ParPoissonSampling(float*) replaces each value by a Poisson draw. */
-void ora_rescale_poisson(float *intensity, uint32_t npix, double n_expected, int do_poisson,
-                         uint64_t seed, uint32_t frame)
+/* Simulator::SimulatePart post-processing, IAC/Simulator.cpp:868-888, one frame
+ * in place: the integrated intensity in double (the reference sums serially;
+ * here in 1024-pixel chunks summed in order, then the chunk sums in order --
+ * the same value up to double rounding, and the order the GPU reduction uses);
+ * the array scaled by (float)(n_expected / sum) (ParMultiplyScalar(float*,
+ * float, int), ArrayOperators.h:94-101); if do_poisson, each value replaced by
+ * a Poisson draw (ParPoissonSampling(float*), ArrayOperators.h:203-210) from
+ * stream 0; if adu_noise != 0, (2u - 1) * adu_noise added in double from
+ * stream 1 (ParAddWhiteNoise(float*, float, ...), ArrayOperators.h:219-226);
+ * finally a float multiply by value_per_photon (MultiplyScalar<float>). */
+void ora_sim_postprocess(float *I, uint32_t npix, double n_expected, int do_poisson, float adu_noise,
+                         float value_per_photon, uint64_t seed, uint32_t frame)
 {
     double integ = 0.0;
-    for (uint32_t l = 0; l < npix; l++) integ += intensity[l];
+    for (uint32_t c0 = 0; c0 < npix; c0 += 1024) {
+        double part = 0.0;
+        for (uint32_t l = c0; l < npix && l < c0 + 1024; l++) part += I[l];
+        integ += part;
+    }
     float fac = (float)(n_expected / integ);
-    for (uint32_t l = 0; l < npix; l++) intensity[l] *= fac;
-    if (!do_poisson) return;
     for (uint32_t l = 0; l < npix; l++) {
+        float v = I[l] * fac;
         double u[2];
-        ora_uniform2(seed, frame, l, u);
-        intensity[l] = (float)ora_poisson_sample((double)intensity[l], u);
+        if (do_poisson) {
+            ora_uniform2(seed, frame, l, 0, u);
+            v = (float)ora_poisson_sample((double)v, u);
+        }
+        if (adu_noise != 0.0f) {
+            ora_uniform2(seed, frame, l, 1, u);
+            v = (float)((double)v + (2.0 * u[0] - 1.0) * (double)adu_noise);
+        }
+        I[l] = v * value_per_photon;
     }
 }

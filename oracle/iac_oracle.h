@@ -146,17 +146,17 @@ void ora_simulate_fp64(const float *pixmap, uint32_t npix, uint32_t p0, uint32_t
 
 /* ---------------- Poisson: ArrayOperators.h:25-55 on a Philox4x32-10 stream ---------------- */
 void ora_philox4x32_10(const uint32_t ctr[4], const uint32_t key[2], uint32_t out[4]);
-/* Uniform pair for (seed, frame, pixel): the two 53-bit doubles in [0,1). */
-void ora_uniform2(uint64_t seed, uint32_t frame, uint32_t pixel, double u[2]);
+/* Uniform pair for (seed, frame, pixel, stream): two 53-bit doubles in [0,1). */
+void ora_uniform2(uint64_t seed, uint32_t frame, uint32_t pixel, uint32_t stream, double u[2]);
 /* Own exp / log / cos (range reduction + polynomial) used by the sampler. */
 double ora_exp(double x);
 double ora_log(double x);
 double ora_cos(double x);
 uint32_t ora_poisson_sample(double mean, const double u[2]);
-/* Rescale to n_expected photons (Simulator.cpp:870-878) then Poisson sample
- * (ArrayOperators.h:203-210) one frame of float intensities in place. */
-void ora_rescale_poisson(float *intensity, uint32_t npix, double n_expected, int do_poisson,
-                         uint64_t seed, uint32_t frame);
+/* Rescale to n_expected photons (Simulator.cpp:870-878), Poisson sample
+ * (ArrayOperators.h:203-210), white noise, value per photon; one frame in place. */
+void ora_sim_postprocess(float *intensity, uint32_t npix, double n_expected, int do_poisson, float adu_noise,
+                         float value_per_photon, uint64_t seed, uint32_t frame);
 
 #ifdef __cplusplus
 }

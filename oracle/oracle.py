@@ -86,12 +86,12 @@ _SIG = {
     "ora_simulate_fp64": (None, [F, U32, U32, U32, D, U32, D, U32, U32, C.c_double, I32, D, D, I32,
                                  D, I32]),
     "ora_philox4x32_10": (None, [C.POINTER(C.c_uint32), C.POINTER(C.c_uint32), C.POINTER(C.c_uint32)]),
-    "ora_uniform2": (None, [C.c_uint64, U32, U32, D]),
+    "ora_uniform2": (None, [C.c_uint64, U32, U32, U32, D]),
     "ora_exp": (C.c_double, [C.c_double]),
     "ora_log": (C.c_double, [C.c_double]),
     "ora_cos": (C.c_double, [C.c_double]),
     "ora_poisson_sample": (C.c_uint32, [C.c_double, D]),
-    "ora_rescale_poisson": (None, [F, U32, C.c_double, I32, C.c_uint64, U32]),
+    "ora_sim_postprocess": (None, [F, U32, C.c_double, I32, C.c_float, C.c_float, C.c_uint64, U32]),
 }
 
 _lib = None
@@ -330,3 +330,54 @@ def merge_f64(small_cq, small_shape, rw10, big=None):
         big = np.zeros(s.size_ab ** 3, np.float64)
     lib().ora_merge_f64(dp(cq), C.byref(s), fp(_f(rw10).reshape(10)), dp(big))
     return big
+
+
+# ---- simulator -----------------------------------------------------------------
+def simulate_fp64(pixmap, emitters, phases, wavelength, subsampling=0, u_step=(0, 0, 0), v_step=(0, 0, 0),
+                  out_kind=0, modes=1, pixels=None, frames=None, nthreads=0):
+    """FP64 speckle sum.  phases is [nframes * modes][nem] (mode-minor); the
+    result [nf][np] sums the subpixel-averaged |psi| (out_kind 0) or |psi|^2
+    (out_kind 1) over the modes of each frame.  pixels / frames = (start, count)."""
+    pm = _f(pixmap).reshape(-1, 3)
+    em = np.ascontiguousarray(emitters, np.float64).reshape(-1, 3)
+    ph = np.ascontiguousarray(phases, np.float64).reshape(-1, em.shape[0])
+    nfr = ph.shape[0] // modes
+    p0, npx = pixels if pixels is not None else (0, pm.shape[0])
+    f0, nf = frames if frames is not None else (0, nfr)
+    us = np.ascontiguousarray(u_step, np.float64)
+    vs = np.ascontiguousarray(v_step, np.float64)
+    out = np.empty((nf * modes, npx), np.float64)
+    lib().ora_simulate_fp64(fp(pm), pm.shape[0], p0, npx, dp(em), em.shape[0], dp(ph), f0 * modes, nf * modes,
+                            wavelength, subsampling, dp(us), dp(vs), out_kind, dp(out), nthreads)
+    return out.reshape(nf, modes, npx).sum(axis=1)
+
+
+def philox4x32_10(ctr, key):
+    c = (C.c_uint32 * 4)(*ctr)
+    k = (C.c_uint32 * 2)(*key)
+    o = (C.c_uint32 * 4)()
+    lib().ora_philox4x32_10(c, k, o)
+    return list(o)
+
+
+def uniform2(seed, frame, pixel, stream=0):
+    u = np.empty(2, np.float64)
+    lib().ora_uniform2(seed, frame, pixel, stream, dp(u))
+    return u
+
+
+def poisson_sample(mean, u):
+    return int(lib().ora_poisson_sample(float(mean), dp(np.ascontiguousarray(u, np.float64))))
+
+
+def sim_postprocess(intensity, n_expected, do_poisson=True, adu_noise=0.0, value_per_photon=1.0, seed=0,
+                    frame0=0):
+    """Rescale + Poisson + noise + value per photon on [nframes][npix] float
+    frames (a copy); frame f uses counter frame0 + f."""
+    I = np.array(intensity, np.float32, copy=True, order="C")
+    I2 = I.reshape(-1, I.shape[-1])
+    for f in range(I2.shape[0]):
+        row = I2[f]
+        lib().ora_sim_postprocess(fp(row), row.size, n_expected, int(do_poisson), adu_noise, value_per_photon,
+                                  seed, frame0 + f)
+    return I
