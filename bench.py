@@ -1,5 +1,6 @@
 #!/usr/bin/env python3
-"""Benchmark of the sparse 3D pixel-pair accumulation on one MI355X.
+"""Benchmark of the sparse 3D pixel-pair accumulation and of the speckle
+simulator on one MI355X.
 
 Workload (the sparse photon-list configuration): flat 1024x1024 detector,
 75 um pixels at 0.5 m; 10,000 frames of Poisson(0.002) photons per pixel
@@ -10,6 +11,13 @@ CPU path's fixed point.  One step = zero the mesh + one pass over all frames
 (iac_ac3d_sparse_batch).  Inputs are resident in HBM before timing starts.
 
 metric: ordered pair-updates (pairs within Max_Q, self pairs once) per second.
+
+Simulator leg (the "simulator" object of the JSON line): 512x512 detector with
+the same geometry, 10,000 emitters uniform in a 500 A cube, lambda 1.5 A, 1,000
+frames of uniform random phases; one step = the complex GEMM field (split bf16
+or fp16) -> subpixel |psi| -> rescale to 0.01 photons/pixel -> Poisson, for all
+frames; frames/s next to the FP32 direct sum, errors against the FP64 oracle on
+a pixel/frame subset.
 """
 from __future__ import annotations
 
@@ -46,6 +54,13 @@ def parse():
     ap.add_argument("--check", action="store_true", help="bit-exact check of a frame subset")
     ap.add_argument("--kernel", type=int, default=0, help="0 auto, 1 global-atomic tiles, 2 LDS boxes")
     ap.add_argument("--no-tune", action="store_true", help="skip the cell-size / frame-chunk sweep")
+    ap.add_argument("--cell-vox", type=int, default=0, help="fix the LDS cell side (skips the sweep)")
+    ap.add_argument("--frame-chunk", type=int, default=0, help="fix the LDS frames per block (skips the sweep)")
+    ap.add_argument("--no-sim", action="store_true", help="skip the simulator leg")
+    ap.add_argument("--sim-only", action="store_true", help="run only the simulator leg (prints its JSON object)")
+    ap.add_argument("--sim-frames", type=int, default=1000)
+    ap.add_argument("--sim-steps", type=int, default=3)
+    ap.add_argument("--sim-direct-frames", type=int, default=8)
     return ap.parse_args()
 
 
@@ -63,6 +78,9 @@ def main():
 
     dev = torch.device("cuda:0")
     torch.cuda.set_device(dev)
+    if args.sim_only:
+        print(json.dumps(sim_leg(args, iac, W, dev)), flush=True)
+        return
     t0 = time.time()
     _, kmap, max_q = W.detector(1024)
     shape = iac.mesh_shape_big(max_q, 255)
@@ -89,7 +107,11 @@ def main():
 
     ctx.set_option(iac.OPT_AC3D_KERNEL, args.kernel)
     tuned = None
-    if args.kernel != 1 and not args.no_tune:
+    if args.cell_vox or args.frame_chunk:
+        ctx.set_option(iac.OPT_CELL_VOX, args.cell_vox)
+        ctx.set_option(iac.OPT_FRAME_CHUNK, args.frame_chunk)
+        tuned = {"cell_vox": args.cell_vox, "frame_chunk": args.frame_chunk, "fixed": True}
+    elif args.kernel != 1 and not args.no_tune:
         tuned = autotune(ctx, iac, d_kmap, d_pix, d_cnt, data, shape, d_mesh, dev)
 
     def step():
@@ -128,6 +150,12 @@ def main():
     if not args.no_cpu:
         cpu = cpu_baseline(kmap, data, shape, ctx, iac, d_kmap, dev, args)
 
+    sim = None
+    if not args.no_sim:
+        del d_pix, d_cnt, d_off
+        torch.cuda.empty_cache()
+        sim = sim_leg(args, iac, W, dev)
+
     out = {
         "metric": "pixel-pair updates/sec into q-mesh (sparse 1024^2, 257^3 uint64, bit-exact)",
         "value": value,
@@ -163,6 +191,7 @@ def main():
             "bytes_per_update": BYTES_PER_UPDATE,
         },
         "cpu_baseline": cpu,
+        "simulator": sim,
     }
     print(json.dumps(out), flush=True)
     ctx.close()
@@ -259,6 +288,119 @@ def cpu_baseline(kmap, data, shape, ctx, iac, d_kmap, dev, args):
         "kind": "port",
         "sample": f"first {nf} of the {len(off) - 1} frames ({upd} pair-updates), "
                   f"OpenMP {cores} threads, -O3 -fopenmp -ffp-contract=off",
+    }
+
+
+MFMA_PEAK_TFLOPS = {"bf16": 2500.0, "fp16": 2500.0}  # MI355X dense, public spec
+
+
+def sim_leg(args, iac, W, dev):
+    """Speckle simulator: 512^2 pixels x 10,000 emitters x sim_frames frames."""
+    import torch
+
+    sys.path.insert(0, str(ROOT / "oracle"))
+    import oracle as O
+
+    n, nem, nf, wl = 512, 10000, args.sim_frames, 1.5
+    npix = n * n
+    pm, _, _ = W.detector(n)
+    em = W.emitters_cube(nem, 500.0, args.seed)
+    ph = W.emitter_phases(nf, nem, args.seed + 1)
+    d_pm = torch.from_numpy(pm).to(dev)
+    d_em = torch.from_numpy(em).to(dev)
+    d_ph = torch.from_numpy(ph).to(dev)
+    out = torch.empty((nf, npix), dtype=torch.float32, device=dev)
+    ctx = iac.Context(0)
+    # FP64 oracle on 8 central detector rows x 4 frames
+    p0, npx, nfe = (n // 2 - 4) * n, 8 * n, min(4, nf)
+    t = time.perf_counter()
+    ref = {k: O.simulate_fp64(pm, em, ph, wl, out_kind=k, pixels=(p0, npx), frames=(0, nfe)) for k in (0, 1)}
+    log(f"[sim] FP64 oracle subset ({npx} px x {nfe} frames x {nem} emitters, both kinds) "
+        f"in {time.perf_counter() - t:.1f}s")
+
+    def rel(a, b):
+        return float(np.linalg.norm(a - b) / np.linalg.norm(b))
+
+    res = {}
+    flops = 3 * 2.0 * (2 * nf) * npix * (2 * nem)  # three real GEMM products per step
+    for mode, name, dt in ((iac.SIM_BF16X3, "bf16x3", "bf16"), (iac.SIM_FP16X3, "fp16x3", "fp16")):
+        torch.cuda.synchronize()
+        t = time.perf_counter()
+        plan = ctx.sim_plan(d_pm, d_em, wl, mode)
+        torch.cuda.synchronize()
+        plan_s = time.perf_counter() - t
+        errs = {}
+        for k, kname in ((0, "amplitude"), (1, "intensity")):
+            ctx.sim_run(plan, d_ph, out, 1, k)
+            got = out[:nfe, p0:p0 + npx].cpu().numpy().astype(np.float64)
+            errs[kname] = rel(got, ref[k])
+
+        def step():
+            ctx.sim_run(plan, d_ph, out, 1, iac.SIM_AMPLITUDE)
+            ctx.sim_postprocess(out, 0.01 * npix, True, seed=args.seed)
+
+        step()
+        torch.cuda.synchronize()
+        ctx.enable_timing(True)
+        ctx.kernel_time(reset=True)
+        t1 = time.perf_counter()
+        for _ in range(args.sim_steps):
+            step()
+        torch.cuda.synchronize()
+        t2 = time.perf_counter()
+        gms, gl = ctx.kernel_time(reset=True)
+        ctx.enable_timing(False)
+        gemm_ms = gms / max(gl, 1)
+        ms = (t2 - t1) * 1e3 / args.sim_steps
+        ach = flops / (gemm_ms * 1e-3) / 1e12
+        photons = float(out.sum().item()) / nf
+        res[name] = {
+            "frames_per_s": nf / (ms * 1e-3), "ms_per_step": ms, "gemm_ms": gemm_ms, "plan_s": plan_s,
+            "err_vs_fp64": errs, "photons_per_frame": photons,
+            "roofline": {"bound": "mfma", "achieved": ach, "peak": MFMA_PEAK_TFLOPS[dt], "unit": "TFLOP/s",
+                         "frac": ach / MFMA_PEAK_TFLOPS[dt], "traffic": None,
+                         "kernel": f"rocBLAS gemm_ex {dt} x3 + epilogue",
+                         "flops_per_step": flops},
+        }
+        log(f"[sim] {name}: {ms:.2f} ms/step ({nf / (ms * 1e-3):.1f} frames/s), GEMM region {gemm_ms:.2f} ms "
+            f"= {ach:.0f} TFLOP/s, plan {plan_s:.2f}s, err amp {errs['amplitude']:.2e} "
+            f"int {errs['intensity']:.2e}, {photons:.1f} photons/frame")
+        plan.destroy()
+        del plan
+        torch.cuda.empty_cache()
+
+    # the plain FP32 direct sum (reference arithmetic, float positions and phases)
+    nd = min(args.sim_direct_frames, nf)
+    em32 = em.astype(np.float32)
+    ph32 = np.ascontiguousarray(ph[:nd]).astype(np.float32)
+    d_em32 = torch.from_numpy(em32).to(dev)
+    d_ph32 = torch.from_numpy(ph32).to(dev)
+    od = torch.empty((nd, npix), dtype=torch.float32, device=dev)
+    ctx.sim_direct_fp32(d_pm, d_em32, d_ph32[:1], wl, od[:1])
+    torch.cuda.synchronize()
+    t1 = time.perf_counter()
+    ctx.sim_direct_fp32(d_pm, d_em32, d_ph32, wl, od)
+    torch.cuda.synchronize()
+    dms = (time.perf_counter() - t1) * 1e3
+    nfe32 = min(nfe, nd)
+    ref32 = O.simulate_fp64(pm, em32.astype(np.float64), ph32.astype(np.float64), wl, out_kind=0,
+                            pixels=(p0, npx), frames=(0, nfe32))
+    e32 = rel(od[:nfe32, p0:p0 + npx].cpu().numpy().astype(np.float64), ref32)
+    direct_fps = nd / (dms * 1e-3)
+    log(f"[sim] fp32 direct: {dms / nd:.2f} ms/frame ({direct_fps:.1f} frames/s), err amp {e32:.2e}")
+    ctx.close()
+    best = res["bf16x3"]["frames_per_s"]
+    return {
+        "workload": f"{n}x{n} detector (75 um, 0.5 m), {nem} emitters uniform in a 500 A cube, lambda {wl} A, "
+                    f"{nf} frames of uniform phases, |psi| output, rescale to 0.01 photons/pixel + Poisson",
+        "unit": "frames/s",
+        "value": best,
+        "modes": res,
+        "fp32_direct": {"frames_per_s": direct_fps, "ms_per_frame": dms / nd, "frames_timed": nd,
+                        "err_vs_fp64": {"amplitude": e32}},
+        "speedup_bf16x3_vs_fp32_direct": best / direct_fps,
+        "speedup_fp16x3_vs_fp32_direct": res["fp16x3"]["frames_per_s"] / direct_fps,
+        "err_subset": f"pixels [{p0}, {p0 + npx}) (8 central rows) x frames [0, {nfe}), normwise relative",
     }
 
 

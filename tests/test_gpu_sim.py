@@ -70,13 +70,15 @@ def test_subsampling_modes_and_chunks(ctx, dev):
     ref = O.simulate_fp64(pm, em, ph, WL, subsampling=1, u_step=np.array(us), v_step=np.array(vs), modes=2)
     got = _gemm(ctx, dev, pm, em, ph, 1, modes=2, subsampling=1, us=us, vs=vs)
     assert _rel(got, ref) <= 1e-3
-    # many GEMM chunks (rows rounded down to whole pixels) give the same field
+    # many GEMM chunks (rows rounded down to whole pixels): the same field up to the
+    # FP32 summation order, which the GEMM library may pick per problem size
     ctx.set_option(iac.OPT_SIM_CHUNK_ROWS, 100)
     try:
         got2 = _gemm(ctx, dev, pm, em, ph, 1, modes=2, subsampling=1, us=us, vs=vs)
     finally:
         ctx.set_option(iac.OPT_SIM_CHUNK_ROWS, 0)
-    assert np.array_equal(got, got2)
+    assert _rel(got2, ref) <= 1e-3
+    assert _rel(got2, got) <= 1e-5
 
 
 def test_direct_fp32_vs_fp64(ctx, dev):
@@ -124,7 +126,7 @@ def test_postprocess_bit_exact(ctx, dev, npix, poisson, noise, vpp):
 
 
 def test_postprocess_counts_after_gemm(ctx, dev):
-    """The cfg-style chain at small size: GEMM field -> intensity -> 0.01 photons / pixel."""
+    """The simulation chain at small size: GEMM field -> rescale to 0.01 photons / pixel -> Poisson."""
     import torch
 
     pm, em, ph = _setup(64, 256, 16, seed=21)
