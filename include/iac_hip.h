@@ -151,6 +151,25 @@ int iac_sparse_hit_list(iac_ctx *ctx, const float *d_kmap, uint32_t npix, const 
                         uint32_t nframes, const int *d_mask, float threshold, float step,
                         float *d_rec4, uint64_t *d_frame_off, uint64_t *n_total);
 
+/* ---- averaged intensity -------------------------------------------------------- */
+/* Detector::LoadAndAverageIntensity(Events, Threshold, PhotonSamplingStep, Lower, Upper,
+ * true), IAC/Detector.cpp:687-790, over nframes dense frames d_frames ([nframes][npix]
+ * floats, already the events [Lower, Upper)): mask multiply (d_mask [npix] ints or NULL);
+ * step > 0: photon discretisation, per-pixel long tally; else values >= threshold summed
+ * in float, frames in order; then x (1.0f / nframes) into d_avg [npix].  Per event
+ * (optional, device arrays of nframes): MeanIntensity = serial float sum / npix and,
+ * when step > 0, PhotonCount = (int) sum. */
+int iac_average_intensity(iac_ctx *ctx, const float *d_frames, uint32_t nframes, uint32_t npix, const int *d_mask,
+                          float threshold, float step, float *d_avg, float *d_mean_intensity, int *d_photon_count);
+/* The same over consecutive frame stacks: each call continues the per-pixel sequence in
+ * d_sum (float, step <= 0) or d_tally (int64, step > 0), both [npix], zeroed by the caller
+ * before the first stack; iac_intensity_finish then writes d_avg for ntotal_frames frames. */
+int iac_intensity_accumulate(iac_ctx *ctx, const float *d_frames, uint32_t nframes, uint32_t npix, const int *d_mask,
+                             float threshold, float step, float *d_sum, int64_t *d_tally, float *d_mean_intensity,
+                             int *d_photon_count);
+int iac_intensity_finish(iac_ctx *ctx, uint32_t npix, uint32_t ntotal_frames, float step, const float *d_sum,
+                         const int64_t *d_tally, float *d_avg);
+
 /* ---- AC1D: angular-averaged correlation ---------------------------------------- */
 /* AC1D::Initialize(Det, ArraySize, QZoom), IAC/AC1D.cpp:53-58 (host). */
 int iac_ac1d_shape(const float max_q[3], uint32_t size, float q_zoom, float *max_q_out, float *dq_out);
@@ -170,6 +189,12 @@ int iac_ac1d_acuw(iac_ctx *ctx, const float *d_kmap, uint32_t npix, const uint32
                   const uint16_t *d_cnt, const uint64_t *d_frame_off, uint32_t nframes,
                   uint64_t n_total, uint32_t size, float max_q, float dq, int interp,
                   int self_pair_factor, double *d_hist);
+/* The same over ready-made hit-list records {kx,ky,kz,value} (e.g. from
+ * iac_sparse_hit_list with the evaluation's Photonisation, IAC/AC1D.cpp:246-249);
+ * frame f owns records [d_frame_off[f], d_frame_off[f+1]). */
+int iac_ac1d_acuw_records(iac_ctx *ctx, const float *d_rec4, const uint64_t *d_frame_off, uint32_t nframes,
+                          uint64_t n_total, uint32_t size, float max_q, float dq, int interp, int self_pair_factor,
+                          double *d_hist);
 /* AC1D::CalcAC, IAC/AC1D.cpp:482-493: ac = ac_uw / cq, NaN -> 0. */
 int iac_ac1d_calc_ac(iac_ctx *ctx, const double *d_ac_uw, const double *d_cq, uint32_t size,
                      double *d_ac);

@@ -259,6 +259,20 @@ class Context:
     def merge_ac_cq(self, ac, cq, out) -> None:
         check(lib().iac_merge_ac_cq(self._h, _ptr(ac), _ptr(cq), ac.numel(), _ptr(out)), "iac_merge_ac_cq")
 
+    def ac1d_acuw_records(self, rec4, frame_off, n_total: int, size: int, max_q: float, dq: float, interp: int,
+                          hist, self_pair_factor: int = 2) -> None:
+        nframes = frame_off.numel() - 1
+        check(lib().iac_ac1d_acuw_records(self._h, _ptr(rec4), _ptr(frame_off), nframes, n_total, size, max_q, dq,
+                                          interp, self_pair_factor, _ptr(hist)), "iac_ac1d_acuw_records")
+
+    def average_intensity(self, frames, mask, threshold: float, step: float, avg, mean_intensity=None,
+                          photon_count=None) -> None:
+        """Detector::LoadAndAverageIntensity over [nframes][npix] device frames."""
+        nframes, npix = frames.shape
+        check(lib().iac_average_intensity(self._h, _ptr(frames), nframes, npix, _ptr(mask), threshold, step,
+                                          _ptr(avg), _ptr(mean_intensity), _ptr(photon_count)),
+              "iac_average_intensity")
+
     # ---- simulator (Simulator::SimulatePart, IAC/Simulator.cpp:662-944) ----
     def sim_plan(self, pixmap, emitters, wavelength: float, mode: int = SIM_BF16X3, subsampling: int = 0,
                  u_step=(0.0, 0.0, 0.0), v_step=(0.0, 0.0, 0.0)) -> "SimPlan":

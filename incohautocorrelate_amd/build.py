@@ -20,6 +20,7 @@ HOST = PKG / "host"
 BUILD = PKG.parent / "_build"
 LIB = PKG / "libiac_hip.so"
 HOST_LIB = PKG / "libiac_host.so"
+EVAL_BIN = PKG / "iac_eval"
 
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 
@@ -71,20 +72,22 @@ def build_hip(verbose: bool = True) -> Path:
 
 
 def build_host(verbose: bool = True) -> Path | None:
-    """C++ mirror of the reference host classes, linked against libiac_hip.so."""
-    srcs = sorted(HOST.glob("*.cpp"))
+    """C++ mirror of the reference host classes (libiac_host.so, linked against
+    libiac_hip.so) and the iac_eval driver of RunIAC::Run_AutoCorr_DataEval."""
+    srcs = sorted(p for p in HOST.glob("*.cpp") if not p.name.endswith("_main.cpp"))
     if not srcs:
         return None
+    common = ["-O3", "-std=c++17", "-Wall", "-ffp-contract=off", "-I", str(PKG.parent / "include"), "-I", str(HOST)]
     tmp = HOST_LIB.with_suffix(".so.tmp")
-    _run([
-        "g++", "-O3", "-fPIC", "-shared", "-std=c++17", "-fopenmp", "-ffp-contract=off",
-        "-I", str(PKG.parent / "include"), "-I", str(HOST),
-        "-o", str(tmp), *map(str, srcs),
-        "-L", str(PKG), "-liac_hip", "-Wl,-rpath,$ORIGIN",
-    ])
+    _run(["g++", *common, "-fPIC", "-shared", "-o", str(tmp), *map(str, srcs),
+          "-L", str(PKG), "-liac_hip", "-Wl,-rpath,$ORIGIN"])
     os.replace(tmp, HOST_LIB)
+    tmp = EVAL_BIN.with_suffix(".tmp")
+    _run(["g++", *common, "-o", str(tmp), str(HOST / "iac_eval_main.cpp"),
+          "-L", str(PKG), "-liac_host", "-liac_hip", "-Wl,-rpath,$ORIGIN"])
+    os.replace(tmp, EVAL_BIN)
     if verbose:
-        print(f"built {HOST_LIB}")
+        print(f"built {HOST_LIB} and {EVAL_BIN}")
     return HOST_LIB
 
 

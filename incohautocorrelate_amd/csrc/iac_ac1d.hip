@@ -260,7 +260,47 @@ unsigned persistent_blocks(iac_ctx *ctx) { return (unsigned)std::max(1, ctx->num
 
 }  // namespace
 
+namespace {
+
+// Shared tail of the AC_uw entry points: frame tiles, then the histogram
+// kernel (uint64 bins when every weight is an integer, else double bins).
+int acuw_run(iac_ctx *ctx, const float4 *rec, const uint64_t *d_frame_off, uint32_t nframes, uint32_t size,
+             float max_q, float dq, int interp, int self_pair_factor, bool exact, double *d_hist)
+{
+    if (size == 0 || size > (uint32_t)kMaxBins) return iac::fail(IAC_ERR_LIMIT, "AC1D size must be 1..6144 bins");
+    if (interp != IAC_NEAREST && interp != IAC_LINEAR) return iac::fail(IAC_ERR_INVALID_ARG, "bad interpolation");
+    if (self_pair_factor != 1 && self_pair_factor != 2)
+        return iac::fail(IAC_ERR_INVALID_ARG, "self_pair_factor is 1 or 2");
+    void *s1 = nullptr;
+    IAC_TRY(iac::scratch(ctx, 1, (size_t)(nframes + 1) * sizeof(uint64_t) + (size_t)size * 8, &s1));
+    auto *toff = static_cast<uint64_t *>(s1);
+    auto *hu = reinterpret_cast<unsigned long long *>(static_cast<char *>(s1) + (size_t)(nframes + 1) * 8);
+    hipLaunchKernelGGL(tile_scan_kernel, dim3(1), dim3(1024), 0, ctx->stream, d_frame_off, nframes, toff);
+    IAC_HIP(hipGetLastError());
+    const float sf = (float)self_pair_factor;
+    IAC_TRY(iac::timer_begin(ctx));
+    if (exact) {
+        hipLaunchKernelGGL(zero_u64_kernel, dim3(1), dim3(256), 0, ctx->stream, hu, size);
+        hipLaunchKernelGGL(ac1d_acuw_kernel<true>, dim3(persistent_blocks(ctx)), dim3(kBlock), 0, ctx->stream, rec,
+                           d_frame_off, toff, nframes, size, max_q, dq, interp, sf, hu, d_hist, ctx->d_counters);
+    } else {
+        hipLaunchKernelGGL(ac1d_acuw_kernel<false>, dim3(persistent_blocks(ctx)), dim3(kBlock), 0, ctx->stream, rec,
+                           d_frame_off, toff, nframes, size, max_q, dq, interp, sf, hu, d_hist, ctx->d_counters);
+    }
+    IAC_HIP(hipGetLastError());
+    IAC_TRY(iac::timer_end(ctx));
+    if (exact) {
+        hipLaunchKernelGGL(add_u64_as_f64_kernel, dim3(std::max(1u, (size + 255) / 256)), dim3(256), 0, ctx->stream,
+                           hu, size, d_hist);
+        IAC_HIP(hipGetLastError());
+    }
+    return IAC_OK;
+}
+
+}  // namespace
+
 extern "C" {
+
 
 int iac_ac1d_shape(const float max_q[3], uint32_t size, float q_zoom, float *max_q_out, float *dq_out)
 {
@@ -312,42 +352,30 @@ int iac_ac1d_acuw(iac_ctx *ctx, const float *d_kmap, uint32_t npix, const uint32
 {
     IAC_TRY(iac::bind(ctx));
     if (!d_kmap || !d_frame_off || !d_hist || npix == 0) return iac::fail(IAC_ERR_INVALID_ARG, "null input");
-    if (size == 0 || size > (uint32_t)kMaxBins) return iac::fail(IAC_ERR_LIMIT, "AC1D size must be 1..6144 bins");
-    if (interp != IAC_NEAREST && interp != IAC_LINEAR) return iac::fail(IAC_ERR_INVALID_ARG, "bad interpolation");
-    if (self_pair_factor != 1 && self_pair_factor != 2)
-        return iac::fail(IAC_ERR_INVALID_ARG, "self_pair_factor is 1 or 2");
     if (nframes == 0 || n_total == 0) return IAC_OK;
     if (!d_pix || !d_cnt) return iac::fail(IAC_ERR_PRECONDITION, "no photon lists");
-    void *s0 = nullptr, *s1 = nullptr;
+    void *s0 = nullptr;
     IAC_TRY(iac::scratch(ctx, 0, n_total * sizeof(float4), &s0));
-    IAC_TRY(iac::scratch(ctx, 1, (size_t)(nframes + 1) * sizeof(uint64_t) + (size_t)size * 8, &s1));
     auto *rec = static_cast<float4 *>(s0);
-    auto *toff = static_cast<uint64_t *>(s1);
-    auto *hu = reinterpret_cast<unsigned long long *>(static_cast<char *>(s1) + (size_t)(nframes + 1) * 8);
     const unsigned gb = (unsigned)std::min<uint64_t>(65535, (n_total + 255) / 256);
     hipLaunchKernelGGL(gather_kernel, dim3(gb), dim3(256), 0, ctx->stream, d_kmap, npix, d_pix, d_cnt, n_total, rec,
                        ctx->d_counters + 1);
-    hipLaunchKernelGGL(tile_scan_kernel, dim3(1), dim3(1024), 0, ctx->stream, d_frame_off, nframes, toff);
     IAC_HIP(hipGetLastError());
-    const bool exact = interp == IAC_NEAREST;  // uint16 counts: every weight is an integer-valued float
-    const float sf = (float)self_pair_factor;
-    IAC_TRY(iac::timer_begin(ctx));
-    if (exact) {
-        hipLaunchKernelGGL(zero_u64_kernel, dim3(1), dim3(256), 0, ctx->stream, hu, size);
-        hipLaunchKernelGGL(ac1d_acuw_kernel<true>, dim3(persistent_blocks(ctx)), dim3(kBlock), 0, ctx->stream, rec,
-                           d_frame_off, toff, nframes, size, max_q, dq, interp, sf, hu, d_hist, ctx->d_counters);
-    } else {
-        hipLaunchKernelGGL(ac1d_acuw_kernel<false>, dim3(persistent_blocks(ctx)), dim3(kBlock), 0, ctx->stream, rec,
-                           d_frame_off, toff, nframes, size, max_q, dq, interp, sf, hu, d_hist, ctx->d_counters);
-    }
-    IAC_HIP(hipGetLastError());
-    IAC_TRY(iac::timer_end(ctx));
-    if (exact) {
-        hipLaunchKernelGGL(add_u64_as_f64_kernel, dim3(std::max(1u, (size + 255) / 256)), dim3(256), 0, ctx->stream,
-                           hu, size, d_hist);
-        IAC_HIP(hipGetLastError());
-    }
-    return IAC_OK;
+    // uint16 counts: every weight is an integer-valued float, exact in uint64 bins
+    return acuw_run(ctx, rec, d_frame_off, nframes, size, max_q, dq, interp, self_pair_factor,
+                    interp == IAC_NEAREST, d_hist);
+}
+
+int iac_ac1d_acuw_records(iac_ctx *ctx, const float *d_rec4, const uint64_t *d_frame_off, uint32_t nframes,
+                          uint64_t n_total, uint32_t size, float max_q, float dq, int interp, int self_pair_factor,
+                          double *d_hist)
+{
+    IAC_TRY(iac::bind(ctx));
+    if (!d_frame_off || !d_hist) return iac::fail(IAC_ERR_INVALID_ARG, "null input");
+    if (nframes == 0 || n_total == 0) return IAC_OK;
+    if (!d_rec4) return iac::fail(IAC_ERR_PRECONDITION, "no hit-list records");
+    return acuw_run(ctx, reinterpret_cast<const float4 *>(d_rec4), d_frame_off, nframes, size, max_q, dq, interp,
+                    self_pair_factor, false, d_hist);
 }
 
 int iac_ac1d_calc_ac(iac_ctx *ctx, const double *d_ac_uw, const double *d_cq, uint32_t size, double *d_ac)

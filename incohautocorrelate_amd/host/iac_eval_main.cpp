@@ -1,0 +1,114 @@
+// iac_eval_main.cpp -- command-line driver of RunIAC::Run_AutoCorr_DataEval on
+// raw binary inputs (the evaluation run mode of IAC/MainRunModes.cpp:1254-1330
+// without the XML / HDF5 readers):
+//   iac_eval --pixelmap PM --size SS FS --frames STACK --nframes N [options]
+// PM: float32 [SS][FS][3]; STACK: float32 [N][SS][FS]; --mask: int32 [SS][FS];
+// --rotations: float32 [N][9] row major (default identity).  Outputs are raw
+// float64 arrays.  Prints the IAC_Report as one JSON line.
+#include <cstdio>
+#include <cstdlib>
+#include <cstring>
+#include <iostream>
+#include <stdexcept>
+#include <string>
+#include <vector>
+
+#include "host_util.hpp"
+#include "iac_host.hpp"
+
+static void usage()
+{
+    std::cerr << "usage: iac_eval --pixelmap PM --size SS FS --frames STACK --nframes N\n"
+                 "  [--mask M] [--rotations R] [--mesh-size S] [--qzoom Z] [--1d]\n"
+                 "  [--photon-offset T] [--photon-step S] [--interp1 0|1] [--interp2 0|1] [--no-double-map]\n"
+                 "  [--bounds LO HI] [--invert-rot] [--ignore-orientation] [--mask-flatfield] [--batch B]\n"
+                 "  [--out-avint F] [--out-acuw F] [--out-cq F] [--out-cq-small F] [--out-final F] [--out-q F]\n"
+                 "  [--echo L] [--version]\n";
+}
+
+int main(int argc, char **argv)
+{
+    RunIAC::CreateDataEval_Settings E;
+    E.EchoLevel = 0;
+    std::string frames, rotations;
+    unsigned int nframes = 0, batch = 0;
+    for (int i = 1; i < argc; i++) {
+        const std::string a = argv[i];
+        auto next = [&]() -> const char * {
+            if (i + 1 >= argc) {
+                usage();
+                exit(2);
+            }
+            return argv[++i];
+        };
+        if (a == "--version") {
+            std::cout << "iac_eval abi " << iac_abi_version() << "\n";
+            return 0;
+        } else if (a == "--pixelmap") E.PixelMap_Path = next();
+        else if (a == "--size") {
+            E.DetectorSize[0] = (unsigned)std::stoul(next());
+            E.DetectorSize[1] = (unsigned)std::stoul(next());
+        } else if (a == "--frames") frames = next();
+        else if (a == "--nframes") nframes = (unsigned)std::stoul(next());
+        else if (a == "--mask") E.PixelMask_Path = next();
+        else if (a == "--rotations") rotations = next();
+        else if (a == "--mesh-size") E.MeshSize = std::stoi(next());
+        else if (a == "--qzoom") E.QZoom = std::stof(next());
+        else if (a == "--1d") E.AngularAveraged = true;
+        else if (a == "--photon-offset") E.PhotonOffset = std::stof(next());
+        else if (a == "--photon-step") E.PhotonStep = std::stof(next());
+        else if (a == "--interp1") E.AC_FirstMap_Flags.InterpolationMode = (Settings::Interpolation)std::stoi(next());
+        else if (a == "--interp2") E.AC_SecondMap_Flags.InterpolationMode = (Settings::Interpolation)std::stoi(next());
+        else if (a == "--no-double-map") E.DoubleMap = false;
+        else if (a == "--bounds") {
+            E.RestrictStackToBoundaries = true;
+            E.LowerBoundary = (unsigned)std::stoul(next());
+            E.UpperBoundary = (unsigned)std::stoul(next());
+        } else if (a == "--invert-rot") E.InvertRotMatrix = true;
+        else if (a == "--ignore-orientation") E.IgnoreOrientation = true;
+        else if (a == "--mask-flatfield") E.UsePixelMask_as_Flatfield = true;
+        else if (a == "--batch") batch = (unsigned)std::stoul(next());
+        else if (a == "--out-avint") E.Out_AvIntensity_Path = next();
+        else if (a == "--out-acuw") E.Out_ACuw_Path = next();
+        else if (a == "--out-cq") E.Out_Cq_Path = next();
+        else if (a == "--out-cq-small") E.Out_Cq_small_Path = next();
+        else if (a == "--out-final") E.Out_Final_AC_Path = next();
+        else if (a == "--out-q") E.Out_Q_Vector = next();
+        else if (a == "--echo") E.EchoLevel = std::stoi(next());
+        else {
+            usage();
+            return 2;
+        }
+    }
+    if (E.PixelMap_Path.empty() || frames.empty() || nframes == 0 || !E.DetectorSize[0]) {
+        usage();
+        return 2;
+    }
+    try {
+        Settings S;
+        S.echo = E.EchoLevel > 0;
+        if (batch) S.BatchFrames = batch;
+        std::vector<float> rot;
+        if (!rotations.empty()) {
+            rot.resize((size_t)nframes * 9);
+            iac_host::read_raw(rotations, rot.data(), rot.size() * 4);
+        }
+        for (unsigned int i = 0; i < nframes; i++) {
+            Settings::HitEvent ev;
+            ev.Filename = frames;
+            ev.Event = (int)i;
+            ev.SerialNumber = (int)i;
+            if (!rot.empty()) std::memcpy(ev.RotMatrix, rot.data() + 9 * (size_t)i, 36);
+            S.HitEvents.push_back(ev);
+        }
+        RunIAC::IAC_Report R = RunIAC::Run_AutoCorr_DataEval(S, E);
+        std::printf("{\"dQperVox\": %.9g, \"FinalMeshSize\": %u, \"SmallCqMeshSize\": [%u, %u, %u], "
+                    "\"MeanPhotonCount\": %.9g, \"LowerBound\": %u, \"UpperBound\": %u}\n",
+                    R.dQperVox, R.FinalMeshSize, R.SmallCqMeshSize[0], R.SmallCqMeshSize[1], R.SmallCqMeshSize[2],
+                    R.MeanPhotonCount, R.LowerBound, R.UpperBound);
+    } catch (const std::exception &ex) {
+        std::cerr << "ERROR: " << ex.what() << "\n";
+        return 1;
+    }
+    return 0;
+}

@@ -957,3 +957,51 @@ void ora_sim_postprocess(float *I, uint32_t npix, double n_expected, int do_pois
         I[l] = v * value_per_photon;
     }
 }
+
+/* ------------------------------------------------------------------------- */
+/* Detector::LoadAndAverageIntensity(Events, Threshold, PhotonSamplingStep, Lower,
+ * Upper, Pixelmask), IAC/Detector.cpp:687-790, over frames already loaded
+ * ([nframes][npix]): tmpIntensity *= mask (ParMultiplyElementwise); the three
+ * branches (no discretisation: ParAdd with threshold; threshold only: zero below,
+ * ParAdd; discretisation: DiscretizeToPhotones, long tally of floor(v + 0.5));
+ * MeanIntensity = Sum<float>(tmp) / (float)P, PhotonCount = (int)Sum (third branch
+ * only); Intensity = tally (third branch) x (1.0f / (float)N) (ParMultiplyScalar). */
+void ora_average_intensity(const float *frames, uint32_t nframes, uint32_t npix, const int *mask,
+                           float thr, float step, float *avg, float *mean_intensity, int *photon_count)
+{
+    float *tmp = (float *)malloc(sizeof(float) * npix);
+    long *discr = (long *)calloc(npix, sizeof(long));
+    for (uint32_t j = 0; j < npix; j++) avg[j] = 0.0f;
+    for (uint32_t f = 0; f < nframes; f++) {
+        for (uint32_t j = 0; j < npix; j++) {
+            tmp[j] = frames[(size_t)f * npix + j];
+            if (mask) tmp[j] = tmp[j] * (float)mask[j];
+        }
+        if (step <= 0) {
+            if (thr > 0)
+                for (uint32_t j = 0; j < npix; j++)
+                    if (tmp[j] < thr) tmp[j] = 0.0f;
+            for (uint32_t j = 0; j < npix; j++)
+                if (tmp[j] >= thr) avg[j] += tmp[j];
+        } else {
+            for (uint32_t j = 0; j < npix; j++) {
+                if (tmp[j] >= thr) {
+                    tmp[j] = ceilf((tmp[j] - thr) / step) * (float)(tmp[j] >= thr);
+                    discr[j] += (long)floor(tmp[j] + 0.5);
+                } else {
+                    tmp[j] = 0.0f;
+                }
+            }
+        }
+        float s = 0.0f;
+        for (uint32_t j = 0; j < npix; j++) s += tmp[j];
+        if (step > 0 && photon_count) photon_count[f] = (int)s;
+        if (mean_intensity) mean_intensity[f] = s / ((float)npix);
+    }
+    if (step > 0)
+        for (uint32_t j = 0; j < npix; j++) avg[j] = (float)discr[j];
+    float inv = 1.0f / ((float)nframes);
+    for (uint32_t j = 0; j < npix; j++) avg[j] *= inv;
+    free(tmp);
+    free(discr);
+}

@@ -158,6 +158,10 @@ uint32_t ora_poisson_sample(double mean, const double u[2]);
 void ora_sim_postprocess(float *intensity, uint32_t npix, double n_expected, int do_poisson, float adu_noise,
                          float value_per_photon, uint64_t seed, uint32_t frame);
 
+/* ---------------- averaged intensity: Detector.cpp:687-790 ---------------- */
+void ora_average_intensity(const float *frames, uint32_t nframes, uint32_t npix, const int *mask,
+                           float thr, float step, float *avg, float *mean_intensity, int *photon_count);
+
 #ifdef __cplusplus
 }
 #endif

@@ -91,6 +91,8 @@ _SIG = {
     "ora_log": (C.c_double, [C.c_double]),
     "ora_cos": (C.c_double, [C.c_double]),
     "ora_poisson_sample": (C.c_uint32, [C.c_double, D]),
+    "ora_average_intensity": (None, [F, U32, U32, C.POINTER(C.c_int), C.c_float, C.c_float, F, F,
+                                     C.POINTER(C.c_int)]),
     "ora_sim_postprocess": (None, [F, U32, C.c_double, I32, C.c_float, C.c_float, C.c_uint64, U32]),
 }
 
@@ -381,3 +383,17 @@ def sim_postprocess(intensity, n_expected, do_poisson=True, adu_noise=0.0, value
         lib().ora_sim_postprocess(fp(row), row.size, n_expected, int(do_poisson), adu_noise, value_per_photon,
                                   seed, frame0 + f)
     return I
+
+
+# ---- averaged intensity --------------------------------------------------------
+def average_intensity(frames, threshold, step, mask=None):
+    """(avg [npix], mean_intensity [nframes], photon_count [nframes] or -1 where not set)."""
+    fr = _f(frames)
+    nframes, npix = fr.shape
+    m = None if mask is None else np.ascontiguousarray(mask, np.int32)
+    avg = np.empty(npix, np.float32)
+    mi = np.empty(nframes, np.float32)
+    pc = np.full(nframes, -1, np.int32)
+    lib().ora_average_intensity(fp(fr), nframes, npix, None if m is None else m.ctypes.data_as(C.POINTER(C.c_int)),
+                                threshold, step, fp(avg), fp(mi), pc.ctypes.data_as(C.POINTER(C.c_int)))
+    return avg, mi, pc

@@ -61,3 +61,17 @@ def test_context_without_device():
     with pytest.raises(iac.IACError) as e:
         iac.Context(0)
     assert e.value.code == -3
+
+
+def test_host_library_and_eval_driver():
+    """The C++ host mirror links against the C ABI; the driver starts without a GPU."""
+    import ctypes
+    import subprocess
+    from pathlib import Path
+
+    pkg = Path(__file__).resolve().parents[1] / "incohautocorrelate_amd"
+    ctypes.CDLL(str(pkg / "libiac_host.so"))
+    r = subprocess.run([str(pkg / "iac_eval"), "--version"], capture_output=True, text=True, timeout=60)
+    assert r.returncode == 0 and "abi" in r.stdout
+    r = subprocess.run([str(pkg / "iac_eval")], capture_output=True, text=True, timeout=60)
+    assert r.returncode == 2 and "usage" in r.stderr

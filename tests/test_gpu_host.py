@@ -1,0 +1,158 @@
+"""The C++ host mirror end to end: iac_eval (RunIAC::Run_AutoCorr_DataEval over
+Settings / Detector / ACMesh / AC1D on libiac_host.so) on raw inputs, every output
+checked against the oracle: averaged intensity, small C(q), merged C(q), AC_uw
+and the final AC bit-exact (3D), the 1D branch to the histogram's rounding."""
+import json
+import subprocess
+from pathlib import Path
+
+import numpy as np
+import pytest
+
+import oracle as O
+from incohautocorrelate_amd import workloads as W
+
+pytestmark = pytest.mark.gpu
+
+EVAL = Path(__file__).resolve().parents[1] / "incohautocorrelate_amd" / "iac_eval"
+
+
+def _inputs(tmp, n=40, nframes=23, seed=0, rotate=True):
+    rng = np.random.default_rng(seed)
+    pm, km, mq = W.detector(n)
+    mask = (rng.random(n * n) > 0.08).astype(np.int32)
+    # photon-like frames: mostly zeros, a few pixels with 1-3 photons of ~6.4 ADU plus noise
+    ph = rng.poisson(0.03, (nframes, n * n))
+    frames = (ph * 6.4 + rng.normal(0, 0.8, ph.shape) * (ph > 0)).astype(np.float32)
+    rots = W.random_rotations(nframes, seed + 1) if rotate else np.tile(np.eye(3, dtype=np.float32).reshape(9),
+                                                                         (nframes, 1))
+    rots[0] = np.eye(3, dtype=np.float32).reshape(9)
+    files = {"pm": tmp / "pm.bin", "mask": tmp / "mask.bin", "frames": tmp / "frames.bin", "rot": tmp / "rot.bin"}
+    pm.astype(np.float32).tofile(files["pm"])
+    mask.tofile(files["mask"])
+    frames.tofile(files["frames"])
+    rots.astype(np.float32).tofile(files["rot"])
+    return pm, km, mq, mask, frames, rots, files
+
+
+def _run(args, timeout=120):
+    r = subprocess.run([str(EVAL), *map(str, args)], capture_output=True, text=True, timeout=timeout)
+    assert r.returncode == 0, r.stderr + r.stdout
+    return json.loads(r.stdout.strip().splitlines()[-1])
+
+
+@pytest.mark.parametrize("interp1,interp2,rotate", [(0, 0, True), (1, 1, True), (0, 1, False)])
+def test_eval_3d_vs_oracle(tmp_path, interp1, interp2, rotate):
+    import incohautocorrelate_amd as iac
+
+    n, nf, edge, off, step = 40, 23, 31, 3.2, 6.4
+    pm, km, mq, mask, frames, rots, f = _inputs(tmp_path, n, nf, seed=interp1 + 2 * interp2, rotate=rotate)
+    out = {k: tmp_path / f"{k}.bin" for k in ("avint", "acuw", "cq", "cqs", "final")}
+    rep = _run(["--pixelmap", f["pm"], "--size", n, n, "--frames", f["frames"], "--nframes", nf,
+                "--mask", f["mask"], "--rotations", f["rot"], "--mesh-size", edge,
+                "--photon-offset", off, "--photon-step", step, "--interp1", interp1, "--interp2", interp2,
+                "--batch", 7,  # several device stacks
+                "--out-avint", out["avint"], "--out-acuw", out["acuw"], "--out-cq", out["cq"],
+                "--out-cq-small", out["cqs"], "--out-final", out["final"]])
+    # averaged intensity and per-event sums (Detector::LoadAndAverageIntensity)
+    avg, mean_i, pcount = O.average_intensity(frames, off, step, mask)
+    got_avg = np.fromfile(out["avint"], np.float32)
+    assert np.array_equal(got_avg, avg)
+    assert rep["MeanPhotonCount"] == pytest.approx(float(np.float32(pcount.astype(np.float32).sum()) / nf), rel=1e-6)
+    I = avg * mask.astype(np.float32)  # ApplyPixelMask
+    # small C(q)
+    ss = iac.mesh_shape_small_cofq(mq, edge)
+    mult = O.cq_small_multiplicator(I)
+    small_u, _ = O.cq_small(I, km, ss, interp1, mult)
+    small = small_u.astype(np.float64) / mult
+    assert np.array_equal(np.fromfile(out["cqs"], np.float64), small)
+    # merged C(q)
+    rw, m2 = O.merge_prepare(rots, mean_i, small)
+    big_u, _ = O.merge_cq(small, ss, rw, interp2, m2)
+    big = big_u.astype(np.float64) / m2
+    assert np.array_equal(np.fromfile(out["cq"], np.float64), big)
+    # AC_uw: every event sparsified with the photonisation and binned with its rotation
+    bs = iac.mesh_shape_big(mq, edge)
+    mesh = np.zeros(bs.size_ab ** 2 * bs.size_c, np.uint64)
+    for e in range(nf):
+        hl = O.sparse_hit_list(frames[e], km, off, step, mask)
+        mesh, _ = O.ac3d_sparse(hl, bs, rots[e], interp1, interp2, True, mesh=mesh)
+    acuw = (mesh.astype(np.float32) * np.float32(0.001) - np.float32(0.0)).astype(np.float64)
+    assert np.array_equal(np.fromfile(out["acuw"], np.float64), acuw)
+    assert mesh.sum() > 0
+    # final AC
+    assert np.array_equal(np.fromfile(out["final"], np.float64), O.merge_ac_cq(mesh, big))
+    assert rep["FinalMeshSize"] == bs.size_ab and rep["SmallCqMeshSize"] == [ss.size_ab, ss.size_ab, ss.size_c]
+
+
+def test_eval_bounds_and_flatfield(tmp_path):
+    """RestrictStackToBoundaries + UsePixelMask_as_Flatfield: C(q) of the mask itself."""
+    import incohautocorrelate_amd as iac
+
+    n, nf, edge = 32, 12, 25
+    pm, km, mq, mask, frames, rots, f = _inputs(tmp_path, n, nf, seed=9)
+    out = {k: tmp_path / f"{k}.bin" for k in ("acuw", "cqs")}
+    rep = _run(["--pixelmap", f["pm"], "--size", n, n, "--frames", f["frames"], "--nframes", nf,
+                "--mask", f["mask"], "--rotations", f["rot"], "--mesh-size", edge, "--bounds", 3, 9,
+                "--mask-flatfield", "--photon-offset", 3.2, "--photon-step", 6.4,
+                "--out-acuw", out["acuw"], "--out-cq-small", out["cqs"]])
+    assert rep["LowerBound"] == 3 and rep["UpperBound"] == 9
+    flat = mask.astype(np.float32)
+    ss = iac.mesh_shape_small_cofq(mq, edge)
+    mult = O.cq_small_multiplicator(flat)
+    small_u, _ = O.cq_small(flat, km, ss, 0, mult)
+    assert np.array_equal(np.fromfile(out["cqs"], np.float64), small_u.astype(np.float64) / mult)
+    bs = iac.mesh_shape_big(mq, edge)
+    mesh = np.zeros(bs.size_ab ** 2 * bs.size_c, np.uint64)
+    for e in range(3, 9):
+        mesh, _ = O.ac3d_sparse(O.sparse_hit_list(frames[e], km, 3.2, 6.4, mask), bs, rots[e], mesh=mesh)
+    got = np.fromfile(out["acuw"], np.float64)
+    assert np.array_equal(got, (mesh.astype(np.float32) * np.float32(0.001)).astype(np.float64))
+
+
+def test_eval_1d_vs_oracle(tmp_path):
+    n, nf, size, off, step = 40, 17, 301, 3.2, 6.4
+    pm, km, mq, mask, frames, rots, f = _inputs(tmp_path, n, nf, seed=4)
+    out = {k: tmp_path / f"{k}.bin" for k in ("cq", "acuw", "final", "q")}
+    rep = _run(["--pixelmap", f["pm"], "--size", n, n, "--frames", f["frames"], "--nframes", nf,
+                "--mask", f["mask"], "--mesh-size", size, "--1d", "--photon-offset", off, "--photon-step", step,
+                "--batch", 5, "--out-cq", out["cq"], "--out-acuw", out["acuw"], "--out-final", out["final"],
+                "--out-q", out["q"]])
+    avg, _, _ = O.average_intensity(frames, off, step, mask)
+    I = avg * mask.astype(np.float32)
+    max_q, dq = O.ac1d_shape(mq, size)
+    assert rep["dQperVox"] == pytest.approx(dq, rel=1e-7) and rep["FinalMeshSize"] == size
+    mult = O.ac1d_cq_multiplicator(I)
+    hist, _ = O.ac1d_cq(I, km, size, max_q, dq, 1, mult)
+    cq = hist.astype(np.float64) / mult
+    assert np.array_equal(np.fromfile(out["cq"], np.float64), cq)
+    acuw = np.zeros(size)
+    for e in range(nf):
+        acuw, _ = O.ac1d_acuw(O.sparse_hit_list(frames[e], km, off, step, mask), size, max_q, dq, 1, 2, ac=acuw)
+    got = np.fromfile(out["acuw"], np.float64)
+    assert np.allclose(got, acuw, rtol=1e-12, atol=1e-9)  # double atomics: order of the adds
+    fin = np.fromfile(out["final"], np.float64)
+    ref = O.ac1d_calc_ac(got, cq)
+    assert np.array_equal(fin, ref)
+    q = np.fromfile(out["q"], np.float64)
+    assert np.array_equal(q, (np.arange(size, dtype=np.float32) * np.float32(dq)).astype(np.float64))
+
+
+@pytest.mark.parametrize("thr,step,masked", [(3.2, 6.4, True), (1.0, 0.0, True), (-0.5, 0.0, False),
+                                             (0.0, 0.0, True)])
+def test_average_intensity_entry_point(ctx, dev, thr, step, masked):
+    import torch
+
+    from gpu_helpers import to_dev
+
+    rng = np.random.default_rng(int(thr * 10) + 3)
+    fr = (rng.poisson(0.4, (13, 3001)) * 6.4 + rng.normal(0, 1.0, (13, 3001))).astype(np.float32)
+    mask = (rng.random(3001) > 0.1).astype(np.int32) if masked else None
+    avg, mi, pc = O.average_intensity(fr, thr, step, mask)
+    d_avg = torch.zeros(3001, dtype=torch.float32, device=dev)
+    d_mi = torch.zeros(13, dtype=torch.float32, device=dev)
+    d_pc = torch.full((13,), -1, dtype=torch.int32, device=dev)
+    ctx.average_intensity(to_dev(fr, dev), None if mask is None else to_dev(mask, dev), thr, step, d_avg, d_mi, d_pc)
+    assert np.array_equal(d_avg.cpu().numpy(), avg)
+    assert np.array_equal(d_mi.cpu().numpy(), mi)
+    assert np.array_equal(d_pc.cpu().numpy(), pc)
