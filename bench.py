@@ -57,6 +57,8 @@ def parse():
     ap.add_argument("--cell-vox", type=int, default=0, help="fix the LDS cell side (skips the sweep)")
     ap.add_argument("--frame-chunk", type=int, default=0, help="fix the LDS frames per block (skips the sweep)")
     ap.add_argument("--no-sim", action="store_true", help="skip the simulator leg")
+    ap.add_argument("--no-uneven", action="store_true", help="skip the uneven-batch leg")
+    ap.add_argument("--uneven-only", action="store_true", help="run only the uneven-batch leg")
     ap.add_argument("--sim-only", action="store_true", help="run only the simulator leg (prints its JSON object)")
     ap.add_argument("--sim-frames", type=int, default=1000)
     ap.add_argument("--sim-steps", type=int, default=3)
@@ -80,6 +82,9 @@ def main():
     torch.cuda.set_device(dev)
     if args.sim_only:
         print(json.dumps(sim_leg(args, iac, W, dev)), flush=True)
+        return
+    if args.uneven_only:
+        print(json.dumps(uneven_leg(args, iac, W, dev)), flush=True)
         return
     t0 = time.time()
     _, kmap, max_q = W.detector(1024)
@@ -150,11 +155,10 @@ def main():
     if not args.no_cpu:
         cpu = cpu_baseline(kmap, data, shape, ctx, iac, d_kmap, dev, args)
 
-    sim = None
-    if not args.no_sim:
-        del d_pix, d_cnt, d_off
-        torch.cuda.empty_cache()
-        sim = sim_leg(args, iac, W, dev)
+    del d_pix, d_cnt, d_off, d_mesh
+    torch.cuda.empty_cache()
+    uneven = None if args.no_uneven else uneven_leg(args, iac, W, dev)
+    sim = None if args.no_sim else sim_leg(args, iac, W, dev)
 
     out = {
         "metric": "pixel-pair updates/sec into q-mesh (sparse 1024^2, 257^3 uint64, bit-exact)",
@@ -191,6 +195,7 @@ def main():
             "bytes_per_update": BYTES_PER_UPDATE,
         },
         "cpu_baseline": cpu,
+        "uneven_batch": uneven,
         "simulator": sim,
     }
     print(json.dumps(out), flush=True)
@@ -288,6 +293,79 @@ def cpu_baseline(kmap, data, shape, ctx, iac, d_kmap, dev, args):
         "kind": "port",
         "sample": f"first {nf} of the {len(off) - 1} frames ({upd} pair-updates), "
                   f"OpenMP {cores} threads, -O3 -fopenmp -ffp-contract=off",
+    }
+
+
+def uneven_leg(args, iac, W, dev):
+    """4,096 frames on the 1024^2 detector, photons/frame lognormal (median 1,500,
+    sigma 1, clipped to 10..20,000, count 1 at distinct pixels) into the 257^3
+    uint64 mesh: one prepared batch, its graph replayed per step (mesh cleared
+    inside the graph)."""
+    import torch
+
+    _, kmap, max_q = W.detector(1024)
+    shape = iac.mesh_shape_big(max_q, 255)
+    t0 = time.time()
+    data = W.lognormal_photon_frames(1024 * 1024, 4096, 1500, 1.0, 10, 20000, seed=0)
+    n_total = int(data["frame_off"][-1])
+    sizes = np.diff(data["frame_off"].astype(np.int64))
+    log(f"[uneven] {n_total} photons in {len(sizes)} frames (mean {sizes.mean():.0f}, max {sizes.max()}), "
+        f"generated in {time.time() - t0:.1f}s")
+    d_kmap = torch.from_numpy(kmap).to(dev)
+    d_pix = torch.from_numpy(data["pix"].view(np.int32)).to(dev)
+    d_cnt = torch.from_numpy(data["cnt"].view(np.int16)).to(dev)
+    d_off = torch.from_numpy(data["frame_off"].view(np.int64)).to(dev)
+    nvox = shape.size_ab ** 2 * shape.size_c
+    d_mesh = torch.zeros(nvox, dtype=torch.int64, device=dev)
+    ctx = iac.Context(0)
+    tuned = None
+    if not args.no_tune:
+        best = None
+        ctx.enable_timing(True)
+        for cv in (4, 6, 8):
+            for fc in (32, 64, 128):
+                ctx.set_option(iac.OPT_CELL_VOX, cv)
+                ctx.set_option(iac.OPT_FRAME_CHUNK, fc)
+                b = ctx.ac3d_batch(d_kmap, d_pix, d_cnt, d_off, n_total, shape)
+                b.launch(d_mesh, True)
+                ctx.kernel_time(reset=True)
+                b.launch(d_mesh, True)
+                ms, k = ctx.kernel_time(reset=True)
+                b.destroy()
+                log(f"[uneven tune] cell_vox={cv} frame_chunk={fc}: {ms:.2f} ms")
+                if best is None or ms < best[0]:
+                    best = (ms, cv, fc)
+        ctx.enable_timing(False)
+        ctx.set_option(iac.OPT_CELL_VOX, best[1])
+        ctx.set_option(iac.OPT_FRAME_CHUNK, best[2])
+        tuned = {"cell_vox": best[1], "frame_chunk": best[2]}
+    b = ctx.ac3d_batch(d_kmap, d_pix, d_cnt, d_off, n_total, shape)
+    b.capture(d_mesh, True)
+    b.launch(d_mesh, True)
+    torch.cuda.synchronize()
+    updates = int(d_mesh.sum().item()) // 1000  # counts are 1: val(1) = 1000 per update
+    steps = max(1, args.steps)
+    ctx.enable_timing(True)
+    ctx.kernel_time(reset=True)
+    torch.cuda.synchronize()
+    t1 = time.perf_counter()
+    for _ in range(steps):
+        b.launch(d_mesh, True)
+    torch.cuda.synchronize()
+    t2 = time.perf_counter()
+    gms, gl = ctx.kernel_time(reset=True)
+    ctx.enable_timing(False)
+    ms = (t2 - t1) * 1e3 / steps
+    rate = updates / (ms * 1e-3)
+    log(f"[uneven] {ms:.2f} ms/step (graph {gms / max(gl, 1):.2f} ms), {updates:.4e} updates, {rate:.4e} updates/s")
+    b.destroy()
+    ctx.close()
+    return {
+        "workload": "4,096 frames, 1024x1024 detector, photons/frame lognormal(median 1500, sigma 1) clipped "
+                    "10..20,000, count 1 at distinct pixels, 257^3 uint64 mesh, NN/NN double-map, identity R",
+        "value": rate, "unit": "pair-updates/s", "ms_per_step": ms, "graph_ms": gms / max(gl, 1),
+        "steps": steps, "photons": n_total, "pair_updates_per_step": updates, "tuned": tuned,
+        "launch": "hipGraph replay of one prepared batch (zero + gather + cell sort + pairs + mirror)",
     }
 
 

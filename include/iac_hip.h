@@ -140,6 +140,24 @@ int iac_ac3d_records_batch(iac_ctx *ctx, const float *d_kmap, uint32_t npix, con
                            const float *d_rots, const iac_mesh_shape *shape, int interp1, int interp2,
                            int double_map, uint64_t *d_mesh);
 
+/* A prepared frame batch for repeated passes (identity rotation, the LDS engine):
+ * the cell plan, the pair-work-balanced frame chunks and all buffers are made once;
+ * iac_ac3d_batch_launch then enqueues gather + cell sort + pair pass (+ mirror) with
+ * no host synchronisation and no allocation.  iac_ac3d_batch_capture records that
+ * sequence into a HIP graph for (d_mesh, zero_mesh); later launches with the same pair
+ * replay the graph.  d_pix / d_cnt / d_frame_off are kept by pointer: their contents
+ * may change between launches with the frame count and n_total unchanged.  zero_mesh
+ * clears d_mesh first (by kernel).  No counterpart in the reference: it is the
+ * frame loop of IAC/RunIAC.cpp:524-550 as one replayable unit. */
+typedef struct iac_ac3d_batch iac_ac3d_batch;
+int iac_ac3d_batch_create(iac_ctx *ctx, const float *d_kmap, uint32_t npix, const uint32_t *d_pix,
+                          const uint16_t *d_cnt, const uint64_t *d_frame_off, uint32_t nframes, uint64_t n_total,
+                          const iac_mesh_shape *shape, int interp1, int interp2, int double_map,
+                          iac_ac3d_batch **out);
+int iac_ac3d_batch_launch(iac_ctx *ctx, iac_ac3d_batch *batch, uint64_t *d_mesh, int zero_mesh);
+int iac_ac3d_batch_capture(iac_ctx *ctx, iac_ac3d_batch *batch, uint64_t *d_mesh, int zero_mesh);
+int iac_ac3d_batch_destroy(iac_ac3d_batch *batch);
+
 /* ---- sparsification ------------------------------------------------------------- */
 /* Detector::CreateSparseHitList(Threshold, PhotonSamplingStep) after the pixel-mask
  * multiply, IAC/Detector.cpp:926-958, IAC/RunIAC.cpp:530-536, for nframes dense frames

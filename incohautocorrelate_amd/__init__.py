@@ -273,6 +273,18 @@ class Context:
                                           _ptr(avg), _ptr(mean_intensity), _ptr(photon_count)),
               "iac_average_intensity")
 
+    def ac3d_batch(self, kmap, pix, cnt, frame_off, n_total: int, shape: MeshShape, interp1: int = NEAREST,
+                   interp2: int = NEAREST, double_map: bool = True) -> "Ac3dBatch":
+        """A prepared, replayable frame batch (identity rotation) on the LDS engine."""
+        h = C.c_void_p()
+        nframes = frame_off.numel() - 1
+        check(lib().iac_ac3d_batch_create(self._h, _ptr(kmap), kmap.numel() // 3, _ptr(pix), _ptr(cnt),
+                                          _ptr(frame_off), nframes, n_total, C.byref(shape), interp1, interp2,
+                                          int(double_map), C.byref(h)), "iac_ac3d_batch_create")
+        b = Ac3dBatch(self, h)
+        self._plans.add(b)
+        return b
+
     # ---- simulator (Simulator::SimulatePart, IAC/Simulator.cpp:662-944) ----
     def sim_plan(self, pixmap, emitters, wavelength: float, mode: int = SIM_BF16X3, subsampling: int = 0,
                  u_step=(0.0, 0.0, 0.0), v_step=(0.0, 0.0, 0.0)) -> "SimPlan":
@@ -315,6 +327,32 @@ class Context:
         nframes, npix = frames.shape
         check(lib().iac_sim_postprocess(self._h, _ptr(frames), npix, nframes, n_expected, int(poisson),
                                         adu_noise, value_per_photon, seed, frame0), "iac_sim_postprocess")
+
+
+class Ac3dBatch:
+    """Owns an iac_ac3d_batch: launch() enqueues one pass, capture() records it as a graph."""
+
+    def __init__(self, ctx: Context, handle: C.c_void_p):
+        self.ctx, self.handle = ctx, handle
+
+    def launch(self, mesh, zero_mesh: bool = False) -> None:
+        check(lib().iac_ac3d_batch_launch(self.ctx.handle, self.handle, _ptr(mesh), int(zero_mesh)),
+              "iac_ac3d_batch_launch")
+
+    def capture(self, mesh, zero_mesh: bool = False) -> None:
+        check(lib().iac_ac3d_batch_capture(self.ctx.handle, self.handle, _ptr(mesh), int(zero_mesh)),
+              "iac_ac3d_batch_capture")
+
+    def destroy(self) -> None:
+        if self.handle:
+            lib().iac_ac3d_batch_destroy(self.handle)
+            self.handle = None
+
+    def __del__(self):
+        try:
+            self.destroy()
+        except Exception:
+            pass
 
 
 class SimPlan:

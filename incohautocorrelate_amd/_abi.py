@@ -88,6 +88,11 @@ SIGNATURES: dict[str, tuple] = {
     "iac_sim_plan_create": (I32, [P, P, U32, I32, C.POINTER(C.c_double), C.POINTER(C.c_double), C.c_double, P, U32,
                                   I32, C.POINTER(P)]),
     "iac_sim_plan_destroy": (I32, [P]),
+    "iac_ac3d_batch_create": (I32, [P, P, U32, P, P, P, U32, U64, C.POINTER(MeshShape), I32, I32, I32,
+                                    C.POINTER(P)]),
+    "iac_ac3d_batch_launch": (I32, [P, P, P, I32]),
+    "iac_ac3d_batch_capture": (I32, [P, P, P, I32]),
+    "iac_ac3d_batch_destroy": (I32, [P]),
     "iac_average_intensity": (I32, [P, P, U32, U32, P, C.c_float, C.c_float, P, P, P]),
     "iac_intensity_accumulate": (I32, [P, P, U32, U32, P, C.c_float, C.c_float, P, P, P, P]),
     "iac_intensity_finish": (I32, [P, U32, U32, C.c_float, P, P, P]),

@@ -296,9 +296,173 @@ int batch_records(iac_ctx *ctx, const float *d_kmap, uint32_t npix, const float4
                         d_mesh);
 }
 
+__global__ void zero_u64_kernel(unsigned long long *__restrict__ p, size_t n)
+{
+    for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x)
+        p[i] = 0ull;
+}
+
+}  // namespace
+
+// A prepared frame batch on the LDS engine (identity rotation): the plan, chunks
+// and buffers are made once; a launch enqueues gather + sort + pairs (+ mirror)
+// without host synchronisation or allocation, directly or by replaying a graph.
+struct iac_ac3d_batch {
+    iac_ctx *ctx = nullptr;
+    const float *d_kmap = nullptr;
+    uint32_t npix = 0;
+    const uint32_t *d_pix = nullptr;
+    const uint16_t *d_cnt = nullptr;
+    uint64_t n_total = 0;
+    size_t nvox = 0;
+    float4 *rec = nullptr;
+    iac::PairBatch *pb = nullptr;
+    hipGraph_t graph = nullptr;
+    hipGraphExec_t exec = nullptr;
+    uint64_t *graph_mesh = nullptr;
+    int graph_zero = -1;
+};
+
+namespace {
+
+int batch_enqueue(iac_ac3d_batch *b, hipStream_t st, uint64_t *d_mesh, int zero_mesh)
+{
+    if (zero_mesh) {
+        const unsigned zb = (unsigned)std::max<size_t>(1, std::min<size_t>(4096, (b->nvox + 255) / 256));
+        hipLaunchKernelGGL(zero_u64_kernel, dim3(zb), dim3(256), 0, st,
+                           reinterpret_cast<unsigned long long *>(d_mesh), b->nvox);
+    }
+    const unsigned gblocks = (unsigned)std::min<uint64_t>((b->n_total + 255) / 256, 65535);
+    hipLaunchKernelGGL(gather_records_kernel, dim3(gblocks), dim3(256), 0, st, b->d_kmap, b->npix, b->d_pix,
+                       b->d_cnt, b->n_total, b->rec, b->ctx->d_counters + 1);
+    IAC_HIP(hipGetLastError());
+    return iac::pairs_enqueue(b->pb, st, d_mesh);
+}
+
+void drop_graph(iac_ac3d_batch *b)
+{
+    if (b->exec) (void)hipGraphExecDestroy(b->exec);
+    if (b->graph) (void)hipGraphDestroy(b->graph);
+    b->exec = nullptr;
+    b->graph = nullptr;
+    b->graph_mesh = nullptr;
+    b->graph_zero = -1;
+}
+
 }  // namespace
 
 extern "C" {
+
+int iac_ac3d_batch_create(iac_ctx *ctx, const float *d_kmap, uint32_t npix, const uint32_t *d_pix,
+                          const uint16_t *d_cnt, const uint64_t *d_frame_off, uint32_t nframes, uint64_t n_total,
+                          const iac_mesh_shape *shape, int interp1, int interp2, int double_map,
+                          iac_ac3d_batch **out)
+{
+    IAC_TRY(iac::bind(ctx));
+    if (!out || !d_frame_off || !d_kmap || npix == 0 || !d_pix || !d_cnt)
+        return iac::fail(IAC_ERR_INVALID_ARG, "null input of a batch");
+    *out = nullptr;
+    if (nframes == 0 || n_total == 0) return iac::fail(IAC_ERR_INVALID_ARG, "a batch needs frames and photons");
+    if (n_total >= (1ull << 32)) return iac::fail(IAC_ERR_LIMIT, "a batch holds at most 2^32 - 1 photons");
+    iac::BinParams P;
+    IAC_TRY(make_params(shape, interp1, interp2, double_map, &P));
+    auto *b = new iac_ac3d_batch();
+    b->ctx = ctx;
+    b->d_kmap = d_kmap;
+    b->npix = npix;
+    b->d_pix = d_pix;
+    b->d_cnt = d_cnt;
+    b->n_total = n_total;
+    b->nvox = (size_t)P.sab * P.sab * P.sc;
+    hipError_t e = hipMalloc(&b->rec, n_total * sizeof(float4));
+    if (e != hipSuccess) {
+        delete b;
+        return iac::hip_fail(e, "iac_ac3d_batch_create allocation", __FILE__, __LINE__);
+    }
+    iac::PairRun R;
+    std::memset(&R.spec, 0, sizeof(R.spec));
+    R.spec.kind = iac::kKindAc3d;
+    for (int i = 0; i < 9; i++) R.spec.R[i] = (i % 4) == 0 ? 1.0f : 0.0f;
+    R.spec.P = P;
+    R.spec.P.rot_identity = 1;
+    R.ordered = !symmetric_mode(P);
+    R.d_kmap = d_kmap;
+    R.npix = npix;
+    R.rec = b->rec;
+    R.d_frame_off = d_frame_off;
+    R.nframes = nframes;
+    R.n_total = n_total;
+    const uint32_t goff[2] = {0, nframes};
+    R.h_group_off = goff;
+    R.ngroups = 1;
+    R.d_mesh = nullptr;
+    const int rc = iac::pairs_prepare(ctx, R, &b->pb);
+    if (rc != IAC_OK) {
+        (void)hipFree(b->rec);
+        delete b;
+        return rc;
+    }
+    *out = b;
+    return IAC_OK;
+}
+
+int iac_ac3d_batch_launch(iac_ctx *ctx, iac_ac3d_batch *batch, uint64_t *d_mesh, int zero_mesh)
+{
+    IAC_TRY(iac::bind(ctx));
+    if (!batch || batch->ctx != ctx || !d_mesh) return iac::fail(IAC_ERR_INVALID_ARG, "bad batch launch");
+    IAC_TRY(iac::timer_begin(ctx));
+    if (batch->exec && batch->graph_mesh == d_mesh && batch->graph_zero == (zero_mesh ? 1 : 0)) {
+        IAC_HIP(hipGraphLaunch(batch->exec, ctx->stream));
+    } else {
+        IAC_TRY(batch_enqueue(batch, ctx->stream, d_mesh, zero_mesh));
+    }
+    IAC_TRY(iac::timer_end(ctx));
+    return IAC_OK;
+}
+
+int iac_ac3d_batch_capture(iac_ctx *ctx, iac_ac3d_batch *batch, uint64_t *d_mesh, int zero_mesh)
+{
+    IAC_TRY(iac::bind(ctx));
+    if (!batch || batch->ctx != ctx || !d_mesh) return iac::fail(IAC_ERR_INVALID_ARG, "bad batch capture");
+    drop_graph(batch);
+    // capture on a private stream of default priority (the null stream cannot be captured)
+    hipStream_t cs = nullptr;
+    IAC_HIP(hipStreamCreateWithFlags(&cs, hipStreamNonBlocking));
+    hipError_t e = hipStreamBeginCapture(cs, hipStreamCaptureModeThreadLocal);
+    int rc = IAC_OK;
+    if (e == hipSuccess) rc = batch_enqueue(batch, cs, d_mesh, zero_mesh);
+    hipGraph_t g = nullptr;
+    const hipError_t e2 = e == hipSuccess ? hipStreamEndCapture(cs, &g) : e;
+    (void)hipStreamDestroy(cs);
+    if (rc != IAC_OK) {
+        if (g) (void)hipGraphDestroy(g);
+        return rc;
+    }
+    if (e2 != hipSuccess) return iac::hip_fail(e2, "batch stream capture", __FILE__, __LINE__);
+    hipGraphExec_t x = nullptr;
+    e = hipGraphInstantiate(&x, g, nullptr, nullptr, 0);
+    if (e != hipSuccess) {
+        (void)hipGraphDestroy(g);
+        return iac::hip_fail(e, "hipGraphInstantiate", __FILE__, __LINE__);
+    }
+    batch->graph = g;
+    batch->exec = x;
+    batch->graph_mesh = d_mesh;
+    batch->graph_zero = zero_mesh ? 1 : 0;
+    return IAC_OK;
+}
+
+int iac_ac3d_batch_destroy(iac_ac3d_batch *batch)
+{
+    if (!batch) return IAC_OK;
+    (void)hipSetDevice(batch->ctx->device);
+    (void)hipStreamSynchronize(batch->ctx->stream);
+    drop_graph(batch);
+    iac::pairs_batch_free(batch->pb);
+    if (batch->rec) (void)hipFree(batch->rec);
+    delete batch;
+    return IAC_OK;
+}
 
 int iac_ac3d_sparse(iac_ctx *ctx, const float *d_hitlist4, uint32_t n, const float rot[9],
                     const iac_mesh_shape *shape, int interp1, int interp2, int double_map,

@@ -39,4 +39,14 @@ struct PairRun {
 
 int pairs_run(iac_ctx *ctx, const PairRun &run);
 
+// Prepared runs: the plan, the frame chunks and every buffer made once (run.rec,
+// run.d_frame_off and the kmap are kept by pointer; their contents may change
+// between launches as long as sizes and the frame count stay); pairs_enqueue
+// then only enqueues kernels (no synchronisation, no allocation, no memset),
+// so it can be captured into a graph.  run.d_mesh is ignored; each enqueue names it.
+struct PairBatch;
+int pairs_prepare(iac_ctx *ctx, const PairRun &run, PairBatch **out);
+int pairs_enqueue(PairBatch *batch, hipStream_t stream, uint64_t *d_mesh);
+void pairs_batch_free(PairBatch *batch);
+
 }  // namespace iac

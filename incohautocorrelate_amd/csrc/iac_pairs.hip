@@ -647,27 +647,11 @@ iac_pair_plan *find_plan(iac_ctx *ctx, const float *kmap, uint32_t npix, int cel
     return nullptr;
 }
 
-}  // namespace
-
-void free_pair_plans(iac_ctx *ctx)
+// Cell grid of the k-map (two host round trips for the extents) and the task
+// table, from the context's cache when the geometry matches.
+int get_plan(iac_ctx *ctx, hipStream_t st, const PairRun &R, int cell_vox, iac_pair_plan **out)
 {
-    for (auto *p : ctx->pair_plans) {
-        if (p->d_tasks) (void)hipFree(p->d_tasks);
-        delete p;
-    }
-    ctx->pair_plans.clear();
-}
-
-int pairs_run(iac_ctx *ctx, const PairRun &R)
-{
-    hipStream_t st = ctx->stream;
     const PairSpec &S = R.spec;
-    const bool ordered = R.ordered;
-    const int cell_vox = ctx->cell_vox > 0 ? ctx->cell_vox : (S.kind == kKindAc3d ? 6 : 4);
-    const uint32_t fchunk = ctx->frame_chunk > 0 ? ctx->frame_chunk : 64;
-    if (R.n_total >= (1ull << 32)) return fail(IAC_ERR_LIMIT, "pair engine: at most 2^32 - 1 records per call");
-
-    // ---- plan: cell grid from the k extents, per-cell extents, tasks (cached)
     void *ext = nullptr;
     IAC_TRY(scratch(ctx, 3, (6 + 6 * (size_t)kMaxCells) * sizeof(uint32_t), &ext));
     uint32_t *d_ext = static_cast<uint32_t *>(ext);
@@ -722,7 +706,7 @@ int pairs_run(iac_ctx *ctx, const PairRun &R)
         pl->spec = S;
         pl->g = g;
         pl->cell_ext.swap(cext);
-        build_tasks(*pl, ordered);
+        build_tasks(*pl, R.ordered);
         if (!pl->tasks.empty()) {
             IAC_HIP(hipMalloc(&pl->d_tasks, pl->tasks.size() * sizeof(Task)));
             IAC_HIP(hipMemcpy(pl->d_tasks, pl->tasks.data(), pl->tasks.size() * sizeof(Task),
@@ -730,87 +714,254 @@ int pairs_run(iac_ctx *ctx, const PairRun &R)
         }
         pl->valid = true;
     }
+    *out = pl;
+    return IAC_OK;
+}
 
-    // ---- chunks: runs of at most fchunk frames inside one group
+// Frame chunks inside each group, balanced by pair work: a chunk closes once its
+// frames' n^2 reach fchunk times the mean per frame, or at 4 fchunk frames.
+std::vector<Chunk> make_chunks(const std::vector<uint64_t> &off, const uint32_t *group_off, uint32_t ngroups,
+                               uint32_t fchunk)
+{
+    const uint32_t nframes = (uint32_t)off.size() - 1;
+    double total = 0;
+    for (uint32_t f = 0; f < nframes; f++) {
+        const double n = (double)(off[f + 1] - off[f]);
+        total += n * n;
+    }
+    const double budget = std::max(1.0, total / std::max(1u, nframes) * fchunk);
+    const uint32_t fmax = 4 * fchunk;
     std::vector<Chunk> chunks;
-    for (uint32_t gi = 0; gi < R.ngroups; gi++)
-        for (uint32_t f = R.h_group_off[gi]; f < R.h_group_off[gi + 1]; f += fchunk)
-            chunks.push_back(Chunk{f, std::min(R.h_group_off[gi + 1], f + fchunk), gi});
-    const size_t nvox = (size_t)S.P.sab * S.P.sab * S.P.sc;
+    for (uint32_t gi = 0; gi < ngroups; gi++) {
+        uint32_t f0 = group_off[gi];
+        double acc = 0;
+        for (uint32_t f = group_off[gi]; f < group_off[gi + 1]; f++) {
+            const double n = (double)(off[f + 1] - off[f]);
+            acc += n * n;
+            if (acc >= budget || f + 1 - f0 >= fmax) {
+                chunks.push_back(Chunk{f0, f + 1, gi});
+                f0 = f + 1;
+                acc = 0;
+            }
+        }
+        if (f0 < group_off[gi + 1]) chunks.push_back(Chunk{f0, group_off[gi + 1], gi});
+    }
+    return chunks;
+}
 
-    // ---- scratch: sorted records + cell offsets (slot 1), H + group offsets + self + chunks (slot 2)
-    void *s1 = nullptr, *s2 = nullptr;
-    const size_t cs_bytes = (size_t)R.nframes * (nc + 1) * sizeof(uint32_t);
-    IAC_TRY(scratch(ctx, 1, R.n_total * sizeof(float4) + cs_bytes, &s1));
-    float4 *srec = static_cast<float4 *>(s1);
-    uint32_t *cs = reinterpret_cast<uint32_t *>(static_cast<char *>(s1) + R.n_total * sizeof(float4));
-    const size_t h_bytes = ordered ? 0 : nvox * R.ngroups * sizeof(unsigned long long);
-    const size_t goff_bytes = ((R.ngroups + 1) * sizeof(uint32_t) + 255) / 256 * 256;
-    const size_t self_bytes = (R.ngroups * sizeof(unsigned long long) + 255) / 256 * 256;
-    const size_t chunk_bytes = chunks.size() * sizeof(Chunk);
-    IAC_TRY(scratch(ctx, 2, h_bytes + goff_bytes + self_bytes + chunk_bytes, &s2));
-    char *p2 = static_cast<char *>(s2);
-    auto *H = reinterpret_cast<unsigned long long *>(p2);
-    auto *d_goff = reinterpret_cast<uint32_t *>(p2 + h_bytes);
-    auto *d_self = reinterpret_cast<unsigned long long *>(p2 + h_bytes + goff_bytes);
-    auto *d_chunks = reinterpret_cast<Chunk *>(p2 + h_bytes + goff_bytes + self_bytes);
-    IAC_HIP(hipMemcpyAsync(d_goff, R.h_group_off, (R.ngroups + 1) * sizeof(uint32_t), hipMemcpyHostToDevice, st));
-    if (!chunks.empty())
-        IAC_HIP(hipMemcpyAsync(d_chunks, chunks.data(), chunk_bytes, hipMemcpyHostToDevice, st));
-    IAC_HIP(hipStreamSynchronize(st));  // the host arrays above are released on return
+struct Bufs {
+    float4 *srec;
+    uint32_t *cs;
+    unsigned long long *H;  // symmetric modes: ngroups half-mesh accumulators
+    uint32_t *d_goff;
+    unsigned long long *d_self;
+    Chunk *d_chunks;
+    uint32_t nchunks;
+    const Task *d_tasks;
+    uint32_t ntasks;
+    CellGeom g;
+};
+
+// Everything after planning, enqueue only (no host synchronisation, no
+// allocation, fills by kernel): capturable into a graph.
+int enqueue(iac_ctx *ctx, hipStream_t st, const PairRun &R, const Bufs &b, bool timed)
+{
+    const PairSpec &S = R.spec;
+    const bool ordered = R.ordered;
+    const size_t nvox = (size_t)S.P.sab * S.P.sab * S.P.sc;
     const unsigned zb = (unsigned)std::max<size_t>(1, std::min<size_t>(4096, (nvox * R.ngroups + 255) / 256));
     if (!ordered) {
-        hipLaunchKernelGGL(fill_u64_kernel, dim3(zb), dim3(256), 0, st, H, nvox * R.ngroups);
-        hipLaunchKernelGGL(fill_u64_kernel, dim3(1), dim3(256), 0, st, d_self, (size_t)R.ngroups);
+        hipLaunchKernelGGL(fill_u64_kernel, dim3(zb), dim3(256), 0, st, b.H, nvox * R.ngroups);
+        hipLaunchKernelGGL(fill_u64_kernel, dim3(1), dim3(256), 0, st, b.d_self, (size_t)R.ngroups);
     }
-
     if (S.kind == kKindAc3d) {
         if (ordered)
             hipLaunchKernelGGL((cell_sort_kernel<kKindAc3d, true>), dim3(R.nframes), dim3(kBlock), 0, st, R.rec,
-                               R.d_frame_off, g, srec, cs, S, d_goff, R.ngroups, d_self);
+                               R.d_frame_off, b.g, b.srec, b.cs, S, b.d_goff, R.ngroups, b.d_self);
         else
             hipLaunchKernelGGL((cell_sort_kernel<kKindAc3d, false>), dim3(R.nframes), dim3(kBlock), 0, st, R.rec,
-                               R.d_frame_off, g, srec, cs, S, d_goff, R.ngroups, d_self);
+                               R.d_frame_off, b.g, b.srec, b.cs, S, b.d_goff, R.ngroups, b.d_self);
     } else {
         if (ordered)
             hipLaunchKernelGGL((cell_sort_kernel<kKindCq, true>), dim3(R.nframes), dim3(kBlock), 0, st, R.rec,
-                               R.d_frame_off, g, srec, cs, S, d_goff, R.ngroups, d_self);
+                               R.d_frame_off, b.g, b.srec, b.cs, S, b.d_goff, R.ngroups, b.d_self);
         else
             hipLaunchKernelGGL((cell_sort_kernel<kKindCq, false>), dim3(R.nframes), dim3(kBlock), 0, st, R.rec,
-                               R.d_frame_off, g, srec, cs, S, d_goff, R.ngroups, d_self);
+                               R.d_frame_off, b.g, b.srec, b.cs, S, b.d_goff, R.ngroups, b.d_self);
     }
     IAC_HIP(hipGetLastError());
-
-    const uint32_t ntasks = (uint32_t)pl->tasks.size();
-    auto *target = ordered ? reinterpret_cast<unsigned long long *>(R.d_mesh) : H;
-    if (ntasks && !chunks.empty()) {
-        const dim3 grid(ntasks * (uint32_t)chunks.size());
-        IAC_TRY(timer_begin(ctx));
+    auto *target = ordered ? reinterpret_cast<unsigned long long *>(R.d_mesh) : b.H;
+    if (b.ntasks && b.nchunks) {
+        const dim3 grid(b.ntasks * b.nchunks);
+        if (timed) IAC_TRY(timer_begin(ctx));
         if (S.kind == kKindAc3d) {
             if (ordered)
-                hipLaunchKernelGGL((pairs_kernel<kKindAc3d, true>), grid, dim3(kBlock), 0, st, srec, R.d_frame_off,
-                                   cs, d_chunks, pl->d_tasks, ntasks, g, S, target, nvox, ctx->d_counters);
+                hipLaunchKernelGGL((pairs_kernel<kKindAc3d, true>), grid, dim3(kBlock), 0, st, b.srec, R.d_frame_off,
+                                   b.cs, b.d_chunks, b.d_tasks, b.ntasks, b.g, S, target, nvox, ctx->d_counters);
             else
-                hipLaunchKernelGGL((pairs_kernel<kKindAc3d, false>), grid, dim3(kBlock), 0, st, srec, R.d_frame_off,
-                                   cs, d_chunks, pl->d_tasks, ntasks, g, S, target, nvox, ctx->d_counters);
+                hipLaunchKernelGGL((pairs_kernel<kKindAc3d, false>), grid, dim3(kBlock), 0, st, b.srec,
+                                   R.d_frame_off, b.cs, b.d_chunks, b.d_tasks, b.ntasks, b.g, S, target, nvox,
+                                   ctx->d_counters);
         } else {
             if (ordered)
-                hipLaunchKernelGGL((pairs_kernel<kKindCq, true>), grid, dim3(kBlock), 0, st, srec, R.d_frame_off,
-                                   cs, d_chunks, pl->d_tasks, ntasks, g, S, target, nvox, ctx->d_counters);
+                hipLaunchKernelGGL((pairs_kernel<kKindCq, true>), grid, dim3(kBlock), 0, st, b.srec, R.d_frame_off,
+                                   b.cs, b.d_chunks, b.d_tasks, b.ntasks, b.g, S, target, nvox, ctx->d_counters);
             else
-                hipLaunchKernelGGL((pairs_kernel<kKindCq, false>), grid, dim3(kBlock), 0, st, srec, R.d_frame_off,
-                                   cs, d_chunks, pl->d_tasks, ntasks, g, S, target, nvox, ctx->d_counters);
+                hipLaunchKernelGGL((pairs_kernel<kKindCq, false>), grid, dim3(kBlock), 0, st, b.srec, R.d_frame_off,
+                                   b.cs, b.d_chunks, b.d_tasks, b.ntasks, b.g, S, target, nvox, ctx->d_counters);
         }
         IAC_HIP(hipGetLastError());
-        IAC_TRY(timer_end(ctx));
+        if (timed) IAC_TRY(timer_end(ctx));
     }
     if (!ordered) {
         const size_t centre = (size_t)S.P.c0 + (size_t)S.P.c1 * S.P.sab + (size_t)S.P.c2 * S.P.sab * S.P.sab;
-        hipLaunchKernelGGL(symmetrize_kernel, dim3(zb), dim3(256), 0, st, H,
-                           reinterpret_cast<unsigned long long *>(R.d_mesh), nvox, R.ngroups, centre, d_self);
+        hipLaunchKernelGGL(symmetrize_kernel, dim3(zb), dim3(256), 0, st, b.H,
+                           reinterpret_cast<unsigned long long *>(R.d_mesh), nvox, R.ngroups, centre, b.d_self);
         IAC_HIP(hipGetLastError());
     }
     return IAC_OK;
+}
+
+int cell_vox_of(iac_ctx *ctx, const PairSpec &S) { return ctx->cell_vox > 0 ? ctx->cell_vox : (S.kind == kKindAc3d ? 6 : 4); }
+uint32_t fchunk_of(iac_ctx *ctx) { return ctx->frame_chunk > 0 ? ctx->frame_chunk : 64; }
+
+// byte sizes of the per-run buffers
+struct Sizes {
+    size_t srec, cs, h, goff, self, chunks;
+};
+Sizes sizes_of(const PairRun &R, int nc, size_t nchunks)
+{
+    const size_t nvox = (size_t)R.spec.P.sab * R.spec.P.sab * R.spec.P.sc;
+    auto up = [](size_t x) { return (x + 255) / 256 * 256; };
+    return Sizes{up(R.n_total * sizeof(float4)),
+                 up((size_t)R.nframes * (nc + 1) * sizeof(uint32_t)),
+                 R.ordered ? 0 : nvox * R.ngroups * sizeof(unsigned long long),
+                 up((R.ngroups + 1) * sizeof(uint32_t)),
+                 up(R.ngroups * sizeof(unsigned long long)),
+                 up(std::max<size_t>(1, nchunks) * sizeof(Chunk))};
+}
+
+}  // namespace
+
+void free_pair_plans(iac_ctx *ctx)
+{
+    for (auto *p : ctx->pair_plans) {
+        if (p->d_tasks) (void)hipFree(p->d_tasks);
+        delete p;
+    }
+    ctx->pair_plans.clear();
+}
+
+int pairs_run(iac_ctx *ctx, const PairRun &R)
+{
+    hipStream_t st = ctx->stream;
+    if (R.n_total >= (1ull << 32)) return fail(IAC_ERR_LIMIT, "pair engine: at most 2^32 - 1 records per call");
+    iac_pair_plan *pl = nullptr;
+    IAC_TRY(get_plan(ctx, st, R, cell_vox_of(ctx, R.spec), &pl));
+    std::vector<uint64_t> off(R.nframes + 1);
+    IAC_HIP(hipMemcpyAsync(off.data(), R.d_frame_off, off.size() * 8, hipMemcpyDeviceToHost, st));
+    IAC_HIP(hipStreamSynchronize(st));
+    const std::vector<Chunk> chunks = make_chunks(off, R.h_group_off, R.ngroups, fchunk_of(ctx));
+    const int nc = pl->g.ncu * pl->g.ncv;
+    const Sizes z = sizes_of(R, nc, chunks.size());
+    // scratch: sorted records + cell offsets (slot 1), H + group offsets + self + chunks (slot 2)
+    void *s1 = nullptr, *s2 = nullptr;
+    IAC_TRY(scratch(ctx, 1, z.srec + z.cs, &s1));
+    IAC_TRY(scratch(ctx, 2, z.h + z.goff + z.self + z.chunks, &s2));
+    Bufs b{};
+    b.srec = static_cast<float4 *>(s1);
+    b.cs = reinterpret_cast<uint32_t *>(static_cast<char *>(s1) + z.srec);
+    char *p2 = static_cast<char *>(s2);
+    b.H = reinterpret_cast<unsigned long long *>(p2);
+    b.d_goff = reinterpret_cast<uint32_t *>(p2 + z.h);
+    b.d_self = reinterpret_cast<unsigned long long *>(p2 + z.h + z.goff);
+    b.d_chunks = reinterpret_cast<Chunk *>(p2 + z.h + z.goff + z.self);
+    b.nchunks = (uint32_t)chunks.size();
+    b.d_tasks = pl->d_tasks;
+    b.ntasks = (uint32_t)pl->tasks.size();
+    b.g = pl->g;
+    IAC_HIP(hipMemcpyAsync(b.d_goff, R.h_group_off, (R.ngroups + 1) * sizeof(uint32_t), hipMemcpyHostToDevice, st));
+    if (!chunks.empty())
+        IAC_HIP(hipMemcpyAsync(b.d_chunks, chunks.data(), chunks.size() * sizeof(Chunk), hipMemcpyHostToDevice, st));
+    IAC_HIP(hipStreamSynchronize(st));  // the host arrays above are released on return
+    return enqueue(ctx, st, R, b, true);
+}
+
+// ---- prepared runs -----------------------------------------------------------
+
+struct PairBatch {
+    iac_ctx *ctx = nullptr;
+    PairRun R{};
+    std::vector<uint32_t> group_off;
+    void *mem = nullptr;     // srec | cs | H | goff | self | chunks | tasks
+    Bufs b{};
+};
+
+int pairs_prepare(iac_ctx *ctx, const PairRun &R0, PairBatch **out)
+{
+    *out = nullptr;
+    hipStream_t st = ctx->stream;
+    if (R0.n_total >= (1ull << 32)) return fail(IAC_ERR_LIMIT, "pair engine: at most 2^32 - 1 records per call");
+    iac_pair_plan *pl = nullptr;
+    IAC_TRY(get_plan(ctx, st, R0, cell_vox_of(ctx, R0.spec), &pl));
+    std::vector<uint64_t> off(R0.nframes + 1);
+    IAC_HIP(hipMemcpyAsync(off.data(), R0.d_frame_off, off.size() * 8, hipMemcpyDeviceToHost, st));
+    IAC_HIP(hipStreamSynchronize(st));
+    auto *B = new PairBatch();
+    B->ctx = ctx;
+    B->R = R0;
+    B->group_off.assign(R0.h_group_off, R0.h_group_off + R0.ngroups + 1);
+    B->R.h_group_off = B->group_off.data();
+    const std::vector<Chunk> chunks = make_chunks(off, B->R.h_group_off, R0.ngroups, fchunk_of(ctx));
+    const int nc = pl->g.ncu * pl->g.ncv;
+    const Sizes z = sizes_of(B->R, nc, chunks.size());
+    const size_t tbytes = std::max<size_t>(1, pl->tasks.size()) * sizeof(Task);
+    const size_t total = z.srec + z.cs + z.h + z.goff + z.self + z.chunks + tbytes;
+    hipError_t e = hipMalloc(&B->mem, total);
+    if (e != hipSuccess) {
+        delete B;
+        return hip_fail(e, "pairs_prepare allocation", __FILE__, __LINE__);
+    }
+    char *p = static_cast<char *>(B->mem);
+    Bufs &b = B->b;
+    b.srec = reinterpret_cast<float4 *>(p);
+    b.cs = reinterpret_cast<uint32_t *>(p + z.srec);
+    b.H = reinterpret_cast<unsigned long long *>(p + z.srec + z.cs);
+    b.d_goff = reinterpret_cast<uint32_t *>(p + z.srec + z.cs + z.h);
+    b.d_self = reinterpret_cast<unsigned long long *>(p + z.srec + z.cs + z.h + z.goff);
+    b.d_chunks = reinterpret_cast<Chunk *>(p + z.srec + z.cs + z.h + z.goff + z.self);
+    Task *tasks = reinterpret_cast<Task *>(p + z.srec + z.cs + z.h + z.goff + z.self + z.chunks);
+    b.nchunks = (uint32_t)chunks.size();
+    b.d_tasks = tasks;
+    b.ntasks = (uint32_t)pl->tasks.size();
+    b.g = pl->g;
+    e = hipMemcpyAsync(b.d_goff, B->group_off.data(), B->group_off.size() * 4, hipMemcpyHostToDevice, st);
+    if (e == hipSuccess && !chunks.empty())
+        e = hipMemcpyAsync(b.d_chunks, chunks.data(), chunks.size() * sizeof(Chunk), hipMemcpyHostToDevice, st);
+    if (e == hipSuccess && !pl->tasks.empty())
+        e = hipMemcpyAsync(tasks, pl->tasks.data(), pl->tasks.size() * sizeof(Task), hipMemcpyHostToDevice, st);
+    if (e == hipSuccess) e = hipStreamSynchronize(st);
+    if (e != hipSuccess) {
+        (void)hipFree(B->mem);
+        delete B;
+        return hip_fail(e, "pairs_prepare upload", __FILE__, __LINE__);
+    }
+    *out = B;
+    return IAC_OK;
+}
+
+int pairs_enqueue(PairBatch *B, hipStream_t st, uint64_t *d_mesh)
+{
+    PairRun R = B->R;
+    R.d_mesh = d_mesh;
+    return enqueue(B->ctx, st, R, B->b, false);
+}
+
+void pairs_batch_free(PairBatch *B)
+{
+    if (!B) return;
+    if (B->mem) (void)hipFree(B->mem);
+    delete B;
 }
 
 }  // namespace iac

@@ -1,0 +1,173 @@
+"""Prepared frame batches (iac_ac3d_batch_*): one launch or a replayed graph equals
+the unbatched per-frame GPU kernel, the one-call batch and the CPU oracle,
+bit for bit -- including the uneven lognormal workload at full size."""
+import numpy as np
+import pytest
+
+import oracle as O
+from gpu_helpers import mesh_np, to_dev
+from incohautocorrelate_amd import workloads as W
+
+pytestmark = pytest.mark.gpu
+
+
+def _dev_frames(data, dev):
+    return (to_dev(data["pix"], dev), to_dev(data["cnt"], dev), to_dev(data["frame_off"], dev))
+
+
+def _per_frame(ctx, dev, km, data, shape, frames):
+    """The unbatched GPU path: iac_ac3d_sparse once per frame (tile kernel)."""
+    import torch
+
+    m = torch.zeros(shape.size_ab ** 2 * shape.size_c, dtype=torch.int64, device=dev)
+    off = data["frame_off"]
+    eye = np.eye(3, dtype=np.float32).reshape(9)
+    for f in frames:
+        a, b = int(off[f]), int(off[f + 1])
+        if a == b:
+            continue
+        hl = O.hit_list_from_photons(km, data["pix"][a:b], data["cnt"][a:b])
+        ctx.ac3d_sparse(to_dev(hl, dev), eye, shape, m)
+    return mesh_np(m)
+
+
+@pytest.mark.parametrize("interp1,interp2,dm", [(0, 0, True), (1, 1, True), (0, 1, False)])
+def test_batch_equals_per_frame_and_oracle(ctx, dev, interp1, interp2, dm):
+    import torch
+
+    import incohautocorrelate_amd as iac
+
+    n = 128
+    _, km, mq = W.detector(n)
+    shape = iac.mesh_shape_big(mq, 63)
+    data = W.lognormal_photon_frames(n * n, 40, 60, 1.0, 2, 600, seed=5 + interp1)
+    n_total = int(data["frame_off"][-1])
+    d_km = to_dev(km, dev)
+    pix, cnt, off = _dev_frames(data, dev)
+    b = ctx.ac3d_batch(d_km, pix, cnt, off, n_total, shape, interp1, interp2, dm)
+    m = torch.zeros(shape.size_ab ** 2 * shape.size_c, dtype=torch.int64, device=dev)
+    b.launch(m, zero_mesh=True)
+    got = mesh_np(m)
+    ref, _ = O.ac3d_frames(km, data["pix"], data["cnt"], data["frame_off"], shape, None, interp1, interp2, dm)
+    assert np.array_equal(got, ref)
+    if (interp1, interp2, dm) == (0, 0, True):
+        assert np.array_equal(got, _per_frame(ctx, dev, km, data, shape, range(40)))
+    b.destroy()
+
+
+def test_graph_replays_match_eager(ctx, dev):
+    """Replays of the captured graph, with eager work and syncs in between, each
+    equal the eager launch (the mesh is cleared inside the graph by a kernel)."""
+    import torch
+
+    import incohautocorrelate_amd as iac
+
+    n = 192
+    _, km, mq = W.detector(n)
+    shape = iac.mesh_shape_big(mq, 95)
+    data = W.lognormal_photon_frames(n * n, 64, 150, 1.0, 5, 1500, seed=8)
+    n_total = int(data["frame_off"][-1])
+    d_km = to_dev(km, dev)
+    pix, cnt, off = _dev_frames(data, dev)
+    b = ctx.ac3d_batch(d_km, pix, cnt, off, n_total, shape)
+    eager = torch.zeros(shape.size_ab ** 3, dtype=torch.int64, device=dev)
+    b.launch(eager, zero_mesh=True)
+    want = mesh_np(eager)
+    m = torch.zeros_like(eager)
+    b.capture(m, zero_mesh=True)
+    for r in range(4):
+        m.fill_(12345)  # stale content the graph must clear
+        b.launch(m, zero_mesh=True)
+        torch.cuda.synchronize()
+        assert np.array_equal(mesh_np(m), want), f"replay {r}"
+        # eager work between replays
+        other = torch.zeros_like(eager)
+        ctx.ac3d_sparse_batch(d_km, pix, cnt, off, n_total, shape, other)
+        torch.cuda.synchronize()
+        assert np.array_equal(mesh_np(other), want)
+    # new photon counts in the same buffers: the graph reads the current contents
+    cnt2 = np.minimum(data["cnt"].astype(np.int64) * 3, 65535).astype(np.uint16)
+    cnt.copy_(to_dev(cnt2, dev))
+    b.launch(m, zero_mesh=True)
+    ref, _ = O.ac3d_frames(km, data["pix"], cnt2, data["frame_off"], shape)
+    assert np.array_equal(mesh_np(m), ref)
+    b.destroy()
+
+
+def test_accumulating_launches(ctx, dev):
+    import torch
+
+    import incohautocorrelate_amd as iac
+
+    n = 96
+    _, km, mq = W.detector(n)
+    shape = iac.mesh_shape_big(mq, 47)
+    data = W.poisson_photon_frames(n * n, 30, 0.01, 3)
+    n_total = int(data["frame_off"][-1])
+    pix, cnt, off = _dev_frames(data, dev)
+    b = ctx.ac3d_batch(to_dev(km, dev), pix, cnt, off, n_total, shape)
+    m = torch.zeros(shape.size_ab ** 3, dtype=torch.int64, device=dev)
+    for _ in range(3):
+        b.launch(m, zero_mesh=False)
+    ref, _ = O.ac3d_frames(km, data["pix"], data["cnt"], data["frame_off"], shape)
+    assert np.array_equal(mesh_np(m), ref * np.uint64(3))
+    b.destroy()
+
+
+def test_uneven_lognormal_full_size(ctx, dev):
+    """4,096 frames on a 1024^2 detector, photons/frame lognormal (median 1,500,
+    sigma 1, clipped to 10..20,000): the graph-replayed batch equals the
+    global-atomic tile kernel over all frames, the per-frame kernel on a frame
+    subset and the CPU oracle on a smaller subset."""
+    import torch
+
+    import incohautocorrelate_amd as iac
+
+    n = 1024
+    _, km, mq = W.detector(n)
+    shape = iac.mesh_shape_big(mq, 255)
+    data = W.lognormal_photon_frames(n * n, 4096, 1500, 1.0, 10, 20000, seed=0)
+    n_total = int(data["frame_off"][-1])
+    print(f"uneven workload: {n_total} photons, max frame {np.diff(data['frame_off'].astype(np.int64)).max()}")
+    d_km = to_dev(km, dev)
+    pix, cnt, off = _dev_frames(data, dev)
+    nvox = shape.size_ab ** 2 * shape.size_c
+    b = ctx.ac3d_batch(d_km, pix, cnt, off, n_total, shape)
+    m = torch.zeros(nvox, dtype=torch.int64, device=dev)
+    b.capture(m, zero_mesh=True)
+    b.launch(m, zero_mesh=True)
+    got = mesh_np(m)
+    tiles = torch.zeros(nvox, dtype=torch.int64, device=dev)
+    ctx.set_option(iac.OPT_AC3D_KERNEL, 1)
+    try:
+        ctx.ac3d_sparse_batch(d_km, pix, cnt, off, n_total, shape, tiles)
+    finally:
+        ctx.set_option(iac.OPT_AC3D_KERNEL, 0)
+    assert np.array_equal(got, mesh_np(tiles))
+    assert got.sum() > 0 and ctx.dropped() == 0
+    b.destroy()
+    # subsets: per-frame GPU kernel (24 frames incl. the largest) and the CPU oracle (6 frames)
+    sizes = np.diff(data["frame_off"].astype(np.int64))
+    sub = sorted(set(range(16)) | set(np.argsort(sizes)[-8:].tolist()))
+    sub_off = np.zeros(len(sub) + 1, np.uint64)
+    sub_pix, sub_cnt = [], []
+    for i, f in enumerate(sub):
+        a, c = int(data["frame_off"][f]), int(data["frame_off"][f + 1])
+        sub_pix.append(data["pix"][a:c])
+        sub_cnt.append(data["cnt"][a:c])
+        sub_off[i + 1] = sub_off[i] + (c - a)
+    sd = {"pix": np.concatenate(sub_pix), "cnt": np.concatenate(sub_cnt), "frame_off": sub_off}
+    spix, scnt, soff = _dev_frames(sd, dev)
+    bs = ctx.ac3d_batch(d_km, spix, scnt, soff, int(sub_off[-1]), shape)
+    ms = torch.zeros(nvox, dtype=torch.int64, device=dev)
+    bs.launch(ms, zero_mesh=True)
+    assert np.array_equal(mesh_np(ms), _per_frame(ctx, dev, km, data, shape, sub))
+    bs.destroy()
+    six = {"pix": sd["pix"][: int(sub_off[6])], "cnt": sd["cnt"][: int(sub_off[6])], "frame_off": sub_off[:7]}
+    ref, _ = O.ac3d_frames(km, six["pix"], six["cnt"], six["frame_off"], shape)
+    p6, c6, o6 = _dev_frames(six, dev)
+    b6 = ctx.ac3d_batch(d_km, p6, c6, o6, int(sub_off[6]), shape)
+    m6 = torch.zeros(nvox, dtype=torch.int64, device=dev)
+    b6.launch(m6, zero_mesh=True)
+    assert np.array_equal(mesh_np(m6), ref)
+    b6.destroy()

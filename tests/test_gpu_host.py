@@ -82,7 +82,8 @@ def test_eval_3d_vs_oracle(tmp_path, interp1, interp2, rotate):
     assert mesh.sum() > 0
     # final AC
     assert np.array_equal(np.fromfile(out["final"], np.float64), O.merge_ac_cq(mesh, big))
-    assert rep["FinalMeshSize"] == bs.size_ab and rep["SmallCqMeshSize"] == [ss.size_ab, ss.size_ab, ss.size_c]
+    # the reference fills SmallCqMeshSize from the merged (big) C(q) mesh, IAC/RunIAC.cpp:612-614
+    assert rep["FinalMeshSize"] == bs.size_ab and rep["SmallCqMeshSize"] == [bs.size_ab, bs.size_ab, bs.size_c]
 
 
 def test_eval_bounds_and_flatfield(tmp_path):
