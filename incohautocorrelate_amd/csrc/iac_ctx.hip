@@ -279,6 +279,11 @@ int iac_ctx_set_option(iac_ctx *ctx, int option, int64_t value)
         if (value < 0 || value > (1 << 20)) return fail(IAC_ERR_INVALID_ARG, "IAC_OPT_FRAME_CHUNK is 0..2^20");
         ctx->frame_chunk = (uint32_t)value;
         return IAC_OK;
+    case IAC_OPT_PAIR_PREFETCH:
+        if (value != 0 && value != 1 && value != 4 && value != 8)
+            return fail(IAC_ERR_INVALID_ARG, "IAC_OPT_PAIR_PREFETCH is 0, 1, 4 or 8");
+        ctx->pair_prefetch = (int)value;
+        return IAC_OK;
     case IAC_OPT_SIM_CHUNK_ROWS:
         if (value < 0 || value > (1ll << 31)) return fail(IAC_ERR_INVALID_ARG, "IAC_OPT_SIM_CHUNK_ROWS is 0..2^31");
         ctx->sim_chunk_rows = (uint64_t)value;

@@ -338,7 +338,7 @@ struct BoxSink {
     }
 };
 
-template <int KIND, bool ORD>
+template <int KIND, bool ORD, int PF>
 __global__ void __launch_bounds__(kBlock)
     pairs_kernel(const float4 *__restrict__ srec, const uint64_t *__restrict__ off,
                  const uint32_t *__restrict__ cs, const Chunk *__restrict__ chunks,
@@ -411,7 +411,9 @@ __global__ void __launch_bounds__(kBlock)
         }
         __syncthreads();
 
-        // phase 2: thread t bins the contiguous pair range [p0, p1)
+        // phase 2: thread t bins the contiguous pair range [p0, p1), PF pairs at a
+        // time: the enumeration is integer-only, so the 2 PF record loads of a batch
+        // are independent and in flight together before the batch is binned.
         const uint32_t total = pre[kGroup];
         const uint32_t p0 = (uint32_t)(((uint64_t)total * threadIdx.x) / kBlock);
         const uint32_t p1 = (uint32_t)(((uint64_t)total * (threadIdx.x + 1)) / kBlock);
@@ -436,32 +438,44 @@ __global__ void __launch_bounds__(kBlock)
                 ia = r / d.w;
                 ib = r % d.w;
             }
-            float4 ra = srec[d.x + ia];
-            for (uint32_t p = p0;;) {
-                const float4 rb = srec[d.z + ib];
-                if (ORD) {
-                    bin_ordered<KIND>(S, ra, rb, sink);
-                } else {
-                    int fs, ms, ss;
-                    uint64_t v;
-                    if (bin_sym<KIND>(S, ra, rb, fs, ms, ss, v)) sink.add(fs, ms, ss, v);
-                }
-                if (++p == p1) break;
-                // next pair; one exists because p < p1 <= total
-                ib++;
-                if (ib >= (tri ? d.y : d.w)) {
-                    ia++;
-                    ib = tri ? ia + 1 : 0;
-                    if ((tri ? ib : ia) >= d.y) {  // item exhausted: next item with pairs
-                        do {
-                            u++;
-                        } while (pre[u + 1] == pre[u]);
-                        d = item[u];
-                        ia = 0;
-                        ib = tri ? 1 : 0;
+            for (uint32_t p = p0; p < p1;) {
+                const uint32_t nb = min((uint32_t)PF, p1 - p);
+                float4 A[PF], B[PF];
+#pragma unroll
+                for (int k = 0; k < PF; k++) {
+                    if ((uint32_t)k < nb) {
+                        A[k] = srec[d.x + ia];
+                        B[k] = srec[d.z + ib];
+                        if (p + k + 1 < p1) {  // a next pair exists: advance the enumeration
+                            ib++;
+                            if (ib >= (tri ? d.y : d.w)) {
+                                ia++;
+                                ib = tri ? ia + 1 : 0;
+                                if ((tri ? ib : ia) >= d.y) {  // item exhausted: next item with pairs
+                                    do {
+                                        u++;
+                                    } while (pre[u + 1] == pre[u]);
+                                    d = item[u];
+                                    ia = 0;
+                                    ib = tri ? 1 : 0;
+                                }
+                            }
+                        }
                     }
-                    ra = srec[d.x + ia];
                 }
+#pragma unroll
+                for (int k = 0; k < PF; k++) {
+                    if ((uint32_t)k < nb) {
+                        if (ORD) {
+                            bin_ordered<KIND>(S, A[k], B[k], sink);
+                        } else {
+                            int fs, ms, ss;
+                            uint64_t v;
+                            if (bin_sym<KIND>(S, A[k], B[k], fs, ms, ss, v)) sink.add(fs, ms, ss, v);
+                        }
+                    }
+                }
+                p += nb;
             }
         }
         __syncthreads();
@@ -762,6 +776,21 @@ struct Bufs {
     CellGeom g;
 };
 
+template <int K, bool O>
+void launch_pairs(int pf, dim3 grid, hipStream_t st, const PairRun &R, const Bufs &b, unsigned long long *target,
+                  size_t nvox, unsigned long long *dropped)
+{
+    if (O || pf == 1)
+        hipLaunchKernelGGL((pairs_kernel<K, O, 1>), grid, dim3(kBlock), 0, st, b.srec, R.d_frame_off, b.cs,
+                           b.d_chunks, b.d_tasks, b.ntasks, b.g, R.spec, target, nvox, dropped);
+    else if (pf == 4)
+        hipLaunchKernelGGL((pairs_kernel<K, O, O ? 1 : 4>), grid, dim3(kBlock), 0, st, b.srec, R.d_frame_off, b.cs,
+                           b.d_chunks, b.d_tasks, b.ntasks, b.g, R.spec, target, nvox, dropped);
+    else
+        hipLaunchKernelGGL((pairs_kernel<K, O, O ? 1 : 8>), grid, dim3(kBlock), 0, st, b.srec, R.d_frame_off, b.cs,
+                           b.d_chunks, b.d_tasks, b.ntasks, b.g, R.spec, target, nvox, dropped);
+}
+
 // Everything after planning, enqueue only (no host synchronisation, no
 // allocation, fills by kernel): capturable into a graph.
 int enqueue(iac_ctx *ctx, hipStream_t st, const PairRun &R, const Bufs &b, bool timed)
@@ -794,21 +823,14 @@ int enqueue(iac_ctx *ctx, hipStream_t st, const PairRun &R, const Bufs &b, bool 
     if (b.ntasks && b.nchunks) {
         const dim3 grid(b.ntasks * b.nchunks);
         if (timed) IAC_TRY(timer_begin(ctx));
+        // ordered (linear) binning holds eight corner adds per pair in registers: no batching there
+        const int pf = ordered || ctx->pair_prefetch == 1 ? 1 : (ctx->pair_prefetch == 4 ? 4 : 8);
         if (S.kind == kKindAc3d) {
-            if (ordered)
-                hipLaunchKernelGGL((pairs_kernel<kKindAc3d, true>), grid, dim3(kBlock), 0, st, b.srec, R.d_frame_off,
-                                   b.cs, b.d_chunks, b.d_tasks, b.ntasks, b.g, S, target, nvox, ctx->d_counters);
-            else
-                hipLaunchKernelGGL((pairs_kernel<kKindAc3d, false>), grid, dim3(kBlock), 0, st, b.srec,
-                                   R.d_frame_off, b.cs, b.d_chunks, b.d_tasks, b.ntasks, b.g, S, target, nvox,
-                                   ctx->d_counters);
+            if (ordered) launch_pairs<kKindAc3d, true>(pf, grid, st, R, b, target, nvox, ctx->d_counters);
+            else launch_pairs<kKindAc3d, false>(pf, grid, st, R, b, target, nvox, ctx->d_counters);
         } else {
-            if (ordered)
-                hipLaunchKernelGGL((pairs_kernel<kKindCq, true>), grid, dim3(kBlock), 0, st, b.srec, R.d_frame_off,
-                                   b.cs, b.d_chunks, b.d_tasks, b.ntasks, b.g, S, target, nvox, ctx->d_counters);
-            else
-                hipLaunchKernelGGL((pairs_kernel<kKindCq, false>), grid, dim3(kBlock), 0, st, b.srec, R.d_frame_off,
-                                   b.cs, b.d_chunks, b.d_tasks, b.ntasks, b.g, S, target, nvox, ctx->d_counters);
+            if (ordered) launch_pairs<kKindCq, true>(pf, grid, st, R, b, target, nvox, ctx->d_counters);
+            else launch_pairs<kKindCq, false>(pf, grid, st, R, b, target, nvox, ctx->d_counters);
         }
         IAC_HIP(hipGetLastError());
         if (timed) IAC_TRY(timer_end(ctx));
