@@ -19,6 +19,7 @@ def lds(ctx):
     ctx.set_option(iac.OPT_AC3D_KERNEL, 0)
     ctx.set_option(iac.OPT_CELL_VOX, 0)
     ctx.set_option(iac.OPT_FRAME_CHUNK, 0)
+    ctx.set_option(iac.OPT_PAIR_PREFETCH, 0)
 
 
 def test_lds_cfg1_full(lds, dev):
@@ -43,9 +44,10 @@ def test_lds_cfg2_subset(lds, dev):
     assert np.array_equal(got, ref)
 
 
-@pytest.mark.parametrize("cell_vox,chunk", [(2, 1), (3, 7), (4, 128), (8, 64), (16, 300)])
-def test_lds_tuning_invariance(lds, dev, cell_vox, chunk):
-    """Every cell size and frame chunk gives the same bits (the tile kernel's)."""
+@pytest.mark.parametrize("cell_vox,chunk,prefetch", [(2, 1, 8), (3, 7, 4), (4, 128, 1), (8, 64, 8), (16, 300, 4),
+                                                     (6, 64, 1)])
+def test_lds_tuning_invariance(lds, dev, cell_vox, chunk, prefetch):
+    """Every cell size, frame chunk and pair batch depth gives the same bits (the tile kernel's)."""
     import incohautocorrelate_amd as iac
 
     _, km, mq = W.detector(1024)
@@ -56,6 +58,7 @@ def test_lds_tuning_invariance(lds, dev, cell_vox, chunk):
     lds.set_option(iac.OPT_AC3D_KERNEL, 2)
     lds.set_option(iac.OPT_CELL_VOX, cell_vox)
     lds.set_option(iac.OPT_FRAME_CHUNK, chunk)
+    lds.set_option(iac.OPT_PAIR_PREFETCH, prefetch)
     got = mesh_np(gpu_batch(lds, dev, km, d["pix"], d["cnt"], d["frame_off"], shape))
     assert np.array_equal(got, tile)
 
