@@ -95,14 +95,15 @@ __device__ __forceinline__ void add_rotated(Sink &sink, float q0, float q1, floa
         int ms = (int)floorf(q1) + P.c1;
         int ss = (int)floorf(q2) + P.c2;
         double v = (double)value;
-        sink.add(fs + 0, ms + 0, ss + 0, pair_val_d(v * ((sfs * sms) * sss)));
-        sink.add(fs + 1, ms + 0, ss + 0, pair_val_d(v * (((1 - sfs) * sms) * sss)));
-        sink.add(fs + 0, ms + 0, ss + 1, pair_val_d(v * ((sfs * sms) * (1 - sss))));
-        sink.add(fs + 1, ms + 0, ss + 1, pair_val_d(v * (((1 - sfs) * sms) * (1 - sss))));
-        sink.add(fs + 0, ms + 1, ss + 0, pair_val_d(v * ((sfs * (1 - sms)) * sss)));
-        sink.add(fs + 1, ms + 1, ss + 0, pair_val_d(v * (((1 - sfs) * (1 - sms)) * sss)));
-        sink.add(fs + 0, ms + 1, ss + 1, pair_val_d(v * ((sfs * (1 - sms)) * (1 - sss))));
-        sink.add(fs + 1, ms + 1, ss + 1, pair_val_d(v * (((1 - sfs) * (1 - sms)) * (1 - sss))));
+        // corner (ex, ey, ez) gets v * ((wx * wy) * wz), w = s or 1 - s per axis
+#pragma unroll 1
+        for (int c = 0; c < 8; c++) {
+            const int ex = c & 1, ey = (c >> 1) & 1, ez = (c >> 2) & 1;
+            const double wx = ex ? (1 - sfs) : sfs;
+            const double wy = ey ? (1 - sms) : sms;
+            const double wz = ez ? (1 - sss) : sss;
+            sink.add(fs + ex, ms + ey, ss + ez, pair_val_d(v * ((wx * wy) * wz)));
+        }
     }
 }
 
@@ -132,7 +133,7 @@ __device__ __forceinline__ void add_q_entry(Sink &sink, float ax, float ay, floa
             double sss = 1.0 - (double)(q2 - floorf(q2));
             float f0 = floorf(q0), f1 = floorf(q1), f2 = floorf(q2);
             double v = (double)value;
-#pragma unroll
+#pragma unroll 1  // eight corners, each up to eight second-level adds: keep the code (and registers) small
             for (int c = 0; c < 8; c++) {
                 const int ex = c & 1, ey = (c >> 2) & 1, ez = (c >> 1) & 1;
                 // (float)((fs_l + e) - C) == floor(q) + e exactly for small q
