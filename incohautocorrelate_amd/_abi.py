@@ -121,6 +121,11 @@ def lib() -> C.CDLL:
             raise ImportError(
                 f"{LIB_PATH} is missing: run __graft_entry__.build() (or python "
                 "incohautocorrelate_amd/build.py) before using the HIP path")
+        # torch ships its own HIP runtime (libamdhip64.so.7, librocblas.so.5, ...).  Loading it
+        # first makes libiac_hip.so bind to those same libraries by soname, so the process
+        # holds one HIP runtime whichever of the two a caller touches first.
+        import torch  # noqa: F401
+
         lib_ = C.CDLL(str(LIB_PATH))
         for name, (res, args) in SIGNATURES.items():
             fn = getattr(lib_, name)
