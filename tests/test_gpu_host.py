@@ -146,7 +146,7 @@ def test_average_intensity_entry_point(ctx, dev, thr, step, masked):
 
     from gpu_helpers import to_dev
 
-    rng = np.random.default_rng(int(thr * 10) + 3)
+    rng = np.random.default_rng(abs(int(thr * 10)) + 3)
     fr = (rng.poisson(0.4, (13, 3001)) * 6.4 + rng.normal(0, 1.0, (13, 3001))).astype(np.float32)
     mask = (rng.random(3001) > 0.1).astype(np.int32) if masked else None
     avg, mi, pc = O.average_intensity(fr, thr, step, mask)
