@@ -146,8 +146,9 @@ int iac_ac3d_records_batch(iac_ctx *ctx, const float *d_kmap, uint32_t npix, con
  * iac_ac3d_batch_launch then enqueues gather + cell sort + pair pass (+ mirror) with
  * no host synchronisation and no allocation.  iac_ac3d_batch_capture records that
  * sequence into a HIP graph for (d_mesh, zero_mesh); later launches with the same pair
- * replay the graph.  d_pix / d_cnt / d_frame_off are kept by pointer: their contents
- * may change between launches with the frame count and n_total unchanged.  zero_mesh
+ * replay the graph.  d_kmap / d_pix / d_cnt / d_frame_off are kept by pointer and must
+ * outlive the batch; the photon lists may change between launches with the frame count
+ * and n_total unchanged.  zero_mesh
  * clears d_mesh first (by kernel).  No counterpart in the reference: it is the
  * frame loop of IAC/RunIAC.cpp:524-550 as one replayable unit. */
 typedef struct iac_ac3d_batch iac_ac3d_batch;

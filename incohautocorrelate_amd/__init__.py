@@ -282,7 +282,7 @@ class Context:
         check(lib().iac_ac3d_batch_create(self._h, _ptr(kmap), kmap.numel() // 3, _ptr(pix), _ptr(cnt),
                                           _ptr(frame_off), nframes, n_total, C.byref(shape), interp1, interp2,
                                           int(double_map), C.byref(h)), "iac_ac3d_batch_create")
-        b = Ac3dBatch(self, h)
+        b = Ac3dBatch(self, h, (kmap, pix, cnt, frame_off))
         self._plans.add(b)
         return b
 
@@ -331,10 +331,11 @@ class Context:
 
 
 class Ac3dBatch:
-    """Owns an iac_ac3d_batch: launch() enqueues one pass, capture() records it as a graph."""
+    """Owns an iac_ac3d_batch: launch() enqueues one pass, capture() records it as a graph.
+    The batch reads its input tensors by pointer at every launch, so it keeps them alive."""
 
-    def __init__(self, ctx: Context, handle: C.c_void_p):
-        self.ctx, self.handle = ctx, handle
+    def __init__(self, ctx: Context, handle: C.c_void_p, inputs: tuple):
+        self.ctx, self.handle, self._inputs = ctx, handle, inputs
 
     def launch(self, mesh, zero_mesh: bool = False) -> None:
         check(lib().iac_ac3d_batch_launch(self.ctx.handle, self.handle, _ptr(mesh), int(zero_mesh)),
