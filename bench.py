@@ -59,6 +59,7 @@ def parse():
     ap.add_argument("--prefetch", type=int, default=0, help="fix the pair batch depth 1/4/8 (with the two above)")
     ap.add_argument("--no-sim", action="store_true", help="skip the simulator leg")
     ap.add_argument("--no-uneven", action="store_true", help="skip the uneven-batch leg")
+    ap.add_argument("--no-probes", action="store_true", help="skip the copy / atomic ceiling probes")
     ap.add_argument("--uneven-only", action="store_true", help="run only the uneven-batch leg")
     ap.add_argument("--sim-only", action="store_true", help="run only the simulator leg (prints its JSON object)")
     ap.add_argument("--sim-frames", type=int, default=1000)
@@ -160,6 +161,7 @@ def main():
 
     del d_pix, d_cnt, d_off, d_mesh
     torch.cuda.empty_cache()
+    probes = None if args.no_probes else probe_leg(iac, dev, nvox)
     uneven = None if args.no_uneven else uneven_leg(args, iac, W, dev)
     sim = None if args.no_sim else sim_leg(args, iac, W, dev)
 
@@ -196,8 +198,11 @@ def main():
             "kernel": "ac3d_tiles_kernel<true>" if args.kernel == 1 else "pairs_lds_kernel",
             "kernel_avg_ms": k_avg_ms,
             "bytes_per_update": BYTES_PER_UPDATE,
+            "frac_of_measured_copy": achieved / probes["hbm_copy_GBps"] if probes else None,
+            "updates_vs_uniform_u64_atomics": value / probes["atomic_u64_per_s"]["uniform"] if probes else None,
         },
         "cpu_baseline": cpu,
+        "probes": probes,
         "uneven_batch": uneven,
         "simulator": sim,
     }
@@ -300,6 +305,34 @@ def cpu_baseline(kmap, data, shape, ctx, iac, d_kmap, dev, args):
         "sample": f"first {nf} of the {len(off) - 1} frames ({upd} pair-updates), "
                   f"OpenMP {cores} threads, -O3 -fopenmp -ffp-contract=off",
     }
+
+
+def probe_leg(iac, dev, nvox):
+    """Measured ceilings: a 2 GiB device copy (HBM), uint64 atomic adds uniform over
+    the 257^3 mesh, over a hot window of the occupied slab's size (7.3e5 voxels), and
+    into LDS boxes."""
+    import torch
+
+    ctx = iac.Context(0)
+    src = torch.empty(2 << 30, dtype=torch.uint8, device=dev)
+    dst = torch.empty_like(src)
+    ctx.probe_copy(src, dst)
+    ms = min(ctx.probe_copy(src, dst) for _ in range(5))
+    copy_gbs = 2 * src.numel() / (ms * 1e-3) / 1e9
+    del src, dst
+    mesh = torch.zeros(nvox, dtype=torch.int64, device=dev)
+    n = 1 << 30
+    rates = {}
+    for mode, name, hot in ((0, "uniform", 0), (1, "hot_slab", 730000), (2, "lds_box", 0)):
+        ctx.probe_atomics(mesh, n >> 4, mode, hot)
+        t = min(ctx.probe_atomics(mesh, n, mode, hot) for _ in range(3))
+        rates[name] = n / (t * 1e-3)
+    log(f"[probe] copy {copy_gbs:.0f} GB/s; u64 atomics/s uniform {rates['uniform']:.3e}, "
+        f"hot slab {rates['hot_slab']:.3e}, LDS {rates['lds_box']:.3e}")
+    ctx.close()
+    return {"hbm_copy_GBps": copy_gbs, "atomic_u64_per_s": rates,
+            "copy": "2 GiB device-to-device, 16-byte vector loads/stores, best of 5",
+            "atomics": "2^30 adds of 1, best of 3"}
 
 
 def uneven_leg(args, iac, W, dev):

@@ -88,7 +88,7 @@ int iac_ctx_timing(iac_ctx *ctx, double *ms, uint64_t *launches, int reset);
 #define IAC_OPT_AC3D_KERNEL 1 /* 0 automatic, 1 global-atomic tile kernel, 2 LDS cell-box kernel */
 #define IAC_OPT_CELL_VOX 2    /* LDS kernel: cell side in voxels, 0 = default (6 for AC_uw, 4 for C(q)) */
 #define IAC_OPT_FRAME_CHUNK 3 /* LDS kernel: frames per block, 0 = default (64) */
-#define IAC_OPT_PAIR_PREFETCH 5 /* LDS kernel: record pairs loaded per batch, 1, 4 or 8 (0 = 8) */
+#define IAC_OPT_PAIR_PREFETCH 5 /* LDS kernel: record pairs loaded per batch, 1, 4 or 8 (0 = 1) */
 #define IAC_OPT_SIM_CHUNK_ROWS 4 /* simulator: detector rows (pixel x subpixel) per GEMM, 0 = auto (1 GiB of FP32 field) */
 int iac_ctx_set_option(iac_ctx *ctx, int option, int64_t value);
 /* Updates whose voxel fell outside the mesh since the context was created.
@@ -299,6 +299,15 @@ int iac_sim_direct_fp32(iac_ctx *ctx, const float *d_pixmap, uint32_t npix, int 
  * random_device is neither). */
 int iac_sim_postprocess(iac_ctx *ctx, float *d_frames, uint32_t npix, uint32_t nframes, double n_expected,
                         int do_poisson, float adu_noise, float value_per_photon, uint64_t seed, uint32_t frame0);
+
+/* ---- measurement probes (no counterpart in the reference) ------------------------ */
+/* n_updates uint64 atomic adds of 1 into d_mesh (nvox >= 4096): mode 0 uniform over the
+ * mesh, mode 1 over a window of `hot` voxels in its middle, mode 2 into a 4096-voxel LDS
+ * box per block flushed once.  *ms = kernel time (HIP events on the context's stream). */
+int iac_probe_atomics(iac_ctx *ctx, uint64_t *d_mesh, uint64_t nvox, uint64_t n_updates, int mode, uint64_t hot,
+                      float *ms);
+/* A plain 16-byte-vector copy of `bytes` (a multiple of 16); *ms = kernel time. */
+int iac_probe_copy(iac_ctx *ctx, const void *d_src, void *d_dst, size_t bytes, float *ms);
 
 #ifdef __cplusplus
 }

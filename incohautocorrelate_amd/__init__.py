@@ -286,6 +286,19 @@ class Context:
         self._plans.add(b)
         return b
 
+    # ---- measurement probes ----
+    def probe_atomics(self, mesh, n_updates: int, mode: int, hot: int = 0) -> float:
+        ms = C.c_float()
+        check(lib().iac_probe_atomics(self._h, _ptr(mesh), mesh.numel(), n_updates, mode, hot, C.byref(ms)),
+              "iac_probe_atomics")
+        return ms.value
+
+    def probe_copy(self, src, dst) -> float:
+        ms = C.c_float()
+        nbytes = src.numel() * src.element_size()
+        check(lib().iac_probe_copy(self._h, _ptr(src), _ptr(dst), nbytes, C.byref(ms)), "iac_probe_copy")
+        return ms.value
+
     # ---- simulator (Simulator::SimulatePart, IAC/Simulator.cpp:662-944) ----
     def sim_plan(self, pixmap, emitters, wavelength: float, mode: int = SIM_BF16X3, subsampling: int = 0,
                  u_step=(0.0, 0.0, 0.0), v_step=(0.0, 0.0, 0.0)) -> "SimPlan":

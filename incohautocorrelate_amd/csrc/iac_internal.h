@@ -33,7 +33,7 @@ struct iac_ctx {
     int cell_vox = 0;
     uint32_t frame_chunk = 0;
     uint64_t sim_chunk_rows = 0;
-    int pair_prefetch = 0;  // pair engine: pairs per load batch (1, 4 or 8; 0 = 8)
+    int pair_prefetch = 0;  // pair engine: pairs per load batch (1, 4 or 8; 0 = 1)
 };
 
 namespace iac {

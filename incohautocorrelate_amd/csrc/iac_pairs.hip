@@ -411,9 +411,9 @@ __global__ void __launch_bounds__(kBlock)
         }
         __syncthreads();
 
-        // phase 2: thread t bins the contiguous pair range [p0, p1), PF pairs at a
-        // time: the enumeration is integer-only, so the 2 PF record loads of a batch
-        // are independent and in flight together before the batch is binned.
+        // phase 2: thread t bins the contiguous pair range [p0, p1).  PF == 1: pair
+        // by pair, one load each.  PF > 1: PF pairs at a time with an integer-only
+        // enumeration, so the 2 PF record loads of a batch are in flight together.
         const uint32_t total = pre[kGroup];
         const uint32_t p0 = (uint32_t)(((uint64_t)total * threadIdx.x) / kBlock);
         const uint32_t p1 = (uint32_t)(((uint64_t)total * (threadIdx.x + 1)) / kBlock);
@@ -438,6 +438,35 @@ __global__ void __launch_bounds__(kBlock)
                 ia = r / d.w;
                 ib = r % d.w;
             }
+            if (PF == 1) {
+                // one record load per pair: the a record stays in a register along its row
+                float4 ra = srec[d.x + ia];
+                for (uint32_t p = p0;;) {
+                    const float4 rb = srec[d.z + ib];
+                    if (ORD) {
+                        bin_ordered<KIND>(S, ra, rb, sink);
+                    } else {
+                        int fs, ms, ss;
+                        uint64_t v;
+                        if (bin_sym<KIND>(S, ra, rb, fs, ms, ss, v)) sink.add(fs, ms, ss, v);
+                    }
+                    if (++p == p1) break;
+                    ib++;
+                    if (ib >= (tri ? d.y : d.w)) {
+                        ia++;
+                        ib = tri ? ia + 1 : 0;
+                        if ((tri ? ib : ia) >= d.y) {  // item exhausted: next item with pairs
+                            do {
+                                u++;
+                            } while (pre[u + 1] == pre[u]);
+                            d = item[u];
+                            ia = 0;
+                            ib = tri ? 1 : 0;
+                        }
+                        ra = srec[d.x + ia];
+                    }
+                }
+            } else
             for (uint32_t p = p0; p < p1;) {
                 const uint32_t nb = min((uint32_t)PF, p1 - p);
                 float4 A[PF], B[PF];
@@ -824,7 +853,7 @@ int enqueue(iac_ctx *ctx, hipStream_t st, const PairRun &R, const Bufs &b, bool 
         const dim3 grid(b.ntasks * b.nchunks);
         if (timed) IAC_TRY(timer_begin(ctx));
         // ordered (linear) binning holds eight corner adds per pair in registers: no batching there
-        const int pf = ordered || ctx->pair_prefetch == 1 ? 1 : (ctx->pair_prefetch == 4 ? 4 : 8);
+        const int pf = ordered || ctx->pair_prefetch == 0 || ctx->pair_prefetch == 1 ? 1 : (ctx->pair_prefetch == 4 ? 4 : 8);
         if (S.kind == kKindAc3d) {
             if (ordered) launch_pairs<kKindAc3d, true>(pf, grid, st, R, b, target, nvox, ctx->d_counters);
             else launch_pairs<kKindAc3d, false>(pf, grid, st, R, b, target, nvox, ctx->d_counters);
