@@ -60,3 +60,30 @@ def test_int_to_float():
     L = O.lib()
     for i in [0, 1, 999, 1000, 123456789, 2 ** 40]:
         assert L.ora_int_to_float_u64(i) == float(f32(f32(i) * f32(0.001)))
+
+
+def test_round_half_away_by_biased_truncation():
+    """The pair engine's fast path rounds q * vox with trunc(x + copysign(0.49999997f, x))
+    (float32 addition, round to nearest).  It equals roundf for every float with
+    |x| < 2^23 (checked exhaustively once, 2.5e9 values); here: every half-integer and
+    its two float neighbours up to 2^21, plus 4M random floats over the whole range."""
+    import numpy as np
+
+    h = np.float32(0.49999997)
+    assert h == np.nextafter(np.float32(0.5), np.float32(0))
+
+    def ref(x):
+        x64 = x.astype(np.float64)
+        return np.trunc(x64 + np.copysign(0.5, x64))
+
+    def fast(x):
+        return np.trunc((x + np.copysign(h, x)).astype(np.float32)).astype(np.float64)
+
+    half = (np.arange(0, 1 << 21, dtype=np.float64) + 0.5).astype(np.float32)
+    cand = np.concatenate([half, np.nextafter(half, np.float32(0)), np.nextafter(half, np.float32(1 << 23))])
+    cand = np.concatenate([cand, -cand])
+    assert np.array_equal(fast(cand), ref(cand))
+    rng = np.random.default_rng(0)
+    bits = rng.integers(0, 1 << 32, 4 << 20, dtype=np.uint64).astype(np.uint32).view(np.float32)
+    x = bits[np.isfinite(bits) & (np.abs(bits) < np.float32(1 << 23))]
+    assert np.array_equal(fast(x), ref(x))
