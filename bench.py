@@ -57,6 +57,7 @@ def parse():
     ap.add_argument("--cell-vox", type=int, default=0, help="fix the LDS cell side (skips the sweep)")
     ap.add_argument("--frame-chunk", type=int, default=0, help="fix the LDS frames per block (skips the sweep)")
     ap.add_argument("--prefetch", type=int, default=0, help="fix the pair batch depth 1/4/8 (with the two above)")
+    ap.add_argument("--box", type=int, default=0, help="fix the LDS box capacity 2048/4096 (with the above)")
     ap.add_argument("--no-sim", action="store_true", help="skip the simulator leg")
     ap.add_argument("--no-uneven", action="store_true", help="skip the uneven-batch leg")
     ap.add_argument("--no-probes", action="store_true", help="skip the copy / atomic ceiling probes")
@@ -114,12 +115,13 @@ def main():
 
     ctx.set_option(iac.OPT_AC3D_KERNEL, args.kernel)
     tuned = None
-    if args.cell_vox or args.frame_chunk or args.prefetch:
+    if args.cell_vox or args.frame_chunk or args.prefetch or args.box:
         ctx.set_option(iac.OPT_CELL_VOX, args.cell_vox)
         ctx.set_option(iac.OPT_FRAME_CHUNK, args.frame_chunk)
         ctx.set_option(iac.OPT_PAIR_PREFETCH, args.prefetch)
+        ctx.set_option(iac.OPT_BOX_CAP, args.box)
         tuned = {"cell_vox": args.cell_vox, "frame_chunk": args.frame_chunk, "prefetch": args.prefetch,
-                 "fixed": True}
+                 "box": args.box, "fixed": True}
     elif args.kernel != 1 and not args.no_tune:
         tuned = autotune(ctx, iac, d_kmap, d_pix, d_cnt, data, shape, d_mesh, dev)
 
@@ -221,27 +223,29 @@ def autotune(ctx, iac, d_kmap, d_pix, d_cnt, data, shape, d_mesh, dev):
     d_off = torch.from_numpy(off[: nf + 1].view(np.int64)).to(dev)
     best = None
     ctx.enable_timing(True)
-    for pf in (1, 8):
-        for cv in (4, 5, 6, 8):
-            for fc in (32, 64, 128):
-                ctx.set_option(iac.OPT_PAIR_PREFETCH, pf)
-                ctx.set_option(iac.OPT_CELL_VOX, cv)
-                ctx.set_option(iac.OPT_FRAME_CHUNK, fc)
-                ctx.ac3d_sparse_batch(d_kmap, d_pix[:n], d_cnt[:n], d_off, n, shape, d_mesh)  # plan
-                ctx.kernel_time(reset=True)
-                for _ in range(2):
-                    ctx.ac3d_sparse_batch(d_kmap, d_pix[:n], d_cnt[:n], d_off, n, shape, d_mesh)
-                ms, k = ctx.kernel_time(reset=True)
-                t = ms / max(k, 1)
-                log(f"[tune] prefetch={pf} cell_vox={cv} frame_chunk={fc}: {t:.3f} ms / {nf} frames")
-                if best is None or t < best[0]:
-                    best = (t, cv, fc, pf)
+    grid = [(box, cv, fc, 1) for box in (4096, 2048) for cv in (4, 5, 6, 7) for fc in (32, 64)]
+    grid.append((4096, 6, 32, 8))
+    for box, cv, fc, pf in grid:
+        ctx.set_option(iac.OPT_BOX_CAP, box)
+        ctx.set_option(iac.OPT_PAIR_PREFETCH, pf)
+        ctx.set_option(iac.OPT_CELL_VOX, cv)
+        ctx.set_option(iac.OPT_FRAME_CHUNK, fc)
+        ctx.ac3d_sparse_batch(d_kmap, d_pix[:n], d_cnt[:n], d_off, n, shape, d_mesh)  # plan
+        ctx.kernel_time(reset=True)
+        for _ in range(2):
+            ctx.ac3d_sparse_batch(d_kmap, d_pix[:n], d_cnt[:n], d_off, n, shape, d_mesh)
+        ms, k = ctx.kernel_time(reset=True)
+        t = ms / max(k, 1)
+        log(f"[tune] box={box} prefetch={pf} cell_vox={cv} frame_chunk={fc}: {t:.3f} ms / {nf} frames")
+        if best is None or t < best[0]:
+            best = (t, cv, fc, pf, box)
     ctx.enable_timing(False)
     ctx.set_option(iac.OPT_CELL_VOX, best[1])
     ctx.set_option(iac.OPT_FRAME_CHUNK, best[2])
     ctx.set_option(iac.OPT_PAIR_PREFETCH, best[3])
-    log(f"[tune] chose cell_vox={best[1]} frame_chunk={best[2]} prefetch={best[3]}")
-    return {"cell_vox": best[1], "frame_chunk": best[2], "prefetch": best[3]}
+    ctx.set_option(iac.OPT_BOX_CAP, best[4])
+    log(f"[tune] chose box={best[4]} cell_vox={best[1]} frame_chunk={best[2]} prefetch={best[3]}")
+    return {"cell_vox": best[1], "frame_chunk": best[2], "prefetch": best[3], "box": best[4]}
 
 
 def check_subset(iac, W, kmap, data, shape, dev):
@@ -361,24 +365,26 @@ def uneven_leg(args, iac, W, dev):
     if not args.no_tune:
         best = None
         ctx.enable_timing(True)
-        for cv in (4, 6, 8):
-            for fc in (32, 64, 128):
-                ctx.set_option(iac.OPT_PAIR_PREFETCH, 8)
-                ctx.set_option(iac.OPT_CELL_VOX, cv)
-                ctx.set_option(iac.OPT_FRAME_CHUNK, fc)
-                b = ctx.ac3d_batch(d_kmap, d_pix, d_cnt, d_off, n_total, shape)
-                b.launch(d_mesh, True)
-                ctx.kernel_time(reset=True)
-                b.launch(d_mesh, True)
-                ms, k = ctx.kernel_time(reset=True)
-                b.destroy()
-                log(f"[uneven tune] cell_vox={cv} frame_chunk={fc}: {ms:.2f} ms")
-                if best is None or ms < best[0]:
-                    best = (ms, cv, fc)
+        for box in (4096, 2048):
+            for cv in (5, 6):
+                for fc in (32, 64):
+                    ctx.set_option(iac.OPT_BOX_CAP, box)
+                    ctx.set_option(iac.OPT_CELL_VOX, cv)
+                    ctx.set_option(iac.OPT_FRAME_CHUNK, fc)
+                    b = ctx.ac3d_batch(d_kmap, d_pix, d_cnt, d_off, n_total, shape)
+                    b.launch(d_mesh, True)
+                    ctx.kernel_time(reset=True)
+                    b.launch(d_mesh, True)
+                    ms, k = ctx.kernel_time(reset=True)
+                    b.destroy()
+                    log(f"[uneven tune] box={box} cell_vox={cv} frame_chunk={fc}: {ms:.2f} ms")
+                    if best is None or ms < best[0]:
+                        best = (ms, cv, fc, box)
         ctx.enable_timing(False)
         ctx.set_option(iac.OPT_CELL_VOX, best[1])
         ctx.set_option(iac.OPT_FRAME_CHUNK, best[2])
-        tuned = {"cell_vox": best[1], "frame_chunk": best[2]}
+        ctx.set_option(iac.OPT_BOX_CAP, best[3])
+        tuned = {"cell_vox": best[1], "frame_chunk": best[2], "box": best[3]}
     b = ctx.ac3d_batch(d_kmap, d_pix, d_cnt, d_off, n_total, shape)
     b.capture(d_mesh, True)
     b.launch(d_mesh, True)

@@ -36,9 +36,15 @@ namespace iac {
 namespace {
 
 constexpr int kBlock = 256;
-constexpr int kItemsPerThread = 4;
-constexpr int kGroup = kBlock * kItemsPerThread;  // items per scan group
-constexpr int kBoxMax = 4096;                      // LDS box capacity (uint64 voxels, 32 KiB)
+// LDS box capacity (uint64 voxels): 4096 (32 KiB, 1024 items per scan group, 3 blocks
+// per CU) or 2048 (16 KiB, 512 items, 6 blocks per CU); chosen per plan.
+constexpr int kBoxMax = 4096;
+constexpr int kBoxSmall = 2048;
+template <int BOX>
+struct BoxCfg {
+    static constexpr int group = BOX == kBoxMax ? 1024 : 512;  // items per scan group
+    static constexpr int ipt = group / kBlock;                  // items per thread
+};
 constexpr int kMaxCells = 4096;                    // NC_u * NC_v
 
 struct CellGeom {
@@ -353,7 +359,7 @@ struct BoxSink {
     }
 };
 
-template <int KIND, bool ORD, int PF, bool FAST>
+template <int KIND, bool ORD, int PF, bool FAST, int BOX>
 __global__ void __launch_bounds__(kBlock)
     pairs_kernel(const float4 *__restrict__ srec, const uint64_t *__restrict__ off,
                  const uint32_t *__restrict__ cs, const Chunk *__restrict__ chunks,
@@ -361,7 +367,9 @@ __global__ void __launch_bounds__(kBlock)
                  unsigned long long *__restrict__ target, size_t nvox,
                  unsigned long long *__restrict__ dropped)
 {
-    __shared__ unsigned long long box[kBoxMax];
+    constexpr int kGroup = BoxCfg<BOX>::group;
+    constexpr int kItemsPerThread = BoxCfg<BOX>::ipt;
+    __shared__ unsigned long long box[BOX];
     __shared__ uint4 item[kGroup];        // {a base, na, b base, nb}
     __shared__ uint32_t pre[kGroup + 1];  // exclusive prefix of pair counts
     __shared__ uint32_t wsum[kBlock];
@@ -631,6 +639,7 @@ struct iac_pair_plan {
     const float *kmap = nullptr;
     uint32_t npix = 0;
     int cell_vox = 0;
+    int box_cap = 0;  // LDS box capacity the tasks were sized for
     iac::PairSpec spec{};
     iac::CellGeom g{};
     std::vector<uint32_t> cell_ext;  // 6 keys per cell
@@ -705,7 +714,7 @@ void build_tasks(iac_pair_plan &pl, bool ordered)
                 ihi[d] = (int)std::min((double)dim[d] - 1, hi[d] + cen[d]);
             }
             const long vol = (long)(ihi[0] - ilo[0] + 1) * (ihi[1] - ilo[1] + 1) * (ihi[2] - ilo[2] + 1);
-            if (ihi[0] >= ilo[0] && ihi[1] >= ilo[1] && ihi[2] >= ilo[2] && vol <= kBoxMax) {
+            if (ihi[0] >= ilo[0] && ihi[1] >= ilo[1] && ihi[2] >= ilo[2] && vol <= pl.box_cap) {
                 t.bx0 = ilo[0];
                 t.by0 = ilo[1];
                 t.bz0 = ilo[2];
@@ -718,17 +727,19 @@ void build_tasks(iac_pair_plan &pl, bool ordered)
     }
 }
 
-iac_pair_plan *find_plan(iac_ctx *ctx, const float *kmap, uint32_t npix, int cell_vox, const PairSpec &S)
+iac_pair_plan *find_plan(iac_ctx *ctx, const float *kmap, uint32_t npix, int cell_vox, int box_cap,
+                         const PairSpec &S)
 {
     for (auto *p : ctx->pair_plans)
-        if (p->valid && p->kmap == kmap && p->npix == npix && p->cell_vox == cell_vox && same_spec(p->spec, S))
+        if (p->valid && p->kmap == kmap && p->npix == npix && p->cell_vox == cell_vox && p->box_cap == box_cap &&
+            same_spec(p->spec, S))
             return p;
     return nullptr;
 }
 
 // Cell grid of the k-map (two host round trips for the extents) and the task
 // table, from the context's cache when the geometry matches.
-int get_plan(iac_ctx *ctx, hipStream_t st, const PairRun &R, int cell_vox, iac_pair_plan **out)
+int get_plan(iac_ctx *ctx, hipStream_t st, const PairRun &R, int cell_vox, int box_cap, iac_pair_plan **out)
 {
     const PairSpec &S = R.spec;
     void *ext = nullptr;
@@ -767,7 +778,7 @@ int get_plan(iac_ctx *ctx, hipStream_t st, const PairRun &R, int cell_vox, iac_p
     std::vector<uint32_t> cext(6 * (size_t)nc);
     IAC_HIP(hipMemcpyAsync(cext.data(), d_ext + 6, cext.size() * 4, hipMemcpyDeviceToHost, st));
     IAC_HIP(hipStreamSynchronize(st));
-    iac_pair_plan *pl = find_plan(ctx, R.d_kmap, R.npix, cell_vox, S);
+    iac_pair_plan *pl = find_plan(ctx, R.d_kmap, R.npix, cell_vox, box_cap, S);
     if (pl && (std::memcmp(&pl->g, &g, sizeof(g)) != 0 || pl->cell_ext != cext)) pl = nullptr;
     if (!pl) {
         // reuse the oldest slot beyond four cached plans
@@ -782,6 +793,7 @@ int get_plan(iac_ctx *ctx, hipStream_t st, const PairRun &R, int cell_vox, iac_p
         pl->kmap = R.d_kmap;
         pl->npix = R.npix;
         pl->cell_vox = cell_vox;
+        pl->box_cap = box_cap;
         pl->spec = S;
         pl->g = g;
         pl->cell_ext.swap(cext);
@@ -839,28 +851,39 @@ struct Bufs {
     const Task *d_tasks;
     uint32_t ntasks;
     CellGeom g;
+    int box_cap;
 };
+
+template <int K, bool O, int BOX>
+void launch_pairs_box(int pf, bool fast, dim3 grid, hipStream_t st, const PairRun &R, const Bufs &b,
+                      unsigned long long *target, size_t nvox, unsigned long long *dropped)
+{
+    if ((O || pf == 1) && fast)
+        hipLaunchKernelGGL((pairs_kernel<K, O, 1, K == kKindAc3d && !O, BOX>), grid, dim3(kBlock), 0, st, b.srec,
+                           R.d_frame_off, b.cs, b.d_chunks, b.d_tasks, b.ntasks, b.g, R.spec, target, nvox, dropped);
+    else if (O || pf == 1 || BOX != kBoxMax)
+        hipLaunchKernelGGL((pairs_kernel<K, O, 1, false, BOX>), grid, dim3(kBlock), 0, st, b.srec, R.d_frame_off,
+                           b.cs, b.d_chunks, b.d_tasks, b.ntasks, b.g, R.spec, target, nvox, dropped);
+    else if (pf == 4)
+        hipLaunchKernelGGL((pairs_kernel<K, O, O ? 1 : 4, false, BOX>), grid, dim3(kBlock), 0, st, b.srec,
+                           R.d_frame_off, b.cs, b.d_chunks, b.d_tasks, b.ntasks, b.g, R.spec, target, nvox, dropped);
+    else
+        hipLaunchKernelGGL((pairs_kernel<K, O, O ? 1 : 8, false, BOX>), grid, dim3(kBlock), 0, st, b.srec,
+                           R.d_frame_off, b.cs, b.d_chunks, b.d_tasks, b.ntasks, b.g, R.spec, target, nvox, dropped);
+}
 
 template <int K, bool O>
 void launch_pairs(int pf, dim3 grid, hipStream_t st, const PairRun &R, const Bufs &b, unsigned long long *target,
                   size_t nvox, unsigned long long *dropped)
 {
-    // the specialised pair path: AC_uw, symmetric (nearest-neighbour) binning, identity rotation
-    // round_away needs |q * vox| < 2^23; the cut bounds |q| by Max_Q
+    // the specialised pair path: AC_uw, symmetric (nearest-neighbour) binning, identity rotation;
+    // round_away needs |q * vox| < 2^23 and the cut bounds |q| by Max_Q
     const bool fast = K == kKindAc3d && !O && R.spec.P.rot_identity &&
                       std::sqrt((double)R.spec.P.cut_sq) * R.spec.P.vox < 4194304.0;
-    if ((O || pf == 1) && fast)
-        hipLaunchKernelGGL((pairs_kernel<K, O, 1, K == kKindAc3d && !O>), grid, dim3(kBlock), 0, st, b.srec,
-                           R.d_frame_off, b.cs, b.d_chunks, b.d_tasks, b.ntasks, b.g, R.spec, target, nvox, dropped);
-    else if (O || pf == 1)
-        hipLaunchKernelGGL((pairs_kernel<K, O, 1, false>), grid, dim3(kBlock), 0, st, b.srec, R.d_frame_off, b.cs,
-                           b.d_chunks, b.d_tasks, b.ntasks, b.g, R.spec, target, nvox, dropped);
-    else if (pf == 4)
-        hipLaunchKernelGGL((pairs_kernel<K, O, O ? 1 : 4, false>), grid, dim3(kBlock), 0, st, b.srec, R.d_frame_off,
-                           b.cs, b.d_chunks, b.d_tasks, b.ntasks, b.g, R.spec, target, nvox, dropped);
+    if (b.box_cap == kBoxSmall)
+        launch_pairs_box<K, O, kBoxSmall>(pf, fast, grid, st, R, b, target, nvox, dropped);
     else
-        hipLaunchKernelGGL((pairs_kernel<K, O, O ? 1 : 8, false>), grid, dim3(kBlock), 0, st, b.srec, R.d_frame_off,
-                           b.cs, b.d_chunks, b.d_tasks, b.ntasks, b.g, R.spec, target, nvox, dropped);
+        launch_pairs_box<K, O, kBoxMax>(pf, fast, grid, st, R, b, target, nvox, dropped);
 }
 
 // Everything after planning, enqueue only (no host synchronisation, no
@@ -917,6 +940,7 @@ int enqueue(iac_ctx *ctx, hipStream_t st, const PairRun &R, const Bufs &b, bool 
 }
 
 int cell_vox_of(iac_ctx *ctx, const PairSpec &S) { return ctx->cell_vox > 0 ? ctx->cell_vox : (S.kind == kKindAc3d ? 6 : 4); }
+int box_cap_of(iac_ctx *ctx) { return ctx->box_cap == kBoxSmall ? kBoxSmall : kBoxMax; }
 uint32_t fchunk_of(iac_ctx *ctx) { return ctx->frame_chunk > 0 ? ctx->frame_chunk : 64; }
 
 // byte sizes of the per-run buffers
@@ -951,7 +975,7 @@ int pairs_run(iac_ctx *ctx, const PairRun &R)
     hipStream_t st = ctx->stream;
     if (R.n_total >= (1ull << 32)) return fail(IAC_ERR_LIMIT, "pair engine: at most 2^32 - 1 records per call");
     iac_pair_plan *pl = nullptr;
-    IAC_TRY(get_plan(ctx, st, R, cell_vox_of(ctx, R.spec), &pl));
+    IAC_TRY(get_plan(ctx, st, R, cell_vox_of(ctx, R.spec), box_cap_of(ctx), &pl));
     std::vector<uint64_t> off(R.nframes + 1);
     IAC_HIP(hipMemcpyAsync(off.data(), R.d_frame_off, off.size() * 8, hipMemcpyDeviceToHost, st));
     IAC_HIP(hipStreamSynchronize(st));
@@ -974,6 +998,7 @@ int pairs_run(iac_ctx *ctx, const PairRun &R)
     b.d_tasks = pl->d_tasks;
     b.ntasks = (uint32_t)pl->tasks.size();
     b.g = pl->g;
+    b.box_cap = pl->box_cap;
     IAC_HIP(hipMemcpyAsync(b.d_goff, R.h_group_off, (R.ngroups + 1) * sizeof(uint32_t), hipMemcpyHostToDevice, st));
     if (!chunks.empty())
         IAC_HIP(hipMemcpyAsync(b.d_chunks, chunks.data(), chunks.size() * sizeof(Chunk), hipMemcpyHostToDevice, st));
@@ -997,7 +1022,7 @@ int pairs_prepare(iac_ctx *ctx, const PairRun &R0, PairBatch **out)
     hipStream_t st = ctx->stream;
     if (R0.n_total >= (1ull << 32)) return fail(IAC_ERR_LIMIT, "pair engine: at most 2^32 - 1 records per call");
     iac_pair_plan *pl = nullptr;
-    IAC_TRY(get_plan(ctx, st, R0, cell_vox_of(ctx, R0.spec), &pl));
+    IAC_TRY(get_plan(ctx, st, R0, cell_vox_of(ctx, R0.spec), box_cap_of(ctx), &pl));
     std::vector<uint64_t> off(R0.nframes + 1);
     IAC_HIP(hipMemcpyAsync(off.data(), R0.d_frame_off, off.size() * 8, hipMemcpyDeviceToHost, st));
     IAC_HIP(hipStreamSynchronize(st));
@@ -1029,6 +1054,7 @@ int pairs_prepare(iac_ctx *ctx, const PairRun &R0, PairBatch **out)
     b.d_tasks = tasks;
     b.ntasks = (uint32_t)pl->tasks.size();
     b.g = pl->g;
+    b.box_cap = pl->box_cap;
     e = hipMemcpyAsync(b.d_goff, B->group_off.data(), B->group_off.size() * 4, hipMemcpyHostToDevice, st);
     if (e == hipSuccess && !chunks.empty())
         e = hipMemcpyAsync(b.d_chunks, chunks.data(), chunks.size() * sizeof(Chunk), hipMemcpyHostToDevice, st);

@@ -20,6 +20,7 @@ def lds(ctx):
     ctx.set_option(iac.OPT_CELL_VOX, 0)
     ctx.set_option(iac.OPT_FRAME_CHUNK, 0)
     ctx.set_option(iac.OPT_PAIR_PREFETCH, 0)
+    ctx.set_option(iac.OPT_BOX_CAP, 0)
 
 
 def test_lds_cfg1_full(lds, dev):
@@ -44,9 +45,10 @@ def test_lds_cfg2_subset(lds, dev):
     assert np.array_equal(got, ref)
 
 
-@pytest.mark.parametrize("cell_vox,chunk,prefetch", [(2, 1, 8), (3, 7, 4), (4, 128, 1), (8, 64, 8), (16, 300, 4),
-                                                     (6, 64, 1)])
-def test_lds_tuning_invariance(lds, dev, cell_vox, chunk, prefetch):
+@pytest.mark.parametrize("cell_vox,chunk,prefetch,box", [(2, 1, 8, 4096), (3, 7, 4, 4096), (4, 128, 1, 4096),
+                                                         (8, 64, 8, 4096), (16, 300, 4, 4096), (6, 64, 1, 4096),
+                                                         (5, 32, 1, 2048), (6, 64, 1, 2048), (9, 16, 1, 2048)])
+def test_lds_tuning_invariance(lds, dev, cell_vox, chunk, prefetch, box):
     """Every cell size, frame chunk and pair batch depth gives the same bits (the tile kernel's)."""
     import incohautocorrelate_amd as iac
 
@@ -59,6 +61,7 @@ def test_lds_tuning_invariance(lds, dev, cell_vox, chunk, prefetch):
     lds.set_option(iac.OPT_CELL_VOX, cell_vox)
     lds.set_option(iac.OPT_FRAME_CHUNK, chunk)
     lds.set_option(iac.OPT_PAIR_PREFETCH, prefetch)
+    lds.set_option(iac.OPT_BOX_CAP, box)
     got = mesh_np(gpu_batch(lds, dev, km, d["pix"], d["cnt"], d["frame_off"], shape))
     assert np.array_equal(got, tile)
 
