@@ -223,8 +223,9 @@ def autotune(ctx, iac, d_kmap, d_pix, d_cnt, data, shape, d_mesh, dev):
     d_off = torch.from_numpy(off[: nf + 1].view(np.int64)).to(dev)
     best = None
     ctx.enable_timing(True)
-    grid = [(box, cv, fc, 1) for box in (4096, 2048) for cv in (4, 5, 6, 7) for fc in (32, 64)]
-    grid.append((4096, 6, 32, 8))
+    grid = [(4096, cv, fc, 1) for cv in (6, 7) for fc in (32, 64)]
+    grid += [(2048, cv, fc, 1) for cv in (3, 4, 5) for fc in (32, 64)]
+    grid += [(1024, cv, fc, 1) for cv in (2, 3, 4) for fc in (32, 64)]
     for box, cv, fc, pf in grid:
         ctx.set_option(iac.OPT_BOX_CAP, box)
         ctx.set_option(iac.OPT_PAIR_PREFETCH, pf)
@@ -365,9 +366,9 @@ def uneven_leg(args, iac, W, dev):
     if not args.no_tune:
         best = None
         ctx.enable_timing(True)
-        for box in (4096, 2048):
-            for cv in (5, 6):
-                for fc in (32, 64):
+        for box, cv in ((4096, 6), (2048, 4), (2048, 3), (1024, 3), (1024, 2)):
+            for fc in (32, 64):
+                if True:
                     ctx.set_option(iac.OPT_BOX_CAP, box)
                     ctx.set_option(iac.OPT_CELL_VOX, cv)
                     ctx.set_option(iac.OPT_FRAME_CHUNK, fc)

@@ -37,9 +37,11 @@ namespace {
 
 constexpr int kBlock = 256;
 // LDS box capacity (uint64 voxels): 4096 (32 KiB, 1024 items per scan group, 3 blocks
-// per CU) or 2048 (16 KiB, 512 items, 6 blocks per CU); chosen per plan.
+// per CU), 2048 (16 KiB, 512 items, 5 blocks) or 1024 (8 KiB, 512 items, 8 blocks);
+// chosen per plan.
 constexpr int kBoxMax = 4096;
 constexpr int kBoxSmall = 2048;
+constexpr int kBoxTiny = 1024;
 template <int BOX>
 struct BoxCfg {
     static constexpr int group = BOX == kBoxMax ? 1024 : 512;  // items per scan group
@@ -882,6 +884,8 @@ void launch_pairs(int pf, dim3 grid, hipStream_t st, const PairRun &R, const Buf
                       std::sqrt((double)R.spec.P.cut_sq) * R.spec.P.vox < 4194304.0;
     if (b.box_cap == kBoxSmall)
         launch_pairs_box<K, O, kBoxSmall>(pf, fast, grid, st, R, b, target, nvox, dropped);
+    else if (b.box_cap == kBoxTiny)
+        launch_pairs_box<K, O, kBoxTiny>(pf, fast, grid, st, R, b, target, nvox, dropped);
     else
         launch_pairs_box<K, O, kBoxMax>(pf, fast, grid, st, R, b, target, nvox, dropped);
 }
@@ -940,7 +944,10 @@ int enqueue(iac_ctx *ctx, hipStream_t st, const PairRun &R, const Bufs &b, bool 
 }
 
 int cell_vox_of(iac_ctx *ctx, const PairSpec &S) { return ctx->cell_vox > 0 ? ctx->cell_vox : (S.kind == kKindAc3d ? 6 : 4); }
-int box_cap_of(iac_ctx *ctx) { return ctx->box_cap == kBoxSmall ? kBoxSmall : kBoxMax; }
+int box_cap_of(iac_ctx *ctx)
+{
+    return ctx->box_cap == kBoxSmall || ctx->box_cap == kBoxTiny ? ctx->box_cap : kBoxMax;
+}
 uint32_t fchunk_of(iac_ctx *ctx) { return ctx->frame_chunk > 0 ? ctx->frame_chunk : 64; }
 
 // byte sizes of the per-run buffers
