@@ -33,7 +33,7 @@ struct iac_ctx {
     int cell_vox = 0;
     uint32_t frame_chunk = 0;
     uint64_t sim_chunk_rows = 0;
-    int box_cap = 0;        // pair engine: LDS box voxels, 1024, 2048 or 4096 (0 = 4096)
+    int box_cap = 0;        // pair engine: LDS box voxels, 1024, 2048 or 4096 (0 = per kind)
     int pair_prefetch = 0;  // pair engine: pairs per load batch (1, 4 or 8; 0 = 1)
 };
 

@@ -943,12 +943,15 @@ int enqueue(iac_ctx *ctx, hipStream_t st, const PairRun &R, const Bufs &b, bool 
     return IAC_OK;
 }
 
-int cell_vox_of(iac_ctx *ctx, const PairSpec &S) { return ctx->cell_vox > 0 ? ctx->cell_vox : (S.kind == kKindAc3d ? 6 : 4); }
-int box_cap_of(iac_ctx *ctx)
+// defaults from the tuner on the 1024^2 sparse and lognormal workloads: AC_uw cells of 4
+// voxels in 2048-voxel boxes; dense C(q) cells of 4 voxels in 4096-voxel boxes
+int cell_vox_of(iac_ctx *ctx, const PairSpec &) { return ctx->cell_vox > 0 ? ctx->cell_vox : 4; }
+int box_cap_of(iac_ctx *ctx, const PairSpec &S)
 {
-    return ctx->box_cap == kBoxSmall || ctx->box_cap == kBoxTiny ? ctx->box_cap : kBoxMax;
+    if (ctx->box_cap == kBoxMax || ctx->box_cap == kBoxSmall || ctx->box_cap == kBoxTiny) return ctx->box_cap;
+    return S.kind == kKindAc3d ? kBoxSmall : kBoxMax;
 }
-uint32_t fchunk_of(iac_ctx *ctx) { return ctx->frame_chunk > 0 ? ctx->frame_chunk : 64; }
+uint32_t fchunk_of(iac_ctx *ctx) { return ctx->frame_chunk > 0 ? ctx->frame_chunk : 32; }
 
 // byte sizes of the per-run buffers
 struct Sizes {
@@ -982,7 +985,7 @@ int pairs_run(iac_ctx *ctx, const PairRun &R)
     hipStream_t st = ctx->stream;
     if (R.n_total >= (1ull << 32)) return fail(IAC_ERR_LIMIT, "pair engine: at most 2^32 - 1 records per call");
     iac_pair_plan *pl = nullptr;
-    IAC_TRY(get_plan(ctx, st, R, cell_vox_of(ctx, R.spec), box_cap_of(ctx), &pl));
+    IAC_TRY(get_plan(ctx, st, R, cell_vox_of(ctx, R.spec), box_cap_of(ctx, R.spec), &pl));
     std::vector<uint64_t> off(R.nframes + 1);
     IAC_HIP(hipMemcpyAsync(off.data(), R.d_frame_off, off.size() * 8, hipMemcpyDeviceToHost, st));
     IAC_HIP(hipStreamSynchronize(st));
@@ -1029,7 +1032,7 @@ int pairs_prepare(iac_ctx *ctx, const PairRun &R0, PairBatch **out)
     hipStream_t st = ctx->stream;
     if (R0.n_total >= (1ull << 32)) return fail(IAC_ERR_LIMIT, "pair engine: at most 2^32 - 1 records per call");
     iac_pair_plan *pl = nullptr;
-    IAC_TRY(get_plan(ctx, st, R0, cell_vox_of(ctx, R0.spec), box_cap_of(ctx), &pl));
+    IAC_TRY(get_plan(ctx, st, R0, cell_vox_of(ctx, R0.spec), box_cap_of(ctx, R0.spec), &pl));
     std::vector<uint64_t> off(R0.nframes + 1);
     IAC_HIP(hipMemcpyAsync(off.data(), R0.d_frame_off, off.size() * 8, hipMemcpyDeviceToHost, st));
     IAC_HIP(hipStreamSynchronize(st));

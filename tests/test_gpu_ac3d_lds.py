@@ -48,7 +48,7 @@ def test_lds_cfg2_subset(lds, dev):
 @pytest.mark.parametrize("cell_vox,chunk,prefetch,box", [(2, 1, 8, 4096), (3, 7, 4, 4096), (4, 128, 1, 4096),
                                                          (8, 64, 8, 4096), (16, 300, 4, 4096), (6, 64, 1, 4096),
                                                          (5, 32, 1, 2048), (6, 64, 1, 2048), (9, 16, 1, 2048),
-                                                         (3, 32, 1, 1024), (4, 8, 1, 1024)])
+                                                         (3, 32, 1, 1024), (4, 8, 1, 1024), (4, 32, 1, 2048)])
 def test_lds_tuning_invariance(lds, dev, cell_vox, chunk, prefetch, box):
     """Every cell size, frame chunk and pair batch depth gives the same bits (the tile kernel's)."""
     import incohautocorrelate_amd as iac
