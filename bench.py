@@ -35,6 +35,10 @@ sys.path.insert(0, str(ROOT))
 
 HBM_PEAK_GBS = 8000.0  # MI355X HBM3E, public spec
 BYTES_PER_UPDATE = 16  # one 8 B bin read + one 8 B bin write per pair-update
+# Measured HBM bytes per pair-kernel launch on this workload, keyed by the engine settings
+# (box, cell_vox, frame_chunk, prefetch): FETCH_SIZE x2.0 + WRITE_SIZE x1.0 KiB from two
+# rocprofv3 --pmc passes, scaled by a 2 GiB copy kernel (profiles/pairs_lds_sparse1024_10k.md).
+MEASURED_TRAFFIC = {(2048, 4, 32, 1): (2 * 1494022 + 2878269) * 1024}
 
 
 def log(*a):
@@ -167,6 +171,11 @@ def main():
     uneven = None if args.no_uneven else uneven_leg(args, iac, W, dev)
     sim = None if args.no_sim else sim_leg(args, iac, W, dev)
 
+    traffic = None
+    if tuned and args.kernel != 1:
+        key = (tuned.get("box") or 2048, tuned.get("cell_vox") or 4, tuned.get("frame_chunk") or 32,
+               tuned.get("prefetch") or 1)
+        traffic = MEASURED_TRAFFIC.get(key)
     out = {
         "metric": "pixel-pair updates/sec into q-mesh (sparse 1024^2, 257^3 uint64, bit-exact)",
         "value": value,
@@ -197,7 +206,7 @@ def main():
             "unit": "GB/s",
             "frac": achieved / HBM_PEAK_GBS,
             "traffic": None,
-            "kernel": "ac3d_tiles_kernel<true>" if args.kernel == 1 else "pairs_lds_kernel",
+
             "kernel_avg_ms": k_avg_ms,
             "bytes_per_update": BYTES_PER_UPDATE,
             "frac_of_measured_copy": achieved / probes["hbm_copy_GBps"] if probes else None,
