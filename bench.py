@@ -205,8 +205,9 @@ def main():
             "peak": HBM_PEAK_GBS,
             "unit": "GB/s",
             "frac": achieved / HBM_PEAK_GBS,
-            "traffic": None,
-
+            "traffic": traffic,
+            "traffic_unit": "bytes per launch (rocprofv3 FETCH_SIZE + WRITE_SIZE, copy-scaled)",
+            "kernel": "pairs_kernel (LDS cell boxes)" if args.kernel != 1 else "ac3d_tiles_kernel<true>",
             "kernel_avg_ms": k_avg_ms,
             "bytes_per_update": BYTES_PER_UPDATE,
             "frac_of_measured_copy": achieved / probes["hbm_copy_GBps"] if probes else None,
