@@ -457,8 +457,9 @@ def sim_leg(args, iac, W, dev):
         return float(np.linalg.norm(a - b) / np.linalg.norm(b))
 
     res = {}
-    flops = 3 * 2.0 * (2 * nf) * npix * (2 * nem)  # three real GEMM products per step
-    for mode, name, dt in ((iac.SIM_BF16X3, "bf16x3", "bf16"), (iac.SIM_FP16X3, "fp16x3", "fp16")):
+    for mode, name, dt, nterms in ((iac.SIM_BF16X3, "bf16x3", "bf16", 3), (iac.SIM_FP16X3, "fp16x3", "fp16", 3),
+                                   (iac.SIM_FP16, "fp16", "fp16", 1)):
+        flops = nterms * 2.0 * (2 * nf) * npix * (2 * nem)  # real GEMM products per step
         torch.cuda.synchronize()
         t = time.perf_counter()
         plan = ctx.sim_plan(d_pm, d_em, wl, mode)
@@ -494,7 +495,7 @@ def sim_leg(args, iac, W, dev):
             "err_vs_fp64": errs, "photons_per_frame": photons,
             "roofline": {"bound": "mfma", "achieved": ach, "peak": MFMA_PEAK_TFLOPS[dt], "unit": "TFLOP/s",
                          "frac": ach / MFMA_PEAK_TFLOPS[dt], "traffic": None,
-                         "kernel": f"rocBLAS gemm_ex {dt} x3 + epilogue",
+                         "kernel": f"rocBLAS gemm_ex {dt} x{nterms} + epilogue",
                          "flops_per_step": flops},
         }
         log(f"[sim] {name}: {ms:.2f} ms/step ({nf / (ms * 1e-3):.1f} frames/s), GEMM region {gemm_ms:.2f} ms "
@@ -524,17 +525,24 @@ def sim_leg(args, iac, W, dev):
     direct_fps = nd / (dms * 1e-3)
     log(f"[sim] fp32 direct: {dms / nd:.2f} ms/frame ({direct_fps:.1f} frames/s), err amp {e32:.2e}")
     ctx.close()
-    best = res["bf16x3"]["frames_per_s"]
+    # the headline mode: the fastest whose intensity error against FP64 is within 1e-3
+    ok = [k for k, v in res.items() if v["err_vs_fp64"]["intensity"] <= 1e-3]
+    best_mode = max(ok, key=lambda k: res[k]["frames_per_s"])
+    best = res[best_mode]["frames_per_s"]
     return {
         "workload": f"{n}x{n} detector (75 um, 0.5 m), {nem} emitters uniform in a 500 A cube, lambda {wl} A, "
                     f"{nf} frames of uniform phases, |psi| output, rescale to 0.01 photons/pixel + Poisson",
         "unit": "frames/s",
         "value": best,
+        "mode": best_mode,
+        "error_bar": "normwise relative |psi|^2 error vs FP64 <= 1e-3",
+        "algorithmic_flop_per_frame": 8.0 * npix * nem,
         "modes": res,
         "fp32_direct": {"frames_per_s": direct_fps, "ms_per_frame": dms / nd, "frames_timed": nd,
                         "err_vs_fp64": {"amplitude": e32}},
         "speedup_bf16x3_vs_fp32_direct": best / direct_fps,
         "speedup_fp16x3_vs_fp32_direct": res["fp16x3"]["frames_per_s"] / direct_fps,
+        "speedup_best_vs_fp32_direct": best / direct_fps,
         "err_subset": f"pixels [{p0}, {p0 + npx}) (8 central rows) x frames [0, {nfe}), normwise relative",
     }
 
