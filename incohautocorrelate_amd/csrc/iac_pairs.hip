@@ -93,6 +93,23 @@ __device__ __forceinline__ int cell_of(const CellGeom &g, float ku, float kv)
 
 __device__ __forceinline__ float comp(const float4 &r, int a) { return a == 0 ? r.x : (a == 1 ? r.y : r.z); }
 
+// q = u / d, r = u % d for u < 2^24, d >= 1, from the float reciprocal inv = 1.0f / d:
+// the estimate is within one of the quotient, one correction step makes it exact.
+__device__ __forceinline__ void divmod24(uint32_t u, uint32_t d, float inv, uint32_t &q, uint32_t &r)
+{
+    uint32_t t = (uint32_t)((float)u * inv);
+    int32_t m = (int32_t)(u - __umul24(t, d));
+    if (m < 0) {
+        t--;
+        m += (int32_t)d;
+    } else if (m >= (int32_t)d) {
+        t++;
+        m -= (int32_t)d;
+    }
+    q = t;
+    r = (uint32_t)m;
+}
+
 // (int)roundf(x) (half away from zero) for |x| < 2^23: x + copysign(0.49999997f, x)
 // rounded to nearest, truncated.  Equal to roundf for every float in that range
 // (checked exhaustively over all 2.5e9 of them); 3 instructions instead of 7.
@@ -374,7 +391,7 @@ __global__ void __launch_bounds__(kBlock)
     __shared__ unsigned long long box[BOX];
     __shared__ uint4 item[kGroup];        // {a base, na, b base, nb}
     __shared__ uint32_t pre[kGroup + 1];  // exclusive prefix of pair counts
-    __shared__ uint32_t wsum[kBlock];
+    __shared__ uint32_t wtot[kBlock / 64];
 
     const Task T = tasks[blockIdx.x % ntasks];
     const Chunk CH = chunks[blockIdx.x / ntasks];
@@ -390,22 +407,28 @@ __global__ void __launch_bounds__(kBlock)
     for (int v = threadIdx.x; v < vbox; v += kBlock) box[v] = 0ull;
     __syncthreads();
 
+    // items (frame, cell) of the chunk: fewer than 2^24 (at most 4096 frames x 4096 cells),
+    // so quotients come from a float reciprocal with one correction step
     const uint32_t cells = (uint32_t)T.nu * (uint32_t)T.nv;
-    const uint64_t nitems = (uint64_t)(CH.f1 - CH.f0) * cells;
-    for (uint64_t ub = 0; ub < nitems; ub += kGroup) {
-        // phase 1: item descriptors and pair counts (4 contiguous items per thread)
+    const uint32_t nitems = (CH.f1 - CH.f0) * cells;
+    const float inv_cells = 1.0f / (float)cells, inv_nu = 1.0f / (float)T.nu;
+    const uint32_t lane = threadIdx.x & 63u, wave = threadIdx.x >> 6;
+    for (uint32_t ub = 0; ub < nitems; ub += kGroup) {
+        // phase 1: item descriptors and pair counts (contiguous items per thread)
         uint32_t mycnt[kItemsPerThread];
         uint32_t tsum = 0;
 #pragma unroll
         for (int k = 0; k < kItemsPerThread; k++) {
             const uint32_t li = threadIdx.x * kItemsPerThread + k;
-            const uint64_t u = ub + li;
+            const uint32_t u = ub + li;
             uint4 d = make_uint4(0, 0, 0, 0);
             uint32_t c = 0;
             if (u < nitems) {
-                const uint32_t f = CH.f0 + (uint32_t)(u / cells);
-                const uint32_t rem = (uint32_t)(u % cells);
-                const int iv = T.iv0 + (int)(rem / T.nu), iu = T.iu0 + (int)(rem % T.nu);
+                uint32_t fr, rem, row, col;
+                divmod24(u, cells, inv_cells, fr, rem);
+                divmod24(rem, (uint32_t)T.nu, inv_nu, row, col);
+                const uint32_t f = CH.f0 + fr;
+                const int iv = T.iv0 + (int)row, iu = T.iu0 + (int)col;
                 const int ci = iv * g.ncu + iu;
                 const int cj = (iv - T.dv) * g.ncu + (iu - T.du);
                 const uint32_t *csf = cs + (size_t)f * (ncell + 1);
@@ -419,16 +442,19 @@ __global__ void __launch_bounds__(kBlock)
             mycnt[k] = c;
             tsum += c;
         }
-        wsum[threadIdx.x] = tsum;
-        __syncthreads();
-        for (int o = 1; o < kBlock; o <<= 1) {
-            const uint32_t v = threadIdx.x >= (unsigned)o ? wsum[threadIdx.x - o] : 0u;
-            __syncthreads();
-            wsum[threadIdx.x] += v;
-            __syncthreads();
+        // block exclusive scan of the per-thread sums: within each wave by shuffles,
+        // then across the four waves through LDS (two barriers)
+        uint32_t x = tsum;
+#pragma unroll
+        for (int o = 1; o < 64; o <<= 1) {
+            const uint32_t y = __shfl_up(x, o, 64);
+            if (lane >= (uint32_t)o) x += y;
         }
+        if (lane == 63) wtot[wave] = x;
+        __syncthreads();
         {
-            uint32_t run = threadIdx.x ? wsum[threadIdx.x - 1] : 0u;
+            uint32_t run = x - tsum;
+            for (uint32_t w = 0; w < wave; w++) run += wtot[w];
 #pragma unroll
             for (int k = 0; k < kItemsPerThread; k++) {
                 pre[threadIdx.x * kItemsPerThread + k] = run;
@@ -823,7 +849,7 @@ std::vector<Chunk> make_chunks(const std::vector<uint64_t> &off, const uint32_t 
         total += n * n;
     }
     const double budget = std::max(1.0, total / std::max(1u, nframes) * fchunk);
-    const uint32_t fmax = 4 * fchunk;
+    const uint32_t fmax = std::min<uint32_t>(4 * fchunk, 4096);  // items per chunk < 2^24 (pairs_kernel)
     std::vector<Chunk> chunks;
     for (uint32_t gi = 0; gi < ngroups; gi++) {
         uint32_t f0 = group_off[gi];
