@@ -52,7 +52,8 @@ constexpr int kMaxCells = 4096;                    // NC_u * NC_v
 struct CellGeom {
     int au, av;    // k axes spanned by the cell grid
     float u0, v0;  // grid origin
-    float inv_w;   // 1 / cell width (k units)
+    float inv_wu;  // 1 / cell width along u (k units)
+    float inv_wv;  // 1 / cell width along v
     int ncu, ncv;  // cells per axis
 };
 
@@ -84,8 +85,8 @@ inline float unkey(uint32_t k)
 
 __device__ __forceinline__ int cell_of(const CellGeom &g, float ku, float kv)
 {
-    int cu = (int)floorf((ku - g.u0) * g.inv_w);
-    int cv = (int)floorf((kv - g.v0) * g.inv_w);
+    int cu = (int)floorf((ku - g.u0) * g.inv_wu);
+    int cv = (int)floorf((kv - g.v0) * g.inv_wv);
     cu = min(max(cu, 0), g.ncu - 1);
     cv = min(max(cv, 0), g.ncv - 1);
     return cv * g.ncu + cu;
@@ -791,14 +792,20 @@ int get_plan(iac_ctx *ctx, hipStream_t st, const PairRun &R, int cell_vox, int b
     std::sort(order, order + 3, [&](int a, int b) { return ext3[a] > ext3[b]; });
     g.au = std::min(order[0], order[1]);
     g.av = std::max(order[0], order[1]);
-    double w = (double)cell_vox / (double)S.P.vox;
-    auto ncells = [&](int a) { return std::max(1, (int)std::ceil(ext3[a] / w * (1 + 1e-6))); };
-    while ((long)ncells(g.au) * ncells(g.av) > kMaxCells) w *= 1.25;
-    g.inv_w = (float)(1.0 / w);
+    // cell_vox = u side + 1000 * v side (voxels); cells may be longer along u
+    double wu = (double)(cell_vox % 1000) / (double)S.P.vox;
+    double wv = (double)(cell_vox / 1000 ? cell_vox / 1000 : cell_vox % 1000) / (double)S.P.vox;
+    auto ncells = [&](int a, double w) { return std::max(1, (int)std::ceil(ext3[a] / w * (1 + 1e-6))); };
+    while ((long)ncells(g.au, wu) * ncells(g.av, wv) > kMaxCells) {
+        wu *= 1.25;
+        wv *= 1.25;
+    }
+    g.inv_wu = (float)(1.0 / wu);
+    g.inv_wv = (float)(1.0 / wv);
     g.u0 = kmin[g.au];
     g.v0 = kmin[g.av];
-    g.ncu = ncells(g.au);
-    g.ncv = ncells(g.av);
+    g.ncu = ncells(g.au, wu);
+    g.ncv = ncells(g.av, wv);
     const int nc = g.ncu * g.ncv;
     hipLaunchKernelGGL(init_cell_ext_kernel, dim3((nc + 255) / 256), dim3(256), 0, st, d_ext + 6, nc);
     hipLaunchKernelGGL(cell_ext_kernel, dim3(eb), dim3(256), 0, st, R.d_kmap, R.npix, g, d_ext + 6);
@@ -971,7 +978,11 @@ int enqueue(iac_ctx *ctx, hipStream_t st, const PairRun &R, const Bufs &b, bool 
 
 // defaults from the tuner on the 1024^2 sparse and lognormal workloads: AC_uw cells of 4
 // voxels in 2048-voxel boxes; dense C(q) cells of 4 voxels in 4096-voxel boxes
-int cell_vox_of(iac_ctx *ctx, const PairSpec &) { return ctx->cell_vox > 0 ? ctx->cell_vox : 4; }
+int cell_vox_of(iac_ctx *ctx, const PairSpec &)
+{
+    const int u = ctx->cell_vox > 0 ? ctx->cell_vox : 4;
+    return u + 1000 * (ctx->cell_vox_v > 0 ? ctx->cell_vox_v : u);
+}
 int box_cap_of(iac_ctx *ctx, const PairSpec &S)
 {
     if (ctx->box_cap == kBoxMax || ctx->box_cap == kBoxSmall || ctx->box_cap == kBoxTiny) return ctx->box_cap;

@@ -21,6 +21,7 @@ def lds(ctx):
     ctx.set_option(iac.OPT_FRAME_CHUNK, 0)
     ctx.set_option(iac.OPT_PAIR_PREFETCH, 0)
     ctx.set_option(iac.OPT_BOX_CAP, 0)
+    ctx.set_option(iac.OPT_CELL_VOX_V, 0)
 
 
 def test_lds_cfg1_full(lds, dev):
@@ -48,7 +49,8 @@ def test_lds_cfg2_subset(lds, dev):
 @pytest.mark.parametrize("cell_vox,chunk,prefetch,box", [(2, 1, 8, 4096), (3, 7, 4, 4096), (4, 128, 1, 4096),
                                                          (8, 64, 8, 4096), (16, 300, 4, 4096), (6, 64, 1, 4096),
                                                          (5, 32, 1, 2048), (6, 64, 1, 2048), (9, 16, 1, 2048),
-                                                         (3, 32, 1, 1024), (4, 8, 1, 1024), (4, 32, 1, 2048)])
+                                                         (3, 32, 1, 1024), (4, 8, 1, 1024), (4, 32, 1, 2048),
+                                                         (3008, 32, 1, 2048), (12004, 16, 1, 4096)])
 def test_lds_tuning_invariance(lds, dev, cell_vox, chunk, prefetch, box):
     """Every cell size, frame chunk and pair batch depth gives the same bits (the tile kernel's)."""
     import incohautocorrelate_amd as iac
@@ -59,7 +61,8 @@ def test_lds_tuning_invariance(lds, dev, cell_vox, chunk, prefetch, box):
     lds.set_option(iac.OPT_AC3D_KERNEL, 1)
     tile = mesh_np(gpu_batch(lds, dev, km, d["pix"], d["cnt"], d["frame_off"], shape))
     lds.set_option(iac.OPT_AC3D_KERNEL, 2)
-    lds.set_option(iac.OPT_CELL_VOX, cell_vox)
+    lds.set_option(iac.OPT_CELL_VOX, cell_vox % 1000)  # u side; v side in the thousands (0 = same)
+    lds.set_option(iac.OPT_CELL_VOX_V, cell_vox // 1000)
     lds.set_option(iac.OPT_FRAME_CHUNK, chunk)
     lds.set_option(iac.OPT_PAIR_PREFETCH, prefetch)
     lds.set_option(iac.OPT_BOX_CAP, box)
