@@ -63,6 +63,7 @@ def parse():
     ap.add_argument("--prefetch", type=int, default=0, help="fix the pair batch depth 1/4/8 (with the two above)")
     ap.add_argument("--box", type=int, default=0, help="fix the LDS box capacity 1024/2048/4096 (with the above)")
     ap.add_argument("--cell-vox-v", type=int, default=0, help="fix the cell side along the second k axis")
+    ap.add_argument("--region", type=int, default=0, help="fix the task region side in cells (0: whole grid)")
     ap.add_argument("--no-sim", action="store_true", help="skip the simulator leg")
     ap.add_argument("--no-uneven", action="store_true", help="skip the uneven-batch leg")
     ap.add_argument("--no-probes", action="store_true", help="skip the copy / atomic ceiling probes")
@@ -120,14 +121,15 @@ def main():
 
     ctx.set_option(iac.OPT_AC3D_KERNEL, args.kernel)
     tuned = None
-    if args.cell_vox or args.frame_chunk or args.prefetch or args.box or args.cell_vox_v:
+    if args.cell_vox or args.frame_chunk or args.prefetch or args.box or args.cell_vox_v or args.region:
+        ctx.set_option(iac.OPT_TASK_REGION, args.region)
         ctx.set_option(iac.OPT_CELL_VOX, args.cell_vox)
         ctx.set_option(iac.OPT_CELL_VOX_V, args.cell_vox_v)
         ctx.set_option(iac.OPT_FRAME_CHUNK, args.frame_chunk)
         ctx.set_option(iac.OPT_PAIR_PREFETCH, args.prefetch)
         ctx.set_option(iac.OPT_BOX_CAP, args.box)
         tuned = {"cell_vox": args.cell_vox, "cell_vox_v": args.cell_vox_v, "frame_chunk": args.frame_chunk,
-                 "prefetch": args.prefetch, "box": args.box, "fixed": True}
+                 "prefetch": args.prefetch, "box": args.box, "region": args.region, "fixed": True}
     elif args.kernel != 1 and not args.no_tune:
         tuned = autotune(ctx, iac, d_kmap, d_pix, d_cnt, data, shape, d_mesh, dev)
 
@@ -177,7 +179,7 @@ def main():
     if tuned and args.kernel != 1:
         key = (tuned.get("box") or 2048, tuned.get("cell_vox") or 4, tuned.get("frame_chunk") or 32,
                tuned.get("prefetch") or 1)
-        if (tuned.get("cell_vox_v") or key[1]) != key[1]:
+        if (tuned.get("cell_vox_v") or key[1]) != key[1] or tuned.get("region"):
             key = None
         traffic = MEASURED_TRAFFIC.get(key) if key else None
     out = {
@@ -237,15 +239,16 @@ def autotune(ctx, iac, d_kmap, d_pix, d_cnt, data, shape, d_mesh, dev):
     d_off = torch.from_numpy(off[: nf + 1].view(np.int64)).to(dev)
     best = None
     ctx.enable_timing(True)
-    # (box, cell side along u, along v, frame chunk)
-    grid = [(2048, u, v, 32) for (u, v) in ((4, 4), (5, 4), (6, 4), (8, 4), (6, 3), (8, 3), (10, 3), (12, 2))]
-    grid += [(4096, u, v, 32) for (u, v) in ((6, 6), (8, 5), (10, 4), (12, 4), (16, 3))]
-    grid += [(2048, 4, 4, 64)]
-    for box, cu, cv, fc in grid:
+    # (box, cell side along u, along v, task region in cells, frame chunk)
+    grid = [(2048, 4, 4, 0, 32), (2048, 4, 4, 16, 32), (2048, 5, 5, 12, 32), (2048, 6, 6, 8, 32),
+            (2048, 6, 6, 12, 32), (2048, 8, 8, 6, 32), (2048, 8, 8, 8, 32), (4096, 8, 8, 8, 32),
+            (4096, 8, 8, 12, 32), (4096, 10, 10, 6, 32), (4096, 12, 12, 4, 32), (2048, 6, 6, 8, 64)]
+    for box, cu, cv, reg, fc in grid:
         ctx.set_option(iac.OPT_BOX_CAP, box)
         ctx.set_option(iac.OPT_PAIR_PREFETCH, 1)
         ctx.set_option(iac.OPT_CELL_VOX, cu)
         ctx.set_option(iac.OPT_CELL_VOX_V, cv)
+        ctx.set_option(iac.OPT_TASK_REGION, reg)
         ctx.set_option(iac.OPT_FRAME_CHUNK, fc)
         ctx.ac3d_sparse_batch(d_kmap, d_pix[:n], d_cnt[:n], d_off, n, shape, d_mesh)  # plan
         ctx.kernel_time(reset=True)
@@ -253,17 +256,19 @@ def autotune(ctx, iac, d_kmap, d_pix, d_cnt, data, shape, d_mesh, dev):
             ctx.ac3d_sparse_batch(d_kmap, d_pix[:n], d_cnt[:n], d_off, n, shape, d_mesh)
         ms, k = ctx.kernel_time(reset=True)
         t = ms / max(k, 1)
-        log(f"[tune] box={box} cell={cu}x{cv} frame_chunk={fc}: {t:.3f} ms / {nf} frames")
+        log(f"[tune] box={box} cell={cu}x{cv} region={reg} frame_chunk={fc}: {t:.3f} ms / {nf} frames")
         if best is None or t < best[0]:
-            best = (t, cu, fc, 1, box, cv)
+            best = (t, cu, fc, 1, box, cv, reg)
     ctx.enable_timing(False)
     ctx.set_option(iac.OPT_CELL_VOX, best[1])
     ctx.set_option(iac.OPT_CELL_VOX_V, best[5])
     ctx.set_option(iac.OPT_FRAME_CHUNK, best[2])
     ctx.set_option(iac.OPT_PAIR_PREFETCH, best[3])
     ctx.set_option(iac.OPT_BOX_CAP, best[4])
-    log(f"[tune] chose box={best[4]} cell={best[1]}x{best[5]} frame_chunk={best[2]}")
-    return {"cell_vox": best[1], "cell_vox_v": best[5], "frame_chunk": best[2], "prefetch": best[3], "box": best[4]}
+    ctx.set_option(iac.OPT_TASK_REGION, best[6])
+    log(f"[tune] chose box={best[4]} cell={best[1]}x{best[5]} region={best[6]} frame_chunk={best[2]}")
+    return {"cell_vox": best[1], "cell_vox_v": best[5], "frame_chunk": best[2], "prefetch": best[3], "box": best[4],
+            "region": best[6]}
 
 
 def check_subset(iac, W, kmap, data, shape, dev):
@@ -383,12 +388,14 @@ def uneven_leg(args, iac, W, dev):
     if not args.no_tune:
         best = None
         ctx.enable_timing(True)
-        for box, cu, cv in ((2048, 4, 4), (2048, 6, 4), (2048, 8, 4), (2048, 8, 3), (4096, 10, 4), (4096, 6, 6)):
+        for box, cu, cv, reg in ((2048, 4, 4, 0), (2048, 6, 6, 8), (2048, 8, 8, 6), (4096, 8, 8, 8),
+                                 (4096, 10, 10, 6)):
             for fc in (32,):
                 if True:
                     ctx.set_option(iac.OPT_BOX_CAP, box)
                     ctx.set_option(iac.OPT_CELL_VOX, cu)
                     ctx.set_option(iac.OPT_CELL_VOX_V, cv)
+                    ctx.set_option(iac.OPT_TASK_REGION, reg)
                     ctx.set_option(iac.OPT_FRAME_CHUNK, fc)
                     b = ctx.ac3d_batch(d_kmap, d_pix, d_cnt, d_off, n_total, shape)
                     b.launch(d_mesh, True)
@@ -396,15 +403,17 @@ def uneven_leg(args, iac, W, dev):
                     b.launch(d_mesh, True)
                     ms, k = ctx.kernel_time(reset=True)
                     b.destroy()
-                    log(f"[uneven tune] box={box} cell={cu}x{cv} frame_chunk={fc}: {ms:.2f} ms")
+                    log(f"[uneven tune] box={box} cell={cu}x{cv} region={reg} frame_chunk={fc}: {ms:.2f} ms")
                     if best is None or ms < best[0]:
-                        best = (ms, cu, fc, box, cv)
+                        best = (ms, cu, fc, box, cv, reg)
         ctx.enable_timing(False)
         ctx.set_option(iac.OPT_CELL_VOX, best[1])
         ctx.set_option(iac.OPT_CELL_VOX_V, best[4])
         ctx.set_option(iac.OPT_FRAME_CHUNK, best[2])
         ctx.set_option(iac.OPT_BOX_CAP, best[3])
-        tuned = {"cell_vox": best[1], "cell_vox_v": best[4], "frame_chunk": best[2], "box": best[3]}
+        ctx.set_option(iac.OPT_TASK_REGION, best[5])
+        tuned = {"cell_vox": best[1], "cell_vox_v": best[4], "frame_chunk": best[2], "box": best[3],
+                 "region": best[5]}
     b = ctx.ac3d_batch(d_kmap, d_pix, d_cnt, d_off, n_total, shape)
     b.capture(d_mesh, True)
     b.launch(d_mesh, True)

@@ -669,6 +669,7 @@ struct iac_pair_plan {
     uint32_t npix = 0;
     int cell_vox = 0;
     int box_cap = 0;  // LDS box capacity the tasks were sized for
+    int region = 0;   // cells per task region side (0: whole grid)
     iac::PairSpec spec{};
     iac::CellGeom g{};
     std::vector<uint32_t> cell_ext;  // 6 keys per cell
@@ -685,6 +686,10 @@ bool same_spec(const PairSpec &a, const PairSpec &b)
     return std::memcmp(&a, &b, sizeof(PairSpec)) == 0;
 }
 
+// Tasks: one per cell offset delta and region of first cells i.  A region of
+// `region` x `region` cells (0: the whole grid) narrows the q range of its pairs
+// -- q along the beam grows with the distance from the centre -- so larger cells
+// still fit their voxel box.
 void build_tasks(iac_pair_plan &pl, bool ordered)
 {
     const CellGeom &g = pl.g;
@@ -704,64 +709,71 @@ void build_tasks(iac_pair_plan &pl, bool ordered)
     const double maxq = std::sqrt((double)S.P.cut_sq);
     const int cen[3] = {S.P.c0, S.P.c1, S.P.c2};
     const int dim[3] = {S.P.sab, S.P.sab, S.P.sc};
+    const int reg = pl.region > 0 ? pl.region : std::max(g.ncu, g.ncv);
     pl.tasks.clear();
     for (int dv = ordered ? -(g.ncv - 1) : 0; dv < g.ncv; dv++) {
         for (int du = -(g.ncu - 1); du < g.ncu; du++) {
             if (!ordered && dv == 0 && du < 0) continue;  // positive half plane
-            Task t{};
-            t.du = du;
-            t.dv = dv;
-            t.iu0 = std::max(0, du);
-            t.iv0 = std::max(0, dv);
-            t.nu = std::min(g.ncu, g.ncu + du) - t.iu0;
-            t.nv = std::min(g.ncv, g.ncv + dv) - t.iv0;
-            if (t.nu <= 0 || t.nv <= 0) continue;
-            double qlo[3] = {1e30, 1e30, 1e30}, qhi[3] = {-1e30, -1e30, -1e30};
-            bool any = false;
-            for (int iv = t.iv0; iv < t.iv0 + t.nv; iv++)
-                for (int iu = t.iu0; iu < t.iu0 + t.nu; iu++) {
-                    const int ci = iv * g.ncu + iu, cj = (iv - dv) * g.ncu + (iu - du);
-                    if (!has[ci] || !has[cj]) continue;
-                    any = true;
+            const int iu0 = std::max(0, du), iv0 = std::max(0, dv);
+            const int nu = std::min(g.ncu, g.ncu + du) - iu0, nv = std::min(g.ncv, g.ncv + dv) - iv0;
+            if (nu <= 0 || nv <= 0) continue;
+            for (int rv = iv0; rv < iv0 + nv; rv += reg) {
+                for (int ru = iu0; ru < iu0 + nu; ru += reg) {
+                    Task t{};
+                    t.du = du;
+                    t.dv = dv;
+                    t.iu0 = ru;
+                    t.iv0 = rv;
+                    t.nu = std::min(reg, iu0 + nu - ru);
+                    t.nv = std::min(reg, iv0 + nv - rv);
+                    double qlo[3] = {1e30, 1e30, 1e30}, qhi[3] = {-1e30, -1e30, -1e30};
+                    bool any = false;
+                    for (int iv = t.iv0; iv < t.iv0 + t.nv; iv++)
+                        for (int iu = t.iu0; iu < t.iu0 + t.nu; iu++) {
+                            const int ci = iv * g.ncu + iu, cj = (iv - dv) * g.ncu + (iu - du);
+                            if (!has[ci] || !has[cj]) continue;
+                            any = true;
+                            for (int d = 0; d < 3; d++) {
+                                qlo[d] = std::min(qlo[d], kmn[3 * ci + d] - kmx[3 * cj + d]);
+                                qhi[d] = std::max(qhi[d], kmx[3 * ci + d] - kmn[3 * cj + d]);
+                            }
+                        }
+                    if (!any) continue;
+                    double dist2 = 0;  // all pairs of the task are cut if the q box lies beyond Max_Q
                     for (int d = 0; d < 3; d++) {
-                        qlo[d] = std::min(qlo[d], kmn[3 * ci + d] - kmx[3 * cj + d]);
-                        qhi[d] = std::max(qhi[d], kmx[3 * ci + d] - kmn[3 * cj + d]);
+                        const double m = qlo[d] > 0 ? qlo[d] : (qhi[d] < 0 ? -qhi[d] : 0.0);
+                        dist2 += m * m;
                     }
+                    if (std::sqrt(dist2) > maxq * (1.0 + 1e-5) + 1e-7) continue;
+                    double lo[3], hi[3];
+                    voxel_box(S, qlo, qhi, lo, hi);
+                    int ilo[3], ihi[3];
+                    for (int d = 0; d < 3; d++) {
+                        ilo[d] = (int)std::max(0.0, lo[d] + cen[d]);
+                        ihi[d] = (int)std::min((double)dim[d] - 1, hi[d] + cen[d]);
+                    }
+                    const long vol = (long)(ihi[0] - ilo[0] + 1) * (ihi[1] - ilo[1] + 1) * (ihi[2] - ilo[2] + 1);
+                    if (ihi[0] >= ilo[0] && ihi[1] >= ilo[1] && ihi[2] >= ilo[2] && vol <= pl.box_cap) {
+                        t.bx0 = ilo[0];
+                        t.by0 = ilo[1];
+                        t.bz0 = ilo[2];
+                        t.bx = ihi[0] - ilo[0] + 1;
+                        t.by = ihi[1] - ilo[1] + 1;
+                        t.bz = ihi[2] - ilo[2] + 1;
+                    }
+                    pl.tasks.push_back(t);
                 }
-            if (!any) continue;
-            double dist2 = 0;  // all pairs of the task are cut if the q box lies beyond Max_Q
-            for (int d = 0; d < 3; d++) {
-                const double m = qlo[d] > 0 ? qlo[d] : (qhi[d] < 0 ? -qhi[d] : 0.0);
-                dist2 += m * m;
             }
-            if (std::sqrt(dist2) > maxq * (1.0 + 1e-5) + 1e-7) continue;
-            double lo[3], hi[3];
-            voxel_box(S, qlo, qhi, lo, hi);
-            int ilo[3], ihi[3];
-            for (int d = 0; d < 3; d++) {
-                ilo[d] = (int)std::max(0.0, lo[d] + cen[d]);
-                ihi[d] = (int)std::min((double)dim[d] - 1, hi[d] + cen[d]);
-            }
-            const long vol = (long)(ihi[0] - ilo[0] + 1) * (ihi[1] - ilo[1] + 1) * (ihi[2] - ilo[2] + 1);
-            if (ihi[0] >= ilo[0] && ihi[1] >= ilo[1] && ihi[2] >= ilo[2] && vol <= pl.box_cap) {
-                t.bx0 = ilo[0];
-                t.by0 = ilo[1];
-                t.bz0 = ilo[2];
-                t.bx = ihi[0] - ilo[0] + 1;
-                t.by = ihi[1] - ilo[1] + 1;
-                t.bz = ihi[2] - ilo[2] + 1;
-            }
-            pl.tasks.push_back(t);
         }
     }
 }
 
-iac_pair_plan *find_plan(iac_ctx *ctx, const float *kmap, uint32_t npix, int cell_vox, int box_cap,
+iac_pair_plan *find_plan(iac_ctx *ctx, const float *kmap, uint32_t npix, int cell_vox, int box_cap, int region,
                          const PairSpec &S)
 {
     for (auto *p : ctx->pair_plans)
         if (p->valid && p->kmap == kmap && p->npix == npix && p->cell_vox == cell_vox && p->box_cap == box_cap &&
-            same_spec(p->spec, S))
+            p->region == region && same_spec(p->spec, S))
             return p;
     return nullptr;
 }
@@ -813,7 +825,7 @@ int get_plan(iac_ctx *ctx, hipStream_t st, const PairRun &R, int cell_vox, int b
     std::vector<uint32_t> cext(6 * (size_t)nc);
     IAC_HIP(hipMemcpyAsync(cext.data(), d_ext + 6, cext.size() * 4, hipMemcpyDeviceToHost, st));
     IAC_HIP(hipStreamSynchronize(st));
-    iac_pair_plan *pl = find_plan(ctx, R.d_kmap, R.npix, cell_vox, box_cap, S);
+    iac_pair_plan *pl = find_plan(ctx, R.d_kmap, R.npix, cell_vox, box_cap, ctx->task_region, S);
     if (pl && (std::memcmp(&pl->g, &g, sizeof(g)) != 0 || pl->cell_ext != cext)) pl = nullptr;
     if (!pl) {
         // reuse the oldest slot beyond four cached plans
@@ -829,6 +841,7 @@ int get_plan(iac_ctx *ctx, hipStream_t st, const PairRun &R, int cell_vox, int b
         pl->npix = R.npix;
         pl->cell_vox = cell_vox;
         pl->box_cap = box_cap;
+        pl->region = ctx->task_region;
         pl->spec = S;
         pl->g = g;
         pl->cell_ext.swap(cext);

@@ -22,6 +22,7 @@ def lds(ctx):
     ctx.set_option(iac.OPT_PAIR_PREFETCH, 0)
     ctx.set_option(iac.OPT_BOX_CAP, 0)
     ctx.set_option(iac.OPT_CELL_VOX_V, 0)
+    ctx.set_option(iac.OPT_TASK_REGION, 0)
 
 
 def test_lds_cfg1_full(lds, dev):
@@ -51,7 +52,8 @@ def test_lds_cfg2_subset(lds, dev):
                                                          (5, 32, 1, 2048), (6, 64, 1, 2048), (9, 16, 1, 2048),
                                                          (3, 32, 1, 1024), (4, 8, 1, 1024), (4, 32, 1, 2048),
                                                          (3008, 32, 1, 2048), (12004, 16, 1, 4096)])
-def test_lds_tuning_invariance(lds, dev, cell_vox, chunk, prefetch, box):
+@pytest.mark.parametrize("region", [0, 5])
+def test_lds_tuning_invariance(lds, dev, cell_vox, chunk, prefetch, box, region):
     """Every cell size, frame chunk and pair batch depth gives the same bits (the tile kernel's)."""
     import incohautocorrelate_amd as iac
 
@@ -66,6 +68,7 @@ def test_lds_tuning_invariance(lds, dev, cell_vox, chunk, prefetch, box):
     lds.set_option(iac.OPT_FRAME_CHUNK, chunk)
     lds.set_option(iac.OPT_PAIR_PREFETCH, prefetch)
     lds.set_option(iac.OPT_BOX_CAP, box)
+    lds.set_option(iac.OPT_TASK_REGION, region)
     got = mesh_np(gpu_batch(lds, dev, km, d["pix"], d["cnt"], d["frame_off"], shape))
     assert np.array_equal(got, tile)
 
