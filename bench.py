@@ -36,9 +36,9 @@ sys.path.insert(0, str(ROOT))
 HBM_PEAK_GBS = 8000.0  # MI355X HBM3E, public spec
 BYTES_PER_UPDATE = 16  # one 8 B bin read + one 8 B bin write per pair-update
 # Measured HBM bytes per pair-kernel launch on this workload, keyed by the engine settings
-# (box, cell_vox, frame_chunk, prefetch): FETCH_SIZE x2.0 + WRITE_SIZE x1.0 KiB from two
+# (pair mode, box, cell_vox, frame_chunk): FETCH_SIZE x2.0 + WRITE_SIZE x1.0 KiB from two
 # rocprofv3 --pmc passes, scaled by a 2 GiB copy kernel (profiles/pairs_lds_sparse1024_10k.md).
-MEASURED_TRAFFIC = {(2048, 4, 32, 1): (2 * 1494022 + 2878269) * 1024}
+MEASURED_TRAFFIC = {(2, 2048, 4, 32): (2 * 1494022 + 2878269) * 1024}
 
 
 def log(*a):
@@ -60,10 +60,11 @@ def parse():
     ap.add_argument("--no-tune", action="store_true", help="skip the cell-size / frame-chunk sweep")
     ap.add_argument("--cell-vox", type=int, default=0, help="fix the LDS cell side (skips the sweep)")
     ap.add_argument("--frame-chunk", type=int, default=0, help="fix the LDS frames per block (skips the sweep)")
-    ap.add_argument("--prefetch", type=int, default=0, help="fix the pair batch depth 1/4/8 (with the two above)")
     ap.add_argument("--box", type=int, default=0, help="fix the LDS box capacity 1024/2048/4096 (with the above)")
     ap.add_argument("--cell-vox-v", type=int, default=0, help="fix the cell side along the second k axis")
     ap.add_argument("--region", type=int, default=0, help="fix the task region side in cells (0: whole grid)")
+    ap.add_argument("--mode", type=int, default=0, help="fix the LDS pair mode (1 lanes on consecutive pairs, "
+                    "2 range per thread, 3 LDS-staged rows)")
     ap.add_argument("--no-sim", action="store_true", help="skip the simulator leg")
     ap.add_argument("--no-uneven", action="store_true", help="skip the uneven-batch leg")
     ap.add_argument("--no-probes", action="store_true", help="skip the copy / atomic ceiling probes")
@@ -121,15 +122,11 @@ def main():
 
     ctx.set_option(iac.OPT_AC3D_KERNEL, args.kernel)
     tuned = None
-    if args.cell_vox or args.frame_chunk or args.prefetch or args.box or args.cell_vox_v or args.region:
-        ctx.set_option(iac.OPT_TASK_REGION, args.region)
-        ctx.set_option(iac.OPT_CELL_VOX, args.cell_vox)
-        ctx.set_option(iac.OPT_CELL_VOX_V, args.cell_vox_v)
-        ctx.set_option(iac.OPT_FRAME_CHUNK, args.frame_chunk)
-        ctx.set_option(iac.OPT_PAIR_PREFETCH, args.prefetch)
-        ctx.set_option(iac.OPT_BOX_CAP, args.box)
+    if args.cell_vox or args.frame_chunk or args.box or args.cell_vox_v or args.region or args.mode:
         tuned = {"cell_vox": args.cell_vox, "cell_vox_v": args.cell_vox_v, "frame_chunk": args.frame_chunk,
-                 "prefetch": args.prefetch, "box": args.box, "region": args.region, "fixed": True}
+                 "box": args.box, "region": args.region, "mode": args.mode}
+        apply_lds_config(ctx, iac, tuned)
+        tuned["fixed"] = True
     elif args.kernel != 1 and not args.no_tune:
         tuned = autotune(ctx, iac, d_kmap, d_pix, d_cnt, data, shape, d_mesh, dev)
 
@@ -177,9 +174,9 @@ def main():
 
     traffic = None
     if tuned and args.kernel != 1:
-        key = (tuned.get("box") or 2048, tuned.get("cell_vox") or 4, tuned.get("frame_chunk") or 32,
-               tuned.get("prefetch") or 1)
-        if (tuned.get("cell_vox_v") or key[1]) != key[1] or tuned.get("region"):
+        key = (tuned.get("mode") or 1, tuned.get("box") or 2048, tuned.get("cell_vox") or 4,
+               tuned.get("frame_chunk") or 32)
+        if (tuned.get("cell_vox_v") or key[2]) != key[2] or tuned.get("region"):
             key = None
         traffic = MEASURED_TRAFFIC.get(key) if key else None
     out = {
@@ -228,9 +225,39 @@ def main():
     ctx.close()
 
 
+TUNE_KEYS = ("box", "cell_vox", "cell_vox_v", "region", "frame_chunk", "mode")
+
+
+def apply_lds_config(ctx, iac, cfg):
+    """Set the LDS kernel options of one tuning point (0 = library default)."""
+    ctx.set_option(iac.OPT_BOX_CAP, cfg.get("box", 0))
+    ctx.set_option(iac.OPT_PAIR_PREFETCH, 1)
+    ctx.set_option(iac.OPT_CELL_VOX, cfg.get("cell_vox", 0))
+    ctx.set_option(iac.OPT_CELL_VOX_V, cfg.get("cell_vox_v", 0))
+    ctx.set_option(iac.OPT_TASK_REGION, cfg.get("region", 0))
+    ctx.set_option(iac.OPT_FRAME_CHUNK, cfg.get("frame_chunk", 0))
+    ctx.set_option(iac.OPT_PAIR_MODE, cfg.get("mode", 0))
+
+
+def cfg_str(cfg):
+    return " ".join(f"{k}={cfg.get(k, 0)}" for k in TUNE_KEYS)
+
+
+# (pair mode, box, cell side) points; mode 1: lanes on consecutive pairs,
+# 2: a contiguous pair range per thread, 3: LDS-staged rows
+TUNE_GRID = [dict(mode=m, box=b, cell_vox=c, cell_vox_v=c, frame_chunk=fc)
+             for m, b, c, fc in ((1, 2048, 4, 32), (1, 2048, 5, 32), (1, 2048, 6, 32), (1, 2048, 6, 64),
+                                 (1, 4096, 8, 32), (1, 4096, 8, 64), (1, 1024, 4, 32), (2, 2048, 4, 32),
+                                 (3, 2048, 6, 32), (3, 4096, 8, 32))]
+
+
+UNEVEN_GRID = [dict(mode=m, box=b, cell_vox=c, cell_vox_v=c, frame_chunk=32)
+               for m, b, c in ((1, 2048, 4), (1, 2048, 6), (1, 4096, 8), (3, 2048, 6), (3, 4096, 8))]
+
+
 def autotune(ctx, iac, d_kmap, d_pix, d_cnt, data, shape, d_mesh, dev):
-    """Pick the LDS kernel's cell side (voxels) and frames per block on a
-    1,000-frame prefix by measured kernel time."""
+    """Pick the LDS kernel's pair mode, cell side (voxels), box and frames per
+    block on a 1,000-frame prefix by measured kernel time."""
     import torch
 
     off = data["frame_off"]
@@ -239,36 +266,21 @@ def autotune(ctx, iac, d_kmap, d_pix, d_cnt, data, shape, d_mesh, dev):
     d_off = torch.from_numpy(off[: nf + 1].view(np.int64)).to(dev)
     best = None
     ctx.enable_timing(True)
-    # (box, cell side along u, along v, task region in cells, frame chunk)
-    grid = [(2048, 4, 4, 0, 32), (2048, 4, 4, 16, 32), (2048, 5, 5, 12, 32), (2048, 6, 6, 8, 32),
-            (2048, 6, 6, 12, 32), (2048, 8, 8, 6, 32), (2048, 8, 8, 8, 32), (4096, 8, 8, 8, 32),
-            (4096, 8, 8, 12, 32), (4096, 10, 10, 6, 32), (4096, 12, 12, 4, 32), (2048, 6, 6, 8, 64)]
-    for box, cu, cv, reg, fc in grid:
-        ctx.set_option(iac.OPT_BOX_CAP, box)
-        ctx.set_option(iac.OPT_PAIR_PREFETCH, 1)
-        ctx.set_option(iac.OPT_CELL_VOX, cu)
-        ctx.set_option(iac.OPT_CELL_VOX_V, cv)
-        ctx.set_option(iac.OPT_TASK_REGION, reg)
-        ctx.set_option(iac.OPT_FRAME_CHUNK, fc)
+    for cfg in TUNE_GRID:
+        apply_lds_config(ctx, iac, cfg)
         ctx.ac3d_sparse_batch(d_kmap, d_pix[:n], d_cnt[:n], d_off, n, shape, d_mesh)  # plan
         ctx.kernel_time(reset=True)
         for _ in range(2):
             ctx.ac3d_sparse_batch(d_kmap, d_pix[:n], d_cnt[:n], d_off, n, shape, d_mesh)
         ms, k = ctx.kernel_time(reset=True)
         t = ms / max(k, 1)
-        log(f"[tune] box={box} cell={cu}x{cv} region={reg} frame_chunk={fc}: {t:.3f} ms / {nf} frames")
+        log(f"[tune] {cfg_str(cfg)}: {t:.3f} ms / {nf} frames")
         if best is None or t < best[0]:
-            best = (t, cu, fc, 1, box, cv, reg)
+            best = (t, cfg)
     ctx.enable_timing(False)
-    ctx.set_option(iac.OPT_CELL_VOX, best[1])
-    ctx.set_option(iac.OPT_CELL_VOX_V, best[5])
-    ctx.set_option(iac.OPT_FRAME_CHUNK, best[2])
-    ctx.set_option(iac.OPT_PAIR_PREFETCH, best[3])
-    ctx.set_option(iac.OPT_BOX_CAP, best[4])
-    ctx.set_option(iac.OPT_TASK_REGION, best[6])
-    log(f"[tune] chose box={best[4]} cell={best[1]}x{best[5]} region={best[6]} frame_chunk={best[2]}")
-    return {"cell_vox": best[1], "cell_vox_v": best[5], "frame_chunk": best[2], "prefetch": best[3], "box": best[4],
-            "region": best[6]}
+    apply_lds_config(ctx, iac, best[1])
+    log(f"[tune] chose {cfg_str(best[1])}")
+    return dict(best[1])
 
 
 def check_subset(iac, W, kmap, data, shape, dev):
@@ -388,32 +400,20 @@ def uneven_leg(args, iac, W, dev):
     if not args.no_tune:
         best = None
         ctx.enable_timing(True)
-        for box, cu, cv, reg in ((2048, 4, 4, 0), (2048, 6, 6, 8), (2048, 8, 8, 6), (4096, 8, 8, 8),
-                                 (4096, 10, 10, 6)):
-            for fc in (32,):
-                if True:
-                    ctx.set_option(iac.OPT_BOX_CAP, box)
-                    ctx.set_option(iac.OPT_CELL_VOX, cu)
-                    ctx.set_option(iac.OPT_CELL_VOX_V, cv)
-                    ctx.set_option(iac.OPT_TASK_REGION, reg)
-                    ctx.set_option(iac.OPT_FRAME_CHUNK, fc)
-                    b = ctx.ac3d_batch(d_kmap, d_pix, d_cnt, d_off, n_total, shape)
-                    b.launch(d_mesh, True)
-                    ctx.kernel_time(reset=True)
-                    b.launch(d_mesh, True)
-                    ms, k = ctx.kernel_time(reset=True)
-                    b.destroy()
-                    log(f"[uneven tune] box={box} cell={cu}x{cv} region={reg} frame_chunk={fc}: {ms:.2f} ms")
-                    if best is None or ms < best[0]:
-                        best = (ms, cu, fc, box, cv, reg)
+        for cfg in UNEVEN_GRID:
+            apply_lds_config(ctx, iac, cfg)
+            b = ctx.ac3d_batch(d_kmap, d_pix, d_cnt, d_off, n_total, shape)
+            b.launch(d_mesh, True)
+            ctx.kernel_time(reset=True)
+            b.launch(d_mesh, True)
+            ms, k = ctx.kernel_time(reset=True)
+            b.destroy()
+            log(f"[uneven tune] {cfg_str(cfg)}: {ms:.2f} ms")
+            if best is None or ms < best[0]:
+                best = (ms, cfg)
         ctx.enable_timing(False)
-        ctx.set_option(iac.OPT_CELL_VOX, best[1])
-        ctx.set_option(iac.OPT_CELL_VOX_V, best[4])
-        ctx.set_option(iac.OPT_FRAME_CHUNK, best[2])
-        ctx.set_option(iac.OPT_BOX_CAP, best[3])
-        ctx.set_option(iac.OPT_TASK_REGION, best[5])
-        tuned = {"cell_vox": best[1], "cell_vox_v": best[4], "frame_chunk": best[2], "box": best[3],
-                 "region": best[5]}
+        apply_lds_config(ctx, iac, best[1])
+        tuned = dict(best[1])
     b = ctx.ac3d_batch(d_kmap, d_pix, d_cnt, d_off, n_total, shape)
     b.capture(d_mesh, True)
     b.launch(d_mesh, True)

@@ -287,6 +287,10 @@ int iac_ctx_set_option(iac_ctx *ctx, int option, int64_t value)
         if (value < 0 || value > 4096) return fail(IAC_ERR_INVALID_ARG, "IAC_OPT_TASK_REGION is 0..4096");
         ctx->task_region = (int)value;
         return IAC_OK;
+    case IAC_OPT_PAIR_MODE:
+        if (value < 0 || value > 3) return fail(IAC_ERR_INVALID_ARG, "IAC_OPT_PAIR_MODE is 0..3");
+        ctx->pair_mode = (int)value;
+        return IAC_OK;
     case IAC_OPT_BOX_CAP:
         if (value != 0 && value != 1024 && value != 2048 && value != 4096)
             return fail(IAC_ERR_INVALID_ARG, "IAC_OPT_BOX_CAP is 0, 1024, 2048 or 4096");

@@ -48,6 +48,9 @@ struct BoxCfg {
     static constexpr int ipt = group / kBlock;                  // items per thread
 };
 constexpr int kMaxCells = 4096;                    // NC_u * NC_v
+// blocks per pair-kernel launch (tasks x chunks): 2^22 blocks of 256 threads keep
+// the grid's work-item count far below 2^32; longer runs launch in chunk slices
+constexpr uint32_t kMaxGridBlocks = 1u << 22;
 
 struct CellGeom {
     int au, av;    // k axes spanned by the cell grid
@@ -63,6 +66,7 @@ struct Task {
     int nu, nv;         // extent of that cell range
     int bx0, by0, bz0;  // LDS box origin (voxel indices)
     int bx, by, bz;     // LDS box extent; bx == 0 -> no box, every add goes to global
+    int inbox;          // the box lies inside the mesh: every nearest-neighbour pair lands in it
 };
 
 struct Chunk {
@@ -277,7 +281,7 @@ __global__ void __launch_bounds__(kBlock)
     cell_sort_kernel(const float4 *__restrict__ rec, const uint64_t *__restrict__ off, CellGeom g,
                      float4 *__restrict__ srec, uint32_t *__restrict__ cs, PairSpec S,
                      const uint32_t *__restrict__ group_off, uint32_t ngroups,
-                     unsigned long long *__restrict__ self)
+                     unsigned long long *__restrict__ self, uint32_t *__restrict__ wflag)
 {
     __shared__ uint32_t cnt[kMaxCells];
     __shared__ uint32_t part[kBlock];
@@ -289,12 +293,15 @@ __global__ void __launch_bounds__(kBlock)
     for (int c = threadIdx.x; c < ncell; c += kBlock) cnt[c] = 0;
     __syncthreads();
     unsigned long long sv = 0;
+    bool small = true;  // every kept weight an integer in [1, 129]
     for (uint32_t i = threadIdx.x; i < n; i += kBlock) {
         const float4 r = rec[base + i];
         if (r.w == 0.0f) continue;
         atomicAdd(&cnt[cell_of(g, comp(r, g.au), comp(r, g.av))], 1u);
         if (!ORD) sv += self_value<KIND>(S, r.w);
+        small = small && r.w >= 1.0f && r.w <= 129.0f && r.w == truncf(r.w);
     }
+    if (!small) wflag[0] = 0u;  // the run's flag starts at 1
     if (!ORD) {
         for (int o = 32; o > 0; o >>= 1) sv += __shfl_down(sv, o, 64);
         if ((threadIdx.x & 63) == 0) sred[threadIdx.x >> 6] = sv;
@@ -379,13 +386,220 @@ struct BoxSink {
     }
 };
 
+// Pairs [p0, p0 + left) of a group on the specialised path (AC_uw, symmetric
+// nearest-neighbour binning, identity rotation), records read from `src` (the
+// LDS stage, or the sorted records in global memory).  Software pipelined: the
+// next pair's records are requested before this pair is binned.  INBOX: every
+// pair lands in the LDS box -- its voxel range covers the rounded q range of the
+// task and lies inside the mesh -- so there is no bounds test and the box offset
+// is two 24-bit multiply-adds on byte strides; otherwise the bounds test and the
+// global sink.  SMALL: every weight of the run is an integer in [1, 129], so the
+// fixed-point value 1000 wa wb (below 2^24) is exactly (1000 wa) * wb in float.
+template <bool SMALL, bool INBOX>
+__device__ __forceinline__ void fast_pairs(const float4 *__restrict__ src, const uint4 *item, const uint32_t *pre,
+                                           int u, uint4 d, uint32_t ia, uint32_t ib, uint32_t left, bool tri,
+                                           const PairSpec &S, unsigned long long *box, const Task &T,
+                                           BoxSink &sink)
+{
+    const int fox = S.P.c0 - T.bx0, foy = S.P.c1 - T.by0, foz = S.P.c2 - T.bz0;
+    const int bx8 = 8 * T.bx, bxby8 = 8 * T.bx * T.by;
+    const int c8 = 8 * fox + bx8 * foy + bxby8 * foz;
+    const float vox = S.P.vox, cut_sq = S.P.cut_sq;
+    float4 ra = src[d.x + ia];
+    float4 rb = src[d.z + ib];
+    uint32_t bi = d.z + ib, bend = d.z + (tri ? d.y : d.w);
+    char *const boxb = reinterpret_cast<char *>(box);
+    for (;;) {
+        const float4 ca = ra, cb = rb;
+        if (left > 1) {
+            if (++bi == bend) {  // row of a done: next a, or the next item with pairs
+                ia++;
+                uint32_t nb = tri ? ia + 1 : 0;
+                if ((tri ? nb : ia) >= d.y) {
+                    do {
+                        u++;
+                    } while (pre[u + 1] == pre[u]);
+                    d = item[u];
+                    ia = 0;
+                    nb = tri ? 1 : 0;
+                }
+                ra = src[d.x + ia];
+                bi = d.z + nb;
+                bend = d.z + (tri ? d.y : d.w);
+            }
+            rb = src[bi];
+        }
+        asm volatile("" ::"v"(cb.w));  // one 16-byte load
+        const float q0 = ca.x - cb.x, q1 = ca.y - cb.y, q2 = ca.z - cb.z;
+        const float sq = (q0 * q0 + q1 * q1) + q2 * q2;
+        if (sq < cut_sq) {
+            const int dx = round_away(q0 * vox);
+            const int dy = round_away(q1 * vox);
+            const int dz = round_away(q2 * vox);
+            const uint32_t v = SMALL ? (uint32_t)((ca.w * 1000.0f) * cb.w) : pair_val_small(ca.w * cb.w);
+            if (INBOX) {
+                const int o = __mul24(dz, bxby8) + (__mul24(dy, bx8) + (dx * 8 + c8));
+                atomicAdd(reinterpret_cast<unsigned long long *>(boxb + o), (unsigned long long)v);
+            } else {
+                const uint32_t lx = (uint32_t)(dx + fox), ly = (uint32_t)(dy + foy), lz = (uint32_t)(dz + foz);
+                if (lx < (uint32_t)T.bx && ly < (uint32_t)T.by && lz < (uint32_t)T.bz)
+                    atomicAdd(box + lx + __umul24(ly, (uint32_t)T.bx) + __umul24(lz, (uint32_t)(T.bx * T.by)),
+                              (unsigned long long)v);
+                else
+                    sink.add(dx + S.P.c0, dy + S.P.c1, dz + S.P.c2, v);
+            }
+        }
+        if (--left == 0) break;
+    }
+}
+
+template <bool INBOX>
+__device__ __forceinline__ void fast_pairs_w(bool small, const float4 *__restrict__ src, const uint4 *item,
+                                             const uint32_t *pre, int u, uint4 d, uint32_t ia, uint32_t ib,
+                                             uint32_t left, bool tri, const PairSpec &S, unsigned long long *box,
+                                             const Task &T, BoxSink &sink)
+{
+    if (small)
+        fast_pairs<true, INBOX>(src, item, pre, u, d, ia, ib, left, tri, S, box, T, sink);
+    else
+        fast_pairs<false, INBOX>(src, item, pre, u, d, ia, ib, left, tri, S, box, T, sink);
+}
+
+// q = u / d, r = u % d for u < 2^24, d >= 1, from the hardware reciprocal: the
+// estimate is within two of the quotient, two correction steps each way make it exact.
+__device__ __forceinline__ void divmod24_rcp(uint32_t u, uint32_t d, uint32_t &q, uint32_t &r)
+{
+    const float inv = __builtin_amdgcn_rcpf((float)d);
+    int32_t t = (int32_t)((float)u * inv);
+    int32_t m = (int32_t)u - (int32_t)__umul24((uint32_t)t, d);
+    if (m < 0) { t--; m += (int32_t)d; }
+    if (m < 0) { t--; m += (int32_t)d; }
+    if (m >= (int32_t)d) { t++; m -= (int32_t)d; }
+    if (m >= (int32_t)d) { t++; m -= (int32_t)d; }
+    q = (uint32_t)t;
+    r = (uint32_t)m;
+}
+
+// Pair r (row-major, ia < ib) of the distinct unordered pairs of n records.
+__device__ __forceinline__ void tri_decode(uint32_t r, uint32_t n, uint32_t &ia, uint32_t &ib)
+{
+    // rows before row a: S(a) = a (2n - a - 1) / 2; a = the largest with S(a) <= r
+    const double b = 2.0 * n - 1.0;
+    uint32_t a = (uint32_t)fmax(0.0, floor((b - sqrt(fmax(0.0, b * b - 8.0 * (double)r))) * 0.5));
+    auto S = [n](uint64_t x) { return x * (2ull * n - x - 1) / 2; };
+    while (a > 0 && S(a) > r) a--;
+    while (S(a + 1) <= r) a++;
+    ia = a;
+    ib = a + 1 + (uint32_t)(r - S(a));
+}
+
+// Phase 2 of a group on the specialised path with lanes on consecutive pairs:
+// wave w walks the pair range [w0, w1) 64 at a time, lane l on pair P + l, so the
+// records a wave loads lie in the few rows those 64 pairs span rather than in
+// 64 unrelated rows (one or two cache lines per load instead of up to 64).  A
+// lane finds its item by counting the item starts at or below its pair among
+// the next 64 (a search over the wave), then its (ia, ib) by division.
+template <bool SMALL, bool INBOX, int GROUP>
+__device__ __forceinline__ void fast_walk(const float4 *__restrict__ src, const uint4 *item, const uint32_t *pre,
+                                          uint32_t w0, uint32_t w1, bool tri, const PairSpec &S,
+                                          unsigned long long *box, const Task &T, BoxSink &sink)
+{
+    const uint32_t lane = threadIdx.x & 63u;
+    const int fox = S.P.c0 - T.bx0, foy = S.P.c1 - T.by0, foz = S.P.c2 - T.bz0;
+    const int bx8 = 8 * T.bx, bxby8 = 8 * T.bx * T.by;
+    const int c8 = 8 * fox + bx8 * foy + bxby8 * foz;
+    const float vox = S.P.vox, cut_sq = S.P.cut_sq;
+    char *const boxb = reinterpret_cast<char *>(box);
+    int lo = 0, hi = GROUP;  // the item holding w0: largest u with pre[u] <= w0
+    while (hi - lo > 1) {
+        const int mid = (lo + hi) >> 1;
+        if (pre[mid] <= w0) lo = mid; else hi = mid;
+    }
+    uint32_t ub = (uint32_t)lo;
+    for (uint32_t P = w0; P < w1; P += 64) {
+        const bool active = P + lane < w1;
+        const uint32_t p = active ? P + lane : w1 - 1;
+        // item of p: ub + the number of item starts after ub at or below p
+        uint32_t u = 0, base = ub;
+        bool done = false;
+        for (;;) {
+            const uint32_t si = base + 1 + lane;
+            const uint32_t sv = si <= (uint32_t)GROUP ? pre[si] : 0xffffffffu;
+            const uint32_t s63 = __shfl(sv, 63, 64);
+            uint32_t c = 0;
+#pragma unroll
+            for (int step = 32; step > 0; step >>= 1) {
+                const uint32_t t = __shfl(sv, (int)(c + step - 1), 64);
+                if (t <= p) c += step;
+            }
+            if (c == 63 && s63 <= p) c = 64;
+            if (!done && c < 64) {
+                u = base + c;
+                done = true;
+            }
+            if (__all(done)) break;
+            base += 64;
+        }
+        const uint4 d = item[u];
+        const uint32_t r = p - pre[u];
+        uint32_t ia, ib;
+        if (tri) {
+            tri_decode(r, d.y, ia, ib);
+        } else if (r < (1u << 24)) {
+            divmod24_rcp(r, d.w, ia, ib);
+        } else {
+            ia = r / d.w;
+            ib = r - ia * d.w;
+        }
+        ub = __shfl(u, 63, 64);
+        if (active) {
+            const float4 ca = src[d.x + ia];
+            const float4 cb = src[d.z + ib];
+            asm volatile("" ::"v"(cb.w), "v"(ca.w));  // 16-byte loads
+            const float q0 = ca.x - cb.x, q1 = ca.y - cb.y, q2 = ca.z - cb.z;
+            const float sq = (q0 * q0 + q1 * q1) + q2 * q2;
+            if (sq < cut_sq) {
+                const int dx = round_away(q0 * vox);
+                const int dy = round_away(q1 * vox);
+                const int dz = round_away(q2 * vox);
+                const uint32_t v = SMALL ? (uint32_t)((ca.w * 1000.0f) * cb.w) : pair_val_small(ca.w * cb.w);
+                if (INBOX) {
+                    const int o = __mul24(dz, bxby8) + (__mul24(dy, bx8) + (dx * 8 + c8));
+                    atomicAdd(reinterpret_cast<unsigned long long *>(boxb + o), (unsigned long long)v);
+                } else {
+                    const uint32_t lx = (uint32_t)(dx + fox), ly = (uint32_t)(dy + foy), lz = (uint32_t)(dz + foz);
+                    if (lx < (uint32_t)T.bx && ly < (uint32_t)T.by && lz < (uint32_t)T.bz)
+                        atomicAdd(box + lx + __umul24(ly, (uint32_t)T.bx) + __umul24(lz, (uint32_t)(T.bx * T.by)),
+                                  (unsigned long long)v);
+                    else
+                        sink.add(dx + S.P.c0, dy + S.P.c1, dz + S.P.c2, v);
+                }
+            }
+        }
+    }
+}
+
+template <int GROUP>
+__device__ __forceinline__ void fast_walk_w(bool small, bool inbox, const float4 *__restrict__ src, const uint4 *item,
+                                            const uint32_t *pre, uint32_t w0, uint32_t w1, bool tri,
+                                            const PairSpec &S, unsigned long long *box, const Task &T, BoxSink &sink)
+{
+    if (inbox) {
+        if (small) fast_walk<true, true, GROUP>(src, item, pre, w0, w1, tri, S, box, T, sink);
+        else fast_walk<false, true, GROUP>(src, item, pre, w0, w1, tri, S, box, T, sink);
+    } else {
+        if (small) fast_walk<true, false, GROUP>(src, item, pre, w0, w1, tri, S, box, T, sink);
+        else fast_walk<false, false, GROUP>(src, item, pre, w0, w1, tri, S, box, T, sink);
+    }
+}
+
 template <int KIND, bool ORD, int PF, bool FAST, int BOX>
 __global__ void __launch_bounds__(kBlock)
     pairs_kernel(const float4 *__restrict__ srec, const uint64_t *__restrict__ off,
                  const uint32_t *__restrict__ cs, const Chunk *__restrict__ chunks,
                  const Task *__restrict__ tasks, uint32_t ntasks, CellGeom g, PairSpec S,
                  unsigned long long *__restrict__ target, size_t nvox,
-                 unsigned long long *__restrict__ dropped)
+                 unsigned long long *__restrict__ dropped, const uint32_t *__restrict__ wflag, int walk)
 {
     constexpr int kGroup = BoxCfg<BOX>::group;
     constexpr int kItemsPerThread = BoxCfg<BOX>::ipt;
@@ -405,6 +619,8 @@ __global__ void __launch_bounds__(kBlock)
     const int vbox = T.bx * T.by * T.bz;
     const int fox = S.P.c0 - T.bx0, foy = S.P.c1 - T.by0, foz = S.P.c2 - T.bz0;  // fast path offsets
     const uint32_t fbxby = (uint32_t)(T.bx * T.by);
+    const bool inbox = FAST && T.inbox != 0;
+    const bool small = FAST && wflag[0] != 0u;
     for (int v = threadIdx.x; v < vbox; v += kBlock) box[v] = 0ull;
     __syncthreads();
 
@@ -469,6 +685,14 @@ __global__ void __launch_bounds__(kBlock)
         // by pair, one load each.  PF > 1: PF pairs at a time with an integer-only
         // enumeration, so the 2 PF record loads of a batch are in flight together.
         const uint32_t total = pre[kGroup];
+        if (FAST && PF == 1 && walk) {
+            // waves on quarters of the group's pairs, lanes on consecutive pairs
+            const uint32_t w0 = (uint32_t)(((uint64_t)total * wave) / (kBlock / 64));
+            const uint32_t w1 = (uint32_t)(((uint64_t)total * (wave + 1)) / (kBlock / 64));
+            if (w0 < w1) fast_walk_w<kGroup>(small, inbox, srec, item, pre, w0, w1, tri, S, box, T, sink);
+            __syncthreads();
+            continue;
+        }
         const uint32_t p0 = (uint32_t)(((uint64_t)total * threadIdx.x) / kBlock);
         const uint32_t p1 = (uint32_t)(((uint64_t)total * (threadIdx.x + 1)) / kBlock);
         if (p0 < p1) {
@@ -492,7 +716,9 @@ __global__ void __launch_bounds__(kBlock)
                 ia = r / d.w;
                 ib = r % d.w;
             }
-            if (PF == 1) {
+            if (inbox) {
+                fast_pairs_w<true>(small, srec, item, pre, u, d, ia, ib, p1 - p0, tri, S, box, T, sink);
+            } else if (PF == 1) {
                 // one record load per pair: the a record stays in a register along its row
                 float4 ra = srec[d.x + ia];
                 for (uint32_t p = p0;;) {
@@ -596,7 +822,216 @@ __global__ void __launch_bounds__(kBlock)
     }
 }
 
+// ---- staged pair kernel (specialised path) --------------------------------------
+//
+// The pair loop above reads records straight from the sorted array: every lane of
+// a wave walks its own item, so each 16-byte load touches a different cache line
+// and the vector L1 refills from L2 about once per pair.  The staged kernel
+// copies the records of a round of task rows into LDS first: the cells of a task
+// row are consecutive in the cell-sorted frame, so a row's a-side and b-side
+// records are two contiguous runs, copied with coalesced loads, and the pair loop
+// then reads LDS.  A round is as many pieces (frame, task row, run of at most
+// kStageItems cells) as fit kStageRec records and kStageItems cell items; a piece
+// that alone exceeds the stage is paired from global memory.
+constexpr int kStageRec = 1024;   // float4 records staged per round (16 KiB)
+constexpr int kStageItems = 512;  // cell items per round
+constexpr int kStagePieces = 64;  // pieces examined per round (one wave)
+
+template <int BOX>
+__global__ void __launch_bounds__(kBlock)
+    staged_pairs_kernel(const float4 *__restrict__ srec, const uint64_t *__restrict__ off,
+                        const uint32_t *__restrict__ cs, const Chunk *__restrict__ chunks,
+                        const Task *__restrict__ tasks, uint32_t ntasks, CellGeom g, PairSpec S,
+                        unsigned long long *__restrict__ target, size_t nvox,
+                        unsigned long long *__restrict__ dropped, const uint32_t *__restrict__ wflag)
+{
+    constexpr int kIpt = kStageItems / kBlock;
+    __shared__ unsigned long long box[BOX];
+    __shared__ float4 stage[kStageRec];
+    __shared__ uint4 item[kStageItems];
+    __shared__ uint32_t pre[kStageItems + 1];
+    // per piece of the round: frame, first cell, cells, record offset, item offset, a records, b records
+    __shared__ uint32_t pc[7][kStagePieces];
+    __shared__ uint32_t wtot[kBlock / 64];
+    __shared__ uint32_t s_round[2];  // pieces staged (0: one piece from global memory), items
+
+    const Task T = tasks[blockIdx.x % ntasks];
+    const Chunk CH = chunks[blockIdx.x / ntasks];
+    const int ncell = g.ncu * g.ncv;
+    const bool tri = T.du == 0 && T.dv == 0;  // distinct unordered pairs within a cell
+    const int dlin = T.dv * g.ncu + T.du;     // cell index offset of the partner
+    const bool inbox = T.inbox != 0;
+    const bool small = wflag[0] != 0u;
+    BoxSink sink{box, target + (size_t)CH.group * nvox, dropped, T.bx0, T.by0, T.bz0, T.bx, T.by, T.bz,
+                 S.P.sab, S.P.sc, 2ull};
+    const int vbox = T.bx * T.by * T.bz;
+    for (int v = threadIdx.x; v < vbox; v += kBlock) box[v] = 0ull;
+
+    const uint32_t lane = threadIdx.x & 63u, wave = threadIdx.x >> 6;
+    const uint32_t ppr = ((uint32_t)T.nu + kStageItems - 1) / kStageItems;  // pieces per task row
+    const uint32_t npieces = (CH.f1 - CH.f0) * (uint32_t)T.nv * ppr;
+    for (uint32_t q0 = 0; q0 < npieces;) {
+        __syncthreads();  // the previous round is done with stage, item, pre and pc
+        if (wave == 0) {
+            // sizes of the next 64 pieces; the round takes the longest prefix that fits
+            const uint32_t q = q0 + lane;
+            uint32_t f = 0, ci = 0, ncl = 0, lena = 0, lenb = 0;
+            if (q < npieces) {
+                const uint32_t row = q / ppr, pcs = q - row * ppr;
+                const uint32_t fr = row / (uint32_t)T.nv, rv = row - fr * (uint32_t)T.nv;
+                const uint32_t c0 = pcs * kStageItems;
+                f = CH.f0 + fr;
+                ncl = min((uint32_t)T.nu - c0, (uint32_t)kStageItems);
+                ci = (uint32_t)((T.iv0 + (int)rv) * g.ncu + T.iu0) + c0;
+                const uint32_t *csf = cs + (size_t)f * (ncell + 1);
+                lena = csf[ci + ncl] - csf[ci];
+                if (!tri) lenb = csf[ci - dlin + ncl] - csf[ci - dlin];
+            }
+            const uint32_t recs = lena + lenb;
+            uint32_t ir = recs, ic = ncl;  // inclusive scans over the wave
+#pragma unroll
+            for (int o = 1; o < 64; o <<= 1) {
+                const uint32_t yr = __shfl_up(ir, o, 64), yc = __shfl_up(ic, o, 64);
+                if (lane >= (uint32_t)o) {
+                    ir += yr;
+                    ic += yc;
+                }
+            }
+            // both scans are monotone in the lane, so the lanes that fit are a prefix
+            const bool fits = q < npieces && ir <= (uint32_t)kStageRec && ic <= (uint32_t)kStageItems;
+            const uint32_t k = (uint32_t)__popcll(__ballot(fits));
+            if (fits || (k == 0 && lane == 0)) {
+                pc[0][lane] = f;
+                pc[1][lane] = ci;
+                pc[2][lane] = ncl;
+                pc[3][lane] = ir - recs;
+                pc[4][lane] = ic - ncl;
+                pc[5][lane] = lena;
+                pc[6][lane] = lenb;
+            }
+            const uint32_t nit = __shfl(ic, k ? (int)k - 1 : 0, 64);
+            if (lane == 0) {
+                s_round[0] = k;
+                s_round[1] = nit;
+            }
+        }
+        __syncthreads();
+        const uint32_t k = s_round[0];
+        const bool stg = k != 0;
+        const uint32_t npc = stg ? k : 1u, nit = s_round[1];
+        // stage the pieces' record runs and build their cell items: one wave per piece
+        for (uint32_t j = wave; j < npc; j += kBlock / 64) {
+            const uint32_t f = pc[0][j], ci = pc[1][j], ncl = pc[2][j], roff = pc[3][j], ioff = pc[4][j];
+            const uint32_t lena = pc[5][j], lenb = pc[6][j];
+            const uint32_t *csf = cs + (size_t)f * (ncell + 1);
+            const uint32_t base = (uint32_t)off[f];
+            const uint32_t a0 = csf[ci], b0 = tri ? a0 : csf[ci - dlin];
+            if (stg) {
+                for (uint32_t i = lane; i < lena; i += 64) stage[roff + i] = srec[base + a0 + i];
+                for (uint32_t i = lane; i < lenb; i += 64) stage[roff + lena + i] = srec[base + b0 + i];
+            }
+            // record index of a cell's first record = xbase + its sorted offset (mod 2^32)
+            const uint32_t abase = stg ? roff - a0 : base;
+            const uint32_t bbase = stg ? (tri ? roff - a0 : roff + lena - b0) : base;
+            for (uint32_t c = lane; c < ncl; c += 64) {
+                const uint32_t x0 = csf[ci + c], na = csf[ci + c + 1] - x0;
+                uint32_t y0 = x0, nb = na;
+                if (!tri) {
+                    y0 = csf[ci - dlin + c];
+                    nb = csf[ci - dlin + c + 1] - y0;
+                }
+                item[ioff + c] = make_uint4(abase + x0, na, bbase + y0, nb);
+                pre[ioff + c] = tri ? na * (na - 1) / 2 : na * nb;  // pair counts, scanned below
+            }
+        }
+        __syncthreads();
+        // exclusive scan of the item pair counts: contiguous items per thread, waves by
+        // shuffles, across waves through LDS
+        uint32_t mycnt[kIpt];
+        uint32_t tsum = 0;
+#pragma unroll
+        for (int k2 = 0; k2 < kIpt; k2++) {
+            const uint32_t li = threadIdx.x * kIpt + k2;
+            mycnt[k2] = li < nit ? pre[li] : 0u;
+            tsum += mycnt[k2];
+        }
+        uint32_t x = tsum;
+#pragma unroll
+        for (int o = 1; o < 64; o <<= 1) {
+            const uint32_t y = __shfl_up(x, o, 64);
+            if (lane >= (uint32_t)o) x += y;
+        }
+        if (lane == 63) wtot[wave] = x;
+        __syncthreads();
+        {
+            uint32_t run = x - tsum;
+            for (uint32_t w = 0; w < wave; w++) run += wtot[w];
+#pragma unroll
+            for (int k2 = 0; k2 < kIpt; k2++) {
+                pre[threadIdx.x * kIpt + k2] = run;
+                run += mycnt[k2];
+            }
+            if (threadIdx.x == kBlock - 1) pre[kStageItems] = run;
+        }
+        __syncthreads();
+
+        // thread t bins the contiguous pair range [p0, p1) of the round
+        const uint32_t total = pre[kStageItems];
+        const uint32_t p0 = (uint32_t)(((uint64_t)total * threadIdx.x) / kBlock);
+        const uint32_t p1 = (uint32_t)(((uint64_t)total * (threadIdx.x + 1)) / kBlock);
+        if (p0 < p1) {
+            int lo = 0, hi = kStageItems;  // the item holding p0: largest u with pre[u] <= p0
+            while (hi - lo > 1) {
+                const int mid = (lo + hi) >> 1;
+                if (pre[mid] <= p0) lo = mid; else hi = mid;
+            }
+            const uint4 d = item[lo];
+            uint32_t r = p0 - pre[lo];
+            uint32_t ia, ib;
+            if (tri) {
+                ia = 0;
+                while (r >= d.y - 1 - ia) {
+                    r -= d.y - 1 - ia;
+                    ia++;
+                }
+                ib = ia + 1 + r;
+            } else {
+                ia = r / d.w;
+                ib = r - ia * d.w;
+            }
+            if (stg) {
+                if (inbox) fast_pairs_w<true>(small, stage, item, pre, lo, d, ia, ib, p1 - p0, tri, S, box, T, sink);
+                else fast_pairs_w<false>(small, stage, item, pre, lo, d, ia, ib, p1 - p0, tri, S, box, T, sink);
+            } else {
+                if (inbox) fast_pairs_w<true>(small, srec, item, pre, lo, d, ia, ib, p1 - p0, tri, S, box, T, sink);
+                else fast_pairs_w<false>(small, srec, item, pre, lo, d, ia, ib, p1 - p0, tri, S, box, T, sink);
+            }
+        }
+        q0 += npc;
+    }
+    __syncthreads();
+
+    // flush the LDS box
+    unsigned long long *out = target + (size_t)CH.group * nvox;
+    for (int v = threadIdx.x; v < vbox; v += kBlock) {
+        const unsigned long long xv = box[v];
+        if (xv) {
+            const int lx = v % T.bx, rest = v / T.bx, ly = rest % T.by, lz = rest / T.by;
+            const size_t gi = (size_t)(T.bx0 + lx) + (size_t)(T.by0 + ly) * S.P.sab +
+                              (size_t)(T.bz0 + lz) * S.P.sab * S.P.sab;
+            atomicAdd(out + gi, xv);
+        }
+    }
+}
+
 // ---- host planning ---------------------------------------------------------------
+
+// The specialised pair path: AC_uw, symmetric (nearest-neighbour) binning, identity
+// rotation; round_away needs |q * vox| < 2^23 and the cut bounds |q| by Max_Q.
+inline bool fast_path(const PairSpec &S, bool ordered)
+{
+    return S.kind == kKindAc3d && !ordered && S.P.rot_identity && std::sqrt((double)S.P.cut_sq) * S.P.vox < 4194304.0;
+}
 
 // interval of round(x) / of floor(x)+{0,1} for x in [lo, hi]
 inline void nn_range(double &lo, double &hi)
@@ -710,6 +1145,7 @@ void build_tasks(iac_pair_plan &pl, bool ordered)
     const int cen[3] = {S.P.c0, S.P.c1, S.P.c2};
     const int dim[3] = {S.P.sab, S.P.sab, S.P.sc};
     const int reg = pl.region > 0 ? pl.region : std::max(g.ncu, g.ncv);
+    const bool fast = fast_path(S, ordered);
     pl.tasks.clear();
     for (int dv = ordered ? -(g.ncv - 1) : 0; dv < g.ncv; dv++) {
         for (int du = -(g.ncu - 1); du < g.ncu; du++) {
@@ -746,7 +1182,19 @@ void build_tasks(iac_pair_plan &pl, bool ordered)
                     }
                     if (std::sqrt(dist2) > maxq * (1.0 + 1e-5) + 1e-7) continue;
                     double lo[3], hi[3];
-                    voxel_box(S, qlo, qhi, lo, hi);
+                    if (fast) {
+                        // the pair bins at round_away(fl(fl(ka - kb) * vox)): within eps of x in [l, h]
+                        // (two float roundings, |x| < 2^22), so round-half-away lands in
+                        // [ceil(l - eps - 1/2), floor(h + eps + 1/2)]
+                        for (int d = 0; d < 3; d++) {
+                            const double l = qlo[d] * S.P.vox, h = qhi[d] * S.P.vox;
+                            const double eps = 1e-3 + 1e-6 * std::max(std::fabs(l), std::fabs(h));
+                            lo[d] = std::ceil(l - eps - 0.5);
+                            hi[d] = std::floor(h + eps + 0.5);
+                        }
+                    } else {
+                        voxel_box(S, qlo, qhi, lo, hi);
+                    }
                     int ilo[3], ihi[3];
                     for (int d = 0; d < 3; d++) {
                         ilo[d] = (int)std::max(0.0, lo[d] + cen[d]);
@@ -754,6 +1202,9 @@ void build_tasks(iac_pair_plan &pl, bool ordered)
                     }
                     const long vol = (long)(ihi[0] - ilo[0] + 1) * (ihi[1] - ilo[1] + 1) * (ihi[2] - ilo[2] + 1);
                     if (ihi[0] >= ilo[0] && ihi[1] >= ilo[1] && ihi[2] >= ilo[2] && vol <= pl.box_cap) {
+                        t.inbox = 1;
+                        for (int d = 0; d < 3; d++)
+                            if (lo[d] + cen[d] < 0.0 || hi[d] + cen[d] > (double)dim[d] - 1) t.inbox = 0;
                         t.bx0 = ilo[0];
                         t.by0 = ilo[1];
                         t.bz0 = ilo[2];
@@ -846,6 +1297,8 @@ int get_plan(iac_ctx *ctx, hipStream_t st, const PairRun &R, int cell_vox, int b
         pl->g = g;
         pl->cell_ext.swap(cext);
         build_tasks(*pl, R.ordered);
+        if (pl->tasks.size() > kMaxGridBlocks)
+            return fail(IAC_ERR_LIMIT, "pair engine: more than 2^22 tasks (raise the task region or the cell size)");
         if (!pl->tasks.empty()) {
             IAC_HIP(hipMalloc(&pl->d_tasks, pl->tasks.size() * sizeof(Task)));
             IAC_HIP(hipMemcpy(pl->d_tasks, pl->tasks.data(), pl->tasks.size() * sizeof(Task),
@@ -894,6 +1347,7 @@ struct Bufs {
     unsigned long long *H;  // symmetric modes: ngroups half-mesh accumulators
     uint32_t *d_goff;
     unsigned long long *d_self;
+    uint32_t *d_flag;  // 1 while every weight of the run is a small integer (after d_self)
     Chunk *d_chunks;
     uint32_t nchunks;
     const Task *d_tasks;
@@ -903,37 +1357,49 @@ struct Bufs {
 };
 
 template <int K, bool O, int BOX>
-void launch_pairs_box(int pf, bool fast, dim3 grid, hipStream_t st, const PairRun &R, const Bufs &b,
+void launch_pairs_box(int pf, bool fast, int walk, dim3 grid, hipStream_t st, const PairRun &R, const Bufs &b,
                       unsigned long long *target, size_t nvox, unsigned long long *dropped)
 {
     if ((O || pf == 1) && fast)
         hipLaunchKernelGGL((pairs_kernel<K, O, 1, K == kKindAc3d && !O, BOX>), grid, dim3(kBlock), 0, st, b.srec,
-                           R.d_frame_off, b.cs, b.d_chunks, b.d_tasks, b.ntasks, b.g, R.spec, target, nvox, dropped);
+                           R.d_frame_off, b.cs, b.d_chunks, b.d_tasks, b.ntasks, b.g, R.spec, target, nvox, dropped, b.d_flag, walk);
     else if (O || pf == 1 || BOX != kBoxMax)
         hipLaunchKernelGGL((pairs_kernel<K, O, 1, false, BOX>), grid, dim3(kBlock), 0, st, b.srec, R.d_frame_off,
-                           b.cs, b.d_chunks, b.d_tasks, b.ntasks, b.g, R.spec, target, nvox, dropped);
+                           b.cs, b.d_chunks, b.d_tasks, b.ntasks, b.g, R.spec, target, nvox, dropped, b.d_flag, walk);
     else if (pf == 4)
         hipLaunchKernelGGL((pairs_kernel<K, O, O ? 1 : 4, false, BOX>), grid, dim3(kBlock), 0, st, b.srec,
-                           R.d_frame_off, b.cs, b.d_chunks, b.d_tasks, b.ntasks, b.g, R.spec, target, nvox, dropped);
+                           R.d_frame_off, b.cs, b.d_chunks, b.d_tasks, b.ntasks, b.g, R.spec, target, nvox, dropped, b.d_flag, walk);
     else
         hipLaunchKernelGGL((pairs_kernel<K, O, O ? 1 : 8, false, BOX>), grid, dim3(kBlock), 0, st, b.srec,
-                           R.d_frame_off, b.cs, b.d_chunks, b.d_tasks, b.ntasks, b.g, R.spec, target, nvox, dropped);
+                           R.d_frame_off, b.cs, b.d_chunks, b.d_tasks, b.ntasks, b.g, R.spec, target, nvox, dropped, b.d_flag, walk);
 }
 
+// mode: 0 or 1 lanes on consecutive pairs, 2 a contiguous pair range per thread,
+// 3 LDS-staged rounds of task rows (the last two on the specialised path only)
 template <int K, bool O>
-void launch_pairs(int pf, dim3 grid, hipStream_t st, const PairRun &R, const Bufs &b, unsigned long long *target,
-                  size_t nvox, unsigned long long *dropped)
+void launch_pairs(int pf, int mode, dim3 grid, hipStream_t st, const PairRun &R, const Bufs &b,
+                  unsigned long long *target, size_t nvox, unsigned long long *dropped)
 {
-    // the specialised pair path: AC_uw, symmetric (nearest-neighbour) binning, identity rotation;
-    // round_away needs |q * vox| < 2^23 and the cut bounds |q| by Max_Q
-    const bool fast = K == kKindAc3d && !O && R.spec.P.rot_identity &&
-                      std::sqrt((double)R.spec.P.cut_sq) * R.spec.P.vox < 4194304.0;
+    const bool fast = fast_path(R.spec, O);
+    const int walk = mode == 2 ? 0 : 1;
+    if (K == kKindAc3d && !O && fast && mode == 3) {
+        if (b.box_cap == kBoxSmall)
+            hipLaunchKernelGGL(staged_pairs_kernel<kBoxSmall>, grid, dim3(kBlock), 0, st, b.srec, R.d_frame_off, b.cs,
+                               b.d_chunks, b.d_tasks, b.ntasks, b.g, R.spec, target, nvox, dropped, b.d_flag);
+        else if (b.box_cap == kBoxTiny)
+            hipLaunchKernelGGL(staged_pairs_kernel<kBoxTiny>, grid, dim3(kBlock), 0, st, b.srec, R.d_frame_off, b.cs,
+                               b.d_chunks, b.d_tasks, b.ntasks, b.g, R.spec, target, nvox, dropped, b.d_flag);
+        else
+            hipLaunchKernelGGL(staged_pairs_kernel<kBoxMax>, grid, dim3(kBlock), 0, st, b.srec, R.d_frame_off, b.cs,
+                               b.d_chunks, b.d_tasks, b.ntasks, b.g, R.spec, target, nvox, dropped, b.d_flag);
+        return;
+    }
     if (b.box_cap == kBoxSmall)
-        launch_pairs_box<K, O, kBoxSmall>(pf, fast, grid, st, R, b, target, nvox, dropped);
+        launch_pairs_box<K, O, kBoxSmall>(pf, fast, walk, grid, st, R, b, target, nvox, dropped);
     else if (b.box_cap == kBoxTiny)
-        launch_pairs_box<K, O, kBoxTiny>(pf, fast, grid, st, R, b, target, nvox, dropped);
+        launch_pairs_box<K, O, kBoxTiny>(pf, fast, walk, grid, st, R, b, target, nvox, dropped);
     else
-        launch_pairs_box<K, O, kBoxMax>(pf, fast, grid, st, R, b, target, nvox, dropped);
+        launch_pairs_box<K, O, kBoxMax>(pf, fast, walk, grid, st, R, b, target, nvox, dropped);
 }
 
 // Everything after planning, enqueue only (no host synchronisation, no
@@ -948,36 +1414,44 @@ int enqueue(iac_ctx *ctx, hipStream_t st, const PairRun &R, const Bufs &b, bool 
         hipLaunchKernelGGL(fill_u64_kernel, dim3(zb), dim3(256), 0, st, b.H, nvox * R.ngroups);
         hipLaunchKernelGGL(fill_u64_kernel, dim3(1), dim3(256), 0, st, b.d_self, (size_t)R.ngroups);
     }
+    hipLaunchKernelGGL(fill_u32_kernel, dim3(1), dim3(64), 0, st, b.d_flag, 1u, (size_t)1);
     if (S.kind == kKindAc3d) {
         if (ordered)
             hipLaunchKernelGGL((cell_sort_kernel<kKindAc3d, true>), dim3(R.nframes), dim3(kBlock), 0, st, R.rec,
-                               R.d_frame_off, b.g, b.srec, b.cs, S, b.d_goff, R.ngroups, b.d_self);
+                               R.d_frame_off, b.g, b.srec, b.cs, S, b.d_goff, R.ngroups, b.d_self, b.d_flag);
         else
             hipLaunchKernelGGL((cell_sort_kernel<kKindAc3d, false>), dim3(R.nframes), dim3(kBlock), 0, st, R.rec,
-                               R.d_frame_off, b.g, b.srec, b.cs, S, b.d_goff, R.ngroups, b.d_self);
+                               R.d_frame_off, b.g, b.srec, b.cs, S, b.d_goff, R.ngroups, b.d_self, b.d_flag);
     } else {
         if (ordered)
             hipLaunchKernelGGL((cell_sort_kernel<kKindCq, true>), dim3(R.nframes), dim3(kBlock), 0, st, R.rec,
-                               R.d_frame_off, b.g, b.srec, b.cs, S, b.d_goff, R.ngroups, b.d_self);
+                               R.d_frame_off, b.g, b.srec, b.cs, S, b.d_goff, R.ngroups, b.d_self, b.d_flag);
         else
             hipLaunchKernelGGL((cell_sort_kernel<kKindCq, false>), dim3(R.nframes), dim3(kBlock), 0, st, R.rec,
-                               R.d_frame_off, b.g, b.srec, b.cs, S, b.d_goff, R.ngroups, b.d_self);
+                               R.d_frame_off, b.g, b.srec, b.cs, S, b.d_goff, R.ngroups, b.d_self, b.d_flag);
     }
     IAC_HIP(hipGetLastError());
     auto *target = ordered ? reinterpret_cast<unsigned long long *>(R.d_mesh) : b.H;
     if (b.ntasks && b.nchunks) {
-        const dim3 grid(b.ntasks * b.nchunks);
         if (timed) IAC_TRY(timer_begin(ctx));
         // ordered (linear) binning holds eight corner adds per pair in registers: no batching there
         const int pf = ordered || ctx->pair_prefetch == 0 || ctx->pair_prefetch == 1 ? 1 : (ctx->pair_prefetch == 4 ? 4 : 8);
-        if (S.kind == kKindAc3d) {
-            if (ordered) launch_pairs<kKindAc3d, true>(pf, grid, st, R, b, target, nvox, ctx->d_counters);
-            else launch_pairs<kKindAc3d, false>(pf, grid, st, R, b, target, nvox, ctx->d_counters);
-        } else {
-            if (ordered) launch_pairs<kKindCq, true>(pf, grid, st, R, b, target, nvox, ctx->d_counters);
-            else launch_pairs<kKindCq, false>(pf, grid, st, R, b, target, nvox, ctx->d_counters);
+        // block = chunk * ntasks + task; whole chunks per launch, the slice's chunks by pointer
+        const int mode = ctx->pair_mode;
+        const uint32_t per = std::max<uint32_t>(1, kMaxGridBlocks / b.ntasks);
+        for (uint32_t c0 = 0; c0 < b.nchunks; c0 += per) {
+            Bufs bs = b;
+            bs.d_chunks = b.d_chunks + c0;
+            const dim3 grid(b.ntasks * std::min(per, b.nchunks - c0));
+            if (S.kind == kKindAc3d) {
+                if (ordered) launch_pairs<kKindAc3d, true>(pf, mode, grid, st, R, bs, target, nvox, ctx->d_counters);
+                else launch_pairs<kKindAc3d, false>(pf, mode, grid, st, R, bs, target, nvox, ctx->d_counters);
+            } else {
+                if (ordered) launch_pairs<kKindCq, true>(pf, mode, grid, st, R, bs, target, nvox, ctx->d_counters);
+                else launch_pairs<kKindCq, false>(pf, mode, grid, st, R, bs, target, nvox, ctx->d_counters);
+            }
+            IAC_HIP(hipGetLastError());
         }
-        IAC_HIP(hipGetLastError());
         if (timed) IAC_TRY(timer_end(ctx));
     }
     if (!ordered) {
@@ -1015,7 +1489,7 @@ Sizes sizes_of(const PairRun &R, int nc, size_t nchunks)
                  up((size_t)R.nframes * (nc + 1) * sizeof(uint32_t)),
                  R.ordered ? 0 : nvox * R.ngroups * sizeof(unsigned long long),
                  up((R.ngroups + 1) * sizeof(uint32_t)),
-                 up(R.ngroups * sizeof(unsigned long long)),
+                 up(R.ngroups * sizeof(unsigned long long) + sizeof(uint32_t)),
                  up(std::max<size_t>(1, nchunks) * sizeof(Chunk))};
 }
 
@@ -1053,6 +1527,7 @@ int pairs_run(iac_ctx *ctx, const PairRun &R)
     b.H = reinterpret_cast<unsigned long long *>(p2);
     b.d_goff = reinterpret_cast<uint32_t *>(p2 + z.h);
     b.d_self = reinterpret_cast<unsigned long long *>(p2 + z.h + z.goff);
+    b.d_flag = reinterpret_cast<uint32_t *>(b.d_self + R.ngroups);
     b.d_chunks = reinterpret_cast<Chunk *>(p2 + z.h + z.goff + z.self);
     b.nchunks = (uint32_t)chunks.size();
     b.d_tasks = pl->d_tasks;
@@ -1108,6 +1583,7 @@ int pairs_prepare(iac_ctx *ctx, const PairRun &R0, PairBatch **out)
     b.H = reinterpret_cast<unsigned long long *>(p + z.srec + z.cs);
     b.d_goff = reinterpret_cast<uint32_t *>(p + z.srec + z.cs + z.h);
     b.d_self = reinterpret_cast<unsigned long long *>(p + z.srec + z.cs + z.h + z.goff);
+    b.d_flag = reinterpret_cast<uint32_t *>(b.d_self + R0.ngroups);
     b.d_chunks = reinterpret_cast<Chunk *>(p + z.srec + z.cs + z.h + z.goff + z.self);
     Task *tasks = reinterpret_cast<Task *>(p + z.srec + z.cs + z.h + z.goff + z.self + z.chunks);
     b.nchunks = (uint32_t)chunks.size();

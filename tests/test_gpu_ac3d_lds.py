@@ -23,6 +23,7 @@ def lds(ctx):
     ctx.set_option(iac.OPT_BOX_CAP, 0)
     ctx.set_option(iac.OPT_CELL_VOX_V, 0)
     ctx.set_option(iac.OPT_TASK_REGION, 0)
+    ctx.set_option(iac.OPT_PAIR_MODE, 0)
 
 
 def test_lds_cfg1_full(lds, dev):
@@ -52,9 +53,11 @@ def test_lds_cfg2_subset(lds, dev):
                                                          (5, 32, 1, 2048), (6, 64, 1, 2048), (9, 16, 1, 2048),
                                                          (3, 32, 1, 1024), (4, 8, 1, 1024), (4, 32, 1, 2048),
                                                          (3008, 32, 1, 2048), (12004, 16, 1, 4096)])
-@pytest.mark.parametrize("region", [0, 5])
-def test_lds_tuning_invariance(lds, dev, cell_vox, chunk, prefetch, box, region):
-    """Every cell size, frame chunk and pair batch depth gives the same bits (the tile kernel's)."""
+@pytest.mark.parametrize("region,mode", [(0, 1), (0, 2), (0, 3), (5, 1)])
+def test_lds_tuning_invariance(lds, dev, cell_vox, chunk, prefetch, box, region, mode):
+    """Every cell size, frame chunk, pair batch depth, task region and pair walk
+    (lanes on consecutive pairs, a range per thread, LDS-staged rows) gives the
+    same bits (the tile kernel's)."""
     import incohautocorrelate_amd as iac
 
     _, km, mq = W.detector(1024)
@@ -69,6 +72,7 @@ def test_lds_tuning_invariance(lds, dev, cell_vox, chunk, prefetch, box, region)
     lds.set_option(iac.OPT_PAIR_PREFETCH, prefetch)
     lds.set_option(iac.OPT_BOX_CAP, box)
     lds.set_option(iac.OPT_TASK_REGION, region)
+    lds.set_option(iac.OPT_PAIR_MODE, mode)
     got = mesh_np(gpu_batch(lds, dev, km, d["pix"], d["cnt"], d["frame_off"], shape))
     assert np.array_equal(got, tile)
 
@@ -127,3 +131,29 @@ def test_lds_accumulates_into_existing_mesh(lds, dev):
     m = gpu_batch(lds, dev, km, *second, shape, mesh=m)
     ref, _ = O.ac3d_frames(km, d["pix"], d["cnt"], d["frame_off"], shape)
     assert np.array_equal(mesh_np(m), ref)
+
+
+@pytest.mark.parametrize("box,mode", [(2048, 3), (4096, 3), (2048, 1)])
+def test_lds_rows_beyond_the_stage(lds, dev, box, mode):
+    """Dense frames whose task rows hold more records than the LDS stage (paired
+    from global memory) between sparse frames (staged), in one batch."""
+    import incohautocorrelate_amd as iac
+
+    _, km, mq = W.detector(256)
+    shape = iac.mesh_shape_big(mq, 127)
+    rng = np.random.default_rng(11)
+    sizes = [150, 8000, 40, 0, 6000, 1, 300]
+    pix, off = [], [0]
+    for n in sizes:
+        p = np.sort(rng.choice(256 * 256, n, replace=False)) if n else np.zeros(0, np.int64)
+        pix.append(p)
+        off.append(off[-1] + n)
+    pix = np.concatenate(pix).astype(np.uint32)
+    cnt = rng.integers(1, 3, pix.size).astype(np.uint16)
+    off = np.array(off, np.uint64)
+    ref, _ = O.ac3d_frames(km, pix, cnt, off, shape)
+    lds.set_option(iac.OPT_CELL_VOX, 8)
+    lds.set_option(iac.OPT_BOX_CAP, box)
+    lds.set_option(iac.OPT_PAIR_MODE, mode)
+    got = mesh_np(gpu_batch(lds, dev, km, pix, cnt, off, shape))
+    assert np.array_equal(got, ref)
