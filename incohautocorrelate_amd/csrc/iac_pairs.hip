@@ -465,21 +465,6 @@ __device__ __forceinline__ void fast_pairs_w(bool small, const float4 *__restric
         fast_pairs<false, INBOX>(src, item, pre, u, d, ia, ib, left, tri, S, box, T, sink);
 }
 
-// q = u / d, r = u % d for u < 2^24, d >= 1, from the hardware reciprocal: the
-// estimate is within two of the quotient, two correction steps each way make it exact.
-__device__ __forceinline__ void divmod24_rcp(uint32_t u, uint32_t d, uint32_t &q, uint32_t &r)
-{
-    const float inv = __builtin_amdgcn_rcpf((float)d);
-    int32_t t = (int32_t)((float)u * inv);
-    int32_t m = (int32_t)u - (int32_t)__umul24((uint32_t)t, d);
-    if (m < 0) { t--; m += (int32_t)d; }
-    if (m < 0) { t--; m += (int32_t)d; }
-    if (m >= (int32_t)d) { t++; m -= (int32_t)d; }
-    if (m >= (int32_t)d) { t++; m -= (int32_t)d; }
-    q = (uint32_t)t;
-    r = (uint32_t)m;
-}
-
 // Pair r (row-major, ia < ib) of the distinct unordered pairs of n records.
 __device__ __forceinline__ void tri_decode(uint32_t r, uint32_t n, uint32_t &ia, uint32_t &ib)
 {
@@ -496,13 +481,21 @@ __device__ __forceinline__ void tri_decode(uint32_t r, uint32_t n, uint32_t &ia,
 // Phase 2 of a group on the specialised path with lanes on consecutive pairs:
 // wave w walks the pair range [w0, w1) 64 at a time, lane l on pair P + l, so the
 // records a wave loads lie in the few rows those 64 pairs span rather than in
-// 64 unrelated rows (one or two cache lines per load instead of up to 64).  A
-// lane finds its item by counting the item starts at or below its pair among
-// the next 64 (a search over the wave), then its (ia, ib) by division.
-template <bool SMALL, bool INBOX, int GROUP>
+// 64 unrelated rows (one or two cache lines per load instead of up to 64).  The
+// group's items are compacted (every item has pairs, so item starts are
+// distinct): the next 64 starts mark their lanes in a per-wave LDS flag row, and
+// a lane's item is the number of marked lanes up to it (ballot + mbcnt).  A
+// non-triangle item carries 1/nb (correctly rounded) in place of na, so (ia, ib)
+// is a float-reciprocal division with one correction step while the group's
+// pair total is below 2^24.
+// a volatile LDS word: cross-lane flags stay LDS accesses, in program order
+typedef volatile __attribute__((address_space(3))) uint32_t lds_flag_t;
+
+template <bool SMALL, bool INBOX>
 __device__ __forceinline__ void fast_walk(const float4 *__restrict__ src, const uint4 *item, const uint32_t *pre,
-                                          uint32_t w0, uint32_t w1, bool tri, const PairSpec &S,
-                                          unsigned long long *box, const Task &T, BoxSink &sink)
+                                          uint32_t nnz, uint32_t w0, uint32_t w1, bool tri, bool big,
+                                          const PairSpec &S, unsigned long long *box, const Task &T, BoxSink &sink,
+                                          lds_flag_t *fl)
 {
     const uint32_t lane = threadIdx.x & 63u;
     const int fox = S.P.c0 - T.bx0, foy = S.P.c1 - T.by0, foz = S.P.c2 - T.bz0;
@@ -510,48 +503,35 @@ __device__ __forceinline__ void fast_walk(const float4 *__restrict__ src, const 
     const int c8 = 8 * fox + bx8 * foy + bxby8 * foz;
     const float vox = S.P.vox, cut_sq = S.P.cut_sq;
     char *const boxb = reinterpret_cast<char *>(box);
-    int lo = 0, hi = GROUP;  // the item holding w0: largest u with pre[u] <= w0
+    uint32_t lo = 0, hi = nnz;  // the item holding w0: largest u with pre[u] <= w0
     while (hi - lo > 1) {
-        const int mid = (lo + hi) >> 1;
+        const uint32_t mid = (lo + hi) >> 1;
         if (pre[mid] <= w0) lo = mid; else hi = mid;
     }
-    uint32_t ub = (uint32_t)lo;
+    uint32_t ub = lo;  // the item holding P (its end is at or after P)
     for (uint32_t P = w0; P < w1; P += 64) {
         const bool active = P + lane < w1;
         const uint32_t p = active ? P + lane : w1 - 1;
-        // item of p: ub + the number of item starts after ub at or below p
-        uint32_t u = 0, base = ub;
-        bool done = false;
-        for (;;) {
-            const uint32_t si = base + 1 + lane;
-            const uint32_t sv = si <= (uint32_t)GROUP ? pre[si] : 0xffffffffu;
-            const uint32_t s63 = __shfl(sv, 63, 64);
-            uint32_t c = 0;
-#pragma unroll
-            for (int step = 32; step > 0; step >>= 1) {
-                const uint32_t t = __shfl(sv, (int)(c + step - 1), 64);
-                if (t <= p) c += step;
-            }
-            if (c == 63 && s63 <= p) c = 64;
-            if (!done && c < 64) {
-                u = base + c;
-                done = true;
-            }
-            if (__all(done)) break;
-            base += 64;
-        }
+        const uint32_t si = ub + 1 + lane;
+        const uint32_t t = si < nnz ? pre[si] - P : 64u;
+        fl[lane] = 0u;
+        if (t < 64u) fl[t] = 1u;
+        const uint64_t starts = __ballot(fl[lane] != 0u);
+        const uint32_t below = __builtin_amdgcn_mbcnt_hi((uint32_t)(starts >> 32),
+                                                         __builtin_amdgcn_mbcnt_lo((uint32_t)starts, 0u));
+        const uint32_t u = ub + below + (uint32_t)((starts >> lane) & 1ull);
         const uint4 d = item[u];
         const uint32_t r = p - pre[u];
         uint32_t ia, ib;
         if (tri) {
             tri_decode(r, d.y, ia, ib);
-        } else if (r < (1u << 24)) {
-            divmod24_rcp(r, d.w, ia, ib);
+        } else if (!big) {
+            divmod24(r, d.w, __uint_as_float(d.y), ia, ib);
         } else {
             ia = r / d.w;
             ib = r - ia * d.w;
         }
-        ub = __shfl(u, 63, 64);
+        ub = __builtin_amdgcn_readlane(u, 63);
         if (active) {
             const float4 ca = src[d.x + ia];
             const float4 cb = src[d.z + ib];
@@ -579,17 +559,17 @@ __device__ __forceinline__ void fast_walk(const float4 *__restrict__ src, const 
     }
 }
 
-template <int GROUP>
 __device__ __forceinline__ void fast_walk_w(bool small, bool inbox, const float4 *__restrict__ src, const uint4 *item,
-                                            const uint32_t *pre, uint32_t w0, uint32_t w1, bool tri,
-                                            const PairSpec &S, unsigned long long *box, const Task &T, BoxSink &sink)
+                                            const uint32_t *pre, uint32_t nnz, uint32_t w0, uint32_t w1, bool tri,
+                                            bool big, const PairSpec &S, unsigned long long *box, const Task &T,
+                                            BoxSink &sink, lds_flag_t *fl)
 {
     if (inbox) {
-        if (small) fast_walk<true, true, GROUP>(src, item, pre, w0, w1, tri, S, box, T, sink);
-        else fast_walk<false, true, GROUP>(src, item, pre, w0, w1, tri, S, box, T, sink);
+        if (small) fast_walk<true, true>(src, item, pre, nnz, w0, w1, tri, big, S, box, T, sink, fl);
+        else fast_walk<false, true>(src, item, pre, nnz, w0, w1, tri, big, S, box, T, sink, fl);
     } else {
-        if (small) fast_walk<true, false, GROUP>(src, item, pre, w0, w1, tri, S, box, T, sink);
-        else fast_walk<false, false, GROUP>(src, item, pre, w0, w1, tri, S, box, T, sink);
+        if (small) fast_walk<true, false>(src, item, pre, nnz, w0, w1, tri, big, S, box, T, sink, fl);
+        else fast_walk<false, false>(src, item, pre, nnz, w0, w1, tri, big, S, box, T, sink, fl);
     }
 }
 
@@ -607,6 +587,9 @@ __global__ void __launch_bounds__(kBlock)
     __shared__ uint4 item[kGroup];        // {a base, na, b base, nb}
     __shared__ uint32_t pre[kGroup + 1];  // exclusive prefix of pair counts
     __shared__ uint32_t wtot[kBlock / 64];
+    __shared__ uint32_t wnz[kBlock / 64];       // walk: items with pairs per wave
+    __shared__ uint32_t wflags[kBlock / 64][64];  // walk: item-start flags per wave
+    __shared__ uint32_t s_nnz;
 
     const Task T = tasks[blockIdx.x % ntasks];
     const Chunk CH = chunks[blockIdx.x / ntasks];
@@ -630,10 +613,12 @@ __global__ void __launch_bounds__(kBlock)
     const uint32_t nitems = (CH.f1 - CH.f0) * cells;
     const float inv_cells = 1.0f / (float)cells, inv_nu = 1.0f / (float)T.nu;
     const uint32_t lane = threadIdx.x & 63u, wave = threadIdx.x >> 6;
+    const bool walking = FAST && PF == 1 && walk;
     for (uint32_t ub = 0; ub < nitems; ub += kGroup) {
         // phase 1: item descriptors and pair counts (contiguous items per thread)
         uint32_t mycnt[kItemsPerThread];
-        uint32_t tsum = 0;
+        uint4 myd[kItemsPerThread];
+        uint32_t tsum = 0, nz = 0;
 #pragma unroll
         for (int k = 0; k < kItemsPerThread; k++) {
             const uint32_t li = threadIdx.x * kItemsPerThread + k;
@@ -654,22 +639,53 @@ __global__ void __launch_bounds__(kBlock)
                 const uint32_t b0 = csf[cj], nb = csf[cj + 1] - b0;
                 d = make_uint4(base + a0, na, base + b0, nb);
                 c = tri ? na * (na - 1) / 2 : na * nb;
+                // the walk decodes (ia, ib) of a non-triangle item with 1/nb in place of na
+                if (walking && !tri && c) d.y = __float_as_uint(1.0f / (float)nb);
             }
-            item[li] = d;
+            if (!walking) item[li] = d;
+            myd[k] = d;
             mycnt[k] = c;
             tsum += c;
+            nz += c ? 1u : 0u;
         }
         // block exclusive scan of the per-thread sums: within each wave by shuffles,
         // then across the four waves through LDS (two barriers)
-        uint32_t x = tsum;
+        uint32_t x = tsum, z = nz;
 #pragma unroll
         for (int o = 1; o < 64; o <<= 1) {
             const uint32_t y = __shfl_up(x, o, 64);
-            if (lane >= (uint32_t)o) x += y;
+            const uint32_t yz = walking ? __shfl_up(z, o, 64) : 0u;
+            if (lane >= (uint32_t)o) {
+                x += y;
+                z += yz;
+            }
         }
-        if (lane == 63) wtot[wave] = x;
+        if (lane == 63) {
+            wtot[wave] = x;
+            wnz[wave] = z;
+        }
         __syncthreads();
-        {
+        if (walking) {
+            // items with pairs only, in order: item starts become distinct
+            uint32_t run = x - tsum, rz = z - nz;
+            for (uint32_t w = 0; w < wave; w++) {
+                run += wtot[w];
+                rz += wnz[w];
+            }
+#pragma unroll
+            for (int k = 0; k < kItemsPerThread; k++) {
+                if (mycnt[k]) {
+                    item[rz] = myd[k];
+                    pre[rz] = run;
+                    rz++;
+                    run += mycnt[k];
+                }
+            }
+            if (threadIdx.x == kBlock - 1) {
+                pre[kGroup] = run;
+                s_nnz = rz;
+            }
+        } else {
             uint32_t run = x - tsum;
             for (uint32_t w = 0; w < wave; w++) run += wtot[w];
 #pragma unroll
@@ -685,11 +701,13 @@ __global__ void __launch_bounds__(kBlock)
         // by pair, one load each.  PF > 1: PF pairs at a time with an integer-only
         // enumeration, so the 2 PF record loads of a batch are in flight together.
         const uint32_t total = pre[kGroup];
-        if (FAST && PF == 1 && walk) {
+        if (walking) {
             // waves on quarters of the group's pairs, lanes on consecutive pairs
             const uint32_t w0 = (uint32_t)(((uint64_t)total * wave) / (kBlock / 64));
             const uint32_t w1 = (uint32_t)(((uint64_t)total * (wave + 1)) / (kBlock / 64));
-            if (w0 < w1) fast_walk_w<kGroup>(small, inbox, srec, item, pre, w0, w1, tri, S, box, T, sink);
+            if (w0 < w1)
+                fast_walk_w(small, inbox, srec, item, pre, s_nnz, w0, w1, tri, total >= (1u << 24), S, box, T, sink,
+                            (lds_flag_t *)wflags[wave]);
             __syncthreads();
             continue;
         }
