@@ -288,7 +288,7 @@ int iac_ctx_set_option(iac_ctx *ctx, int option, int64_t value)
         ctx->task_region = (int)value;
         return IAC_OK;
     case IAC_OPT_PAIR_MODE:
-        if (value < 0 || value > 3) return fail(IAC_ERR_INVALID_ARG, "IAC_OPT_PAIR_MODE is 0..3");
+        if (value < 0 || value > 2) return fail(IAC_ERR_INVALID_ARG, "IAC_OPT_PAIR_MODE is 0, 1 or 2");
         ctx->pair_mode = (int)value;
         return IAC_OK;
     case IAC_OPT_BOX_CAP:

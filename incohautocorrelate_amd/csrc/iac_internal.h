@@ -33,7 +33,7 @@ struct iac_ctx {
     int cell_vox = 0;     // pair engine: cell side along u in voxels (0 = 4)
     int cell_vox_v = 0;   // pair engine: cell side along v (0 = as along u)
     int task_region = 0;
-    int pair_mode = 0;  // specialised pair path: 0/1 lanes on consecutive pairs, 2 range per thread, 3 LDS-staged rows  // pair engine: cells per task region side (0 = whole grid)
+    int pair_mode = 0;  // specialised pair path: 0/1 lanes on consecutive pairs, 2 a pair range per thread  // pair engine: cells per task region side (0 = whole grid)
     uint32_t frame_chunk = 0;
     uint64_t sim_chunk_rows = 0;
     int box_cap = 0;        // pair engine: LDS box voxels, 1024, 2048 or 4096 (0 = per kind)

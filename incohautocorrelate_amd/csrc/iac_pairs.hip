@@ -243,8 +243,9 @@ __global__ void init_cell_ext_kernel(uint32_t *ext, int ncell)
         }
 }
 
-// min/max of each k component: ext[0..2] min keys, ext[3..5] max keys
-__global__ void kext_kernel(const float *__restrict__ kmap, uint32_t npix, uint32_t *ext)
+// min/max of each k component: ext[0..2] min keys, ext[3..5] max keys; reduced over
+// the wave by shuffles, then one atomic per wave and component
+__global__ void __launch_bounds__(256) kext_kernel(const float *__restrict__ kmap, uint32_t npix, uint32_t *ext)
 {
     uint32_t mn[3] = {0xffffffffu, 0xffffffffu, 0xffffffffu}, mx[3] = {0u, 0u, 0u};
     for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < npix; i += gridDim.x * blockDim.x)
@@ -253,13 +254,50 @@ __global__ void kext_kernel(const float *__restrict__ kmap, uint32_t npix, uint3
             mn[d] = min(mn[d], k);
             mx[d] = max(mx[d], k);
         }
-    for (int d = 0; d < 3; d++) {
-        atomicMin(ext + d, mn[d]);
-        atomicMax(ext + 3 + d, mx[d]);
-    }
+    for (int d = 0; d < 3; d++)
+        for (int o = 32; o > 0; o >>= 1) {
+            mn[d] = min(mn[d], (uint32_t)__shfl_xor(mn[d], o, 64));
+            mx[d] = max(mx[d], (uint32_t)__shfl_xor(mx[d], o, 64));
+        }
+    if ((threadIdx.x & 63u) == 0)
+        for (int d = 0; d < 3; d++) {
+            atomicMin(ext + d, mn[d]);
+            atomicMax(ext + 3 + d, mx[d]);
+        }
 }
 
-// per-cell k extents: ext[6c + d] min key of component d, ext[6c + 3 + d] max key
+// per-cell k extents: ext[6c + d] min key of component d, ext[6c + 3 + d] max key.
+// LDS: the block's extents of every cell (6 ncell keys, dynamic), merged into ext
+// with one atomic per touched cell and key.
+__global__ void __launch_bounds__(256)
+    cell_ext_lds_kernel(const float *__restrict__ kmap, uint32_t npix, CellGeom g, uint32_t *ext)
+{
+    extern __shared__ uint32_t sext[];
+    const int ncell = g.ncu * g.ncv;
+    for (int c = threadIdx.x; c < ncell; c += blockDim.x)
+        for (int d = 0; d < 3; d++) {
+            sext[6 * c + d] = 0xffffffffu;
+            sext[6 * c + 3 + d] = 0u;
+        }
+    __syncthreads();
+    for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < npix; i += gridDim.x * blockDim.x) {
+        const float k[3] = {kmap[3 * (size_t)i], kmap[3 * (size_t)i + 1], kmap[3 * (size_t)i + 2]};
+        const int c = cell_of(g, k[g.au], k[g.av]);
+        for (int d = 0; d < 3; d++) {
+            atomicMin(sext + 6 * c + d, fkey(k[d]));
+            atomicMax(sext + 6 * c + 3 + d, fkey(k[d]));
+        }
+    }
+    __syncthreads();
+    for (int c = threadIdx.x; c < ncell; c += blockDim.x)
+        if (sext[6 * c] != 0xffffffffu)
+            for (int d = 0; d < 3; d++) {
+                atomicMin(ext + 6 * c + d, sext[6 * c + d]);
+                atomicMax(ext + 6 * c + 3 + d, sext[6 * c + 3 + d]);
+            }
+}
+
+// the same with global atomics only (grids too large for the LDS copy)
 __global__ void cell_ext_kernel(const float *__restrict__ kmap, uint32_t npix, CellGeom g, uint32_t *ext)
 {
     for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < npix; i += gridDim.x * blockDim.x) {
@@ -521,7 +559,9 @@ __device__ __forceinline__ void fast_walk(const float4 *__restrict__ src, const 
                                                          __builtin_amdgcn_mbcnt_lo((uint32_t)starts, 0u));
         const uint32_t u = ub + below + (uint32_t)((starts >> lane) & 1ull);
         const uint4 d = item[u];
-        const uint32_t r = p - pre[u];
+        // an inactive lane's item belongs to its unclamped position, not to p: its
+        // offset is meaningless, so it decodes pair 0 (and loads nothing)
+        const uint32_t r = active ? p - pre[u] : 0u;
         uint32_t ia, ib;
         if (tri) {
             tri_decode(r, d.y, ia, ib);
@@ -840,208 +880,6 @@ __global__ void __launch_bounds__(kBlock)
     }
 }
 
-// ---- staged pair kernel (specialised path) --------------------------------------
-//
-// The pair loop above reads records straight from the sorted array: every lane of
-// a wave walks its own item, so each 16-byte load touches a different cache line
-// and the vector L1 refills from L2 about once per pair.  The staged kernel
-// copies the records of a round of task rows into LDS first: the cells of a task
-// row are consecutive in the cell-sorted frame, so a row's a-side and b-side
-// records are two contiguous runs, copied with coalesced loads, and the pair loop
-// then reads LDS.  A round is as many pieces (frame, task row, run of at most
-// kStageItems cells) as fit kStageRec records and kStageItems cell items; a piece
-// that alone exceeds the stage is paired from global memory.
-constexpr int kStageRec = 1024;   // float4 records staged per round (16 KiB)
-constexpr int kStageItems = 512;  // cell items per round
-constexpr int kStagePieces = 64;  // pieces examined per round (one wave)
-
-template <int BOX>
-__global__ void __launch_bounds__(kBlock)
-    staged_pairs_kernel(const float4 *__restrict__ srec, const uint64_t *__restrict__ off,
-                        const uint32_t *__restrict__ cs, const Chunk *__restrict__ chunks,
-                        const Task *__restrict__ tasks, uint32_t ntasks, CellGeom g, PairSpec S,
-                        unsigned long long *__restrict__ target, size_t nvox,
-                        unsigned long long *__restrict__ dropped, const uint32_t *__restrict__ wflag)
-{
-    constexpr int kIpt = kStageItems / kBlock;
-    __shared__ unsigned long long box[BOX];
-    __shared__ float4 stage[kStageRec];
-    __shared__ uint4 item[kStageItems];
-    __shared__ uint32_t pre[kStageItems + 1];
-    // per piece of the round: frame, first cell, cells, record offset, item offset, a records, b records
-    __shared__ uint32_t pc[7][kStagePieces];
-    __shared__ uint32_t wtot[kBlock / 64];
-    __shared__ uint32_t s_round[2];  // pieces staged (0: one piece from global memory), items
-
-    const Task T = tasks[blockIdx.x % ntasks];
-    const Chunk CH = chunks[blockIdx.x / ntasks];
-    const int ncell = g.ncu * g.ncv;
-    const bool tri = T.du == 0 && T.dv == 0;  // distinct unordered pairs within a cell
-    const int dlin = T.dv * g.ncu + T.du;     // cell index offset of the partner
-    const bool inbox = T.inbox != 0;
-    const bool small = wflag[0] != 0u;
-    BoxSink sink{box, target + (size_t)CH.group * nvox, dropped, T.bx0, T.by0, T.bz0, T.bx, T.by, T.bz,
-                 S.P.sab, S.P.sc, 2ull};
-    const int vbox = T.bx * T.by * T.bz;
-    for (int v = threadIdx.x; v < vbox; v += kBlock) box[v] = 0ull;
-
-    const uint32_t lane = threadIdx.x & 63u, wave = threadIdx.x >> 6;
-    const uint32_t ppr = ((uint32_t)T.nu + kStageItems - 1) / kStageItems;  // pieces per task row
-    const uint32_t npieces = (CH.f1 - CH.f0) * (uint32_t)T.nv * ppr;
-    for (uint32_t q0 = 0; q0 < npieces;) {
-        __syncthreads();  // the previous round is done with stage, item, pre and pc
-        if (wave == 0) {
-            // sizes of the next 64 pieces; the round takes the longest prefix that fits
-            const uint32_t q = q0 + lane;
-            uint32_t f = 0, ci = 0, ncl = 0, lena = 0, lenb = 0;
-            if (q < npieces) {
-                const uint32_t row = q / ppr, pcs = q - row * ppr;
-                const uint32_t fr = row / (uint32_t)T.nv, rv = row - fr * (uint32_t)T.nv;
-                const uint32_t c0 = pcs * kStageItems;
-                f = CH.f0 + fr;
-                ncl = min((uint32_t)T.nu - c0, (uint32_t)kStageItems);
-                ci = (uint32_t)((T.iv0 + (int)rv) * g.ncu + T.iu0) + c0;
-                const uint32_t *csf = cs + (size_t)f * (ncell + 1);
-                lena = csf[ci + ncl] - csf[ci];
-                if (!tri) lenb = csf[ci - dlin + ncl] - csf[ci - dlin];
-            }
-            const uint32_t recs = lena + lenb;
-            uint32_t ir = recs, ic = ncl;  // inclusive scans over the wave
-#pragma unroll
-            for (int o = 1; o < 64; o <<= 1) {
-                const uint32_t yr = __shfl_up(ir, o, 64), yc = __shfl_up(ic, o, 64);
-                if (lane >= (uint32_t)o) {
-                    ir += yr;
-                    ic += yc;
-                }
-            }
-            // both scans are monotone in the lane, so the lanes that fit are a prefix
-            const bool fits = q < npieces && ir <= (uint32_t)kStageRec && ic <= (uint32_t)kStageItems;
-            const uint32_t k = (uint32_t)__popcll(__ballot(fits));
-            if (fits || (k == 0 && lane == 0)) {
-                pc[0][lane] = f;
-                pc[1][lane] = ci;
-                pc[2][lane] = ncl;
-                pc[3][lane] = ir - recs;
-                pc[4][lane] = ic - ncl;
-                pc[5][lane] = lena;
-                pc[6][lane] = lenb;
-            }
-            const uint32_t nit = __shfl(ic, k ? (int)k - 1 : 0, 64);
-            if (lane == 0) {
-                s_round[0] = k;
-                s_round[1] = nit;
-            }
-        }
-        __syncthreads();
-        const uint32_t k = s_round[0];
-        const bool stg = k != 0;
-        const uint32_t npc = stg ? k : 1u, nit = s_round[1];
-        // stage the pieces' record runs and build their cell items: one wave per piece
-        for (uint32_t j = wave; j < npc; j += kBlock / 64) {
-            const uint32_t f = pc[0][j], ci = pc[1][j], ncl = pc[2][j], roff = pc[3][j], ioff = pc[4][j];
-            const uint32_t lena = pc[5][j], lenb = pc[6][j];
-            const uint32_t *csf = cs + (size_t)f * (ncell + 1);
-            const uint32_t base = (uint32_t)off[f];
-            const uint32_t a0 = csf[ci], b0 = tri ? a0 : csf[ci - dlin];
-            if (stg) {
-                for (uint32_t i = lane; i < lena; i += 64) stage[roff + i] = srec[base + a0 + i];
-                for (uint32_t i = lane; i < lenb; i += 64) stage[roff + lena + i] = srec[base + b0 + i];
-            }
-            // record index of a cell's first record = xbase + its sorted offset (mod 2^32)
-            const uint32_t abase = stg ? roff - a0 : base;
-            const uint32_t bbase = stg ? (tri ? roff - a0 : roff + lena - b0) : base;
-            for (uint32_t c = lane; c < ncl; c += 64) {
-                const uint32_t x0 = csf[ci + c], na = csf[ci + c + 1] - x0;
-                uint32_t y0 = x0, nb = na;
-                if (!tri) {
-                    y0 = csf[ci - dlin + c];
-                    nb = csf[ci - dlin + c + 1] - y0;
-                }
-                item[ioff + c] = make_uint4(abase + x0, na, bbase + y0, nb);
-                pre[ioff + c] = tri ? na * (na - 1) / 2 : na * nb;  // pair counts, scanned below
-            }
-        }
-        __syncthreads();
-        // exclusive scan of the item pair counts: contiguous items per thread, waves by
-        // shuffles, across waves through LDS
-        uint32_t mycnt[kIpt];
-        uint32_t tsum = 0;
-#pragma unroll
-        for (int k2 = 0; k2 < kIpt; k2++) {
-            const uint32_t li = threadIdx.x * kIpt + k2;
-            mycnt[k2] = li < nit ? pre[li] : 0u;
-            tsum += mycnt[k2];
-        }
-        uint32_t x = tsum;
-#pragma unroll
-        for (int o = 1; o < 64; o <<= 1) {
-            const uint32_t y = __shfl_up(x, o, 64);
-            if (lane >= (uint32_t)o) x += y;
-        }
-        if (lane == 63) wtot[wave] = x;
-        __syncthreads();
-        {
-            uint32_t run = x - tsum;
-            for (uint32_t w = 0; w < wave; w++) run += wtot[w];
-#pragma unroll
-            for (int k2 = 0; k2 < kIpt; k2++) {
-                pre[threadIdx.x * kIpt + k2] = run;
-                run += mycnt[k2];
-            }
-            if (threadIdx.x == kBlock - 1) pre[kStageItems] = run;
-        }
-        __syncthreads();
-
-        // thread t bins the contiguous pair range [p0, p1) of the round
-        const uint32_t total = pre[kStageItems];
-        const uint32_t p0 = (uint32_t)(((uint64_t)total * threadIdx.x) / kBlock);
-        const uint32_t p1 = (uint32_t)(((uint64_t)total * (threadIdx.x + 1)) / kBlock);
-        if (p0 < p1) {
-            int lo = 0, hi = kStageItems;  // the item holding p0: largest u with pre[u] <= p0
-            while (hi - lo > 1) {
-                const int mid = (lo + hi) >> 1;
-                if (pre[mid] <= p0) lo = mid; else hi = mid;
-            }
-            const uint4 d = item[lo];
-            uint32_t r = p0 - pre[lo];
-            uint32_t ia, ib;
-            if (tri) {
-                ia = 0;
-                while (r >= d.y - 1 - ia) {
-                    r -= d.y - 1 - ia;
-                    ia++;
-                }
-                ib = ia + 1 + r;
-            } else {
-                ia = r / d.w;
-                ib = r - ia * d.w;
-            }
-            if (stg) {
-                if (inbox) fast_pairs_w<true>(small, stage, item, pre, lo, d, ia, ib, p1 - p0, tri, S, box, T, sink);
-                else fast_pairs_w<false>(small, stage, item, pre, lo, d, ia, ib, p1 - p0, tri, S, box, T, sink);
-            } else {
-                if (inbox) fast_pairs_w<true>(small, srec, item, pre, lo, d, ia, ib, p1 - p0, tri, S, box, T, sink);
-                else fast_pairs_w<false>(small, srec, item, pre, lo, d, ia, ib, p1 - p0, tri, S, box, T, sink);
-            }
-        }
-        q0 += npc;
-    }
-    __syncthreads();
-
-    // flush the LDS box
-    unsigned long long *out = target + (size_t)CH.group * nvox;
-    for (int v = threadIdx.x; v < vbox; v += kBlock) {
-        const unsigned long long xv = box[v];
-        if (xv) {
-            const int lx = v % T.bx, rest = v / T.bx, ly = rest % T.by, lz = rest / T.by;
-            const size_t gi = (size_t)(T.bx0 + lx) + (size_t)(T.by0 + ly) * S.P.sab +
-                              (size_t)(T.bz0 + lz) * S.P.sab * S.P.sab;
-            atomicAdd(out + gi, xv);
-        }
-    }
-}
-
 // ---- host planning ---------------------------------------------------------------
 
 // The specialised pair path: AC_uw, symmetric (nearest-neighbour) binning, identity
@@ -1289,7 +1127,13 @@ int get_plan(iac_ctx *ctx, hipStream_t st, const PairRun &R, int cell_vox, int b
     g.ncv = ncells(g.av, wv);
     const int nc = g.ncu * g.ncv;
     hipLaunchKernelGGL(init_cell_ext_kernel, dim3((nc + 255) / 256), dim3(256), 0, st, d_ext + 6, nc);
-    hipLaunchKernelGGL(cell_ext_kernel, dim3(eb), dim3(256), 0, st, R.d_kmap, R.npix, g, d_ext + 6);
+    const size_t lds = 6 * sizeof(uint32_t) * (size_t)nc;
+    if (lds <= 48 * 1024) {
+        const unsigned lb = std::max(1u, std::min<unsigned>(512, (R.npix + 4095) / 4096));
+        hipLaunchKernelGGL(cell_ext_lds_kernel, dim3(lb), dim3(256), lds, st, R.d_kmap, R.npix, g, d_ext + 6);
+    } else {
+        hipLaunchKernelGGL(cell_ext_kernel, dim3(eb), dim3(256), 0, st, R.d_kmap, R.npix, g, d_ext + 6);
+    }
     IAC_HIP(hipGetLastError());
     std::vector<uint32_t> cext(6 * (size_t)nc);
     IAC_HIP(hipMemcpyAsync(cext.data(), d_ext + 6, cext.size() * 4, hipMemcpyDeviceToHost, st));
@@ -1392,26 +1236,14 @@ void launch_pairs_box(int pf, bool fast, int walk, dim3 grid, hipStream_t st, co
                            R.d_frame_off, b.cs, b.d_chunks, b.d_tasks, b.ntasks, b.g, R.spec, target, nvox, dropped, b.d_flag, walk);
 }
 
-// mode: 0 or 1 lanes on consecutive pairs, 2 a contiguous pair range per thread,
-// 3 LDS-staged rounds of task rows (the last two on the specialised path only)
+// mode: 0 or 1 lanes on consecutive pairs (the specialised path), 2 a contiguous
+// pair range per thread
 template <int K, bool O>
 void launch_pairs(int pf, int mode, dim3 grid, hipStream_t st, const PairRun &R, const Bufs &b,
                   unsigned long long *target, size_t nvox, unsigned long long *dropped)
 {
     const bool fast = fast_path(R.spec, O);
     const int walk = mode == 2 ? 0 : 1;
-    if (K == kKindAc3d && !O && fast && mode == 3) {
-        if (b.box_cap == kBoxSmall)
-            hipLaunchKernelGGL(staged_pairs_kernel<kBoxSmall>, grid, dim3(kBlock), 0, st, b.srec, R.d_frame_off, b.cs,
-                               b.d_chunks, b.d_tasks, b.ntasks, b.g, R.spec, target, nvox, dropped, b.d_flag);
-        else if (b.box_cap == kBoxTiny)
-            hipLaunchKernelGGL(staged_pairs_kernel<kBoxTiny>, grid, dim3(kBlock), 0, st, b.srec, R.d_frame_off, b.cs,
-                               b.d_chunks, b.d_tasks, b.ntasks, b.g, R.spec, target, nvox, dropped, b.d_flag);
-        else
-            hipLaunchKernelGGL(staged_pairs_kernel<kBoxMax>, grid, dim3(kBlock), 0, st, b.srec, R.d_frame_off, b.cs,
-                               b.d_chunks, b.d_tasks, b.ntasks, b.g, R.spec, target, nvox, dropped, b.d_flag);
-        return;
-    }
     if (b.box_cap == kBoxSmall)
         launch_pairs_box<K, O, kBoxSmall>(pf, fast, walk, grid, st, R, b, target, nvox, dropped);
     else if (b.box_cap == kBoxTiny)
@@ -1481,11 +1313,14 @@ int enqueue(iac_ctx *ctx, hipStream_t st, const PairRun &R, const Bufs &b, bool 
     return IAC_OK;
 }
 
-// defaults from the tuner on the 1024^2 sparse and lognormal workloads: AC_uw cells of 4
-// voxels in 2048-voxel boxes; dense C(q) cells of 4 voxels in 4096-voxel boxes
-int cell_vox_of(iac_ctx *ctx, const PairSpec &)
+// defaults from the tuner on the 1024^2 sparse and lognormal workloads: cells of 6 voxels
+// in 2048-voxel boxes on the specialised path (lanes on consecutive pairs), cells of 4
+// voxels for the other AC_uw modes (a pair range per thread), and dense C(q) cells of
+// 4 voxels in 4096-voxel boxes
+int cell_vox_of(iac_ctx *ctx, const PairRun &R)
 {
-    const int u = ctx->cell_vox > 0 ? ctx->cell_vox : 4;
+    const bool walk = fast_path(R.spec, R.ordered) && ctx->pair_mode != 2;
+    const int u = ctx->cell_vox > 0 ? ctx->cell_vox : (walk ? 6 : 4);
     return u + 1000 * (ctx->cell_vox_v > 0 ? ctx->cell_vox_v : u);
 }
 int box_cap_of(iac_ctx *ctx, const PairSpec &S)
@@ -1527,7 +1362,7 @@ int pairs_run(iac_ctx *ctx, const PairRun &R)
     hipStream_t st = ctx->stream;
     if (R.n_total >= (1ull << 32)) return fail(IAC_ERR_LIMIT, "pair engine: at most 2^32 - 1 records per call");
     iac_pair_plan *pl = nullptr;
-    IAC_TRY(get_plan(ctx, st, R, cell_vox_of(ctx, R.spec), box_cap_of(ctx, R.spec), &pl));
+    IAC_TRY(get_plan(ctx, st, R, cell_vox_of(ctx, R), box_cap_of(ctx, R.spec), &pl));
     std::vector<uint64_t> off(R.nframes + 1);
     IAC_HIP(hipMemcpyAsync(off.data(), R.d_frame_off, off.size() * 8, hipMemcpyDeviceToHost, st));
     IAC_HIP(hipStreamSynchronize(st));
@@ -1575,7 +1410,7 @@ int pairs_prepare(iac_ctx *ctx, const PairRun &R0, PairBatch **out)
     hipStream_t st = ctx->stream;
     if (R0.n_total >= (1ull << 32)) return fail(IAC_ERR_LIMIT, "pair engine: at most 2^32 - 1 records per call");
     iac_pair_plan *pl = nullptr;
-    IAC_TRY(get_plan(ctx, st, R0, cell_vox_of(ctx, R0.spec), box_cap_of(ctx, R0.spec), &pl));
+    IAC_TRY(get_plan(ctx, st, R0, cell_vox_of(ctx, R0), box_cap_of(ctx, R0.spec), &pl));
     std::vector<uint64_t> off(R0.nframes + 1);
     IAC_HIP(hipMemcpyAsync(off.data(), R0.d_frame_off, off.size() * 8, hipMemcpyDeviceToHost, st));
     IAC_HIP(hipStreamSynchronize(st));
