@@ -38,8 +38,7 @@ BYTES_PER_UPDATE = 16  # one 8 B bin read + one 8 B bin write per pair-update
 # Measured HBM bytes per pair-kernel launch on this workload, keyed by the engine settings
 # (pair mode, box, cell_vox, frame_chunk): FETCH_SIZE x2.0 + WRITE_SIZE x1.0 KiB from two
 # rocprofv3 --pmc passes, scaled by a 2 GiB copy kernel (profiles/pairs_lds_sparse1024_10k.md).
-MEASURED_TRAFFIC = {(2, 2048, 4, 32): (2 * 1494022 + 2878269) * 1024,
-                    (1, 2048, 6, 32): (2 * 1396086 + 2436887) * 1024}
+MEASURED_TRAFFIC = {(1, 1920, 6, 32): (2 * 1396086 + 2436887) * 1024}
 
 
 def log(*a):
@@ -61,7 +60,7 @@ def parse():
     ap.add_argument("--no-tune", action="store_true", help="skip the cell-size / frame-chunk sweep")
     ap.add_argument("--cell-vox", type=int, default=0, help="fix the LDS cell side (skips the sweep)")
     ap.add_argument("--frame-chunk", type=int, default=0, help="fix the LDS frames per block (skips the sweep)")
-    ap.add_argument("--box", type=int, default=0, help="fix the LDS box capacity 1024/2048/4096 (with the above)")
+    ap.add_argument("--box", type=int, default=0, help="fix the LDS box capacity 1024/1920/4096 (with the above)")
     ap.add_argument("--cell-vox-v", type=int, default=0, help="fix the cell side along the second k axis")
     ap.add_argument("--region", type=int, default=0, help="fix the task region side in cells (0: whole grid)")
     ap.add_argument("--mode", type=int, default=0, help="fix the LDS pair mode (1 lanes on consecutive pairs, "
@@ -175,7 +174,7 @@ def main():
 
     traffic = None
     if tuned and args.kernel != 1:
-        key = (tuned.get("mode") or 1, tuned.get("box") or 2048, tuned.get("cell_vox") or 4,
+        key = (tuned.get("mode") or 1, tuned.get("box") or 1920, tuned.get("cell_vox") or 6,
                tuned.get("frame_chunk") or 32)
         if (tuned.get("cell_vox_v") or key[2]) != key[2] or tuned.get("region"):
             key = None
@@ -247,12 +246,12 @@ def cfg_str(cfg):
 # (pair mode, box, cell side) points; mode 1: lanes on consecutive pairs,
 # 2: a contiguous pair range per thread
 TUNE_GRID = [dict(mode=m, box=b, cell_vox=c, cell_vox_v=c, frame_chunk=fc)
-             for m, b, c, fc in ((1, 2048, 4, 32), (1, 2048, 5, 32), (1, 2048, 6, 32), (1, 2048, 6, 64),
-                                 (1, 4096, 8, 32), (1, 1024, 4, 32), (1, 1024, 5, 32), (2, 2048, 4, 32))]
+             for m, b, c, fc in ((1, 1920, 4, 32), (1, 1920, 5, 32), (1, 1920, 6, 32), (1, 1920, 6, 64),
+                                 (1, 4096, 8, 32), (1, 1024, 4, 32), (1, 1024, 5, 32), (2, 1920, 4, 32))]
 
 
 UNEVEN_GRID = [dict(mode=m, box=b, cell_vox=c, cell_vox_v=c, frame_chunk=32)
-               for m, b, c in ((1, 2048, 4), (1, 2048, 5), (1, 2048, 6), (1, 4096, 8))]
+               for m, b, c in ((1, 1920, 4), (1, 1920, 5), (1, 1920, 6), (1, 4096, 8))]
 
 
 def autotune(ctx, iac, d_kmap, d_pix, d_cnt, data, shape, d_mesh, dev):

@@ -90,7 +90,8 @@ int iac_ctx_timing(iac_ctx *ctx, double *ms, uint64_t *launches, int reset);
                                  AC_uw nearest-neighbour identity-rotation walk, else 4) */
 #define IAC_OPT_FRAME_CHUNK 3 /* LDS kernel: mean frames per block, 0 = default (32) */
 #define IAC_OPT_PAIR_PREFETCH 5 /* LDS kernel: record pairs loaded per batch, 1, 4 or 8 (0 = 1) */
-#define IAC_OPT_BOX_CAP 6 /* LDS kernel: box capacity in voxels, 1024, 2048 or 4096 (0 = 2048 for AC_uw, 4096 for C(q)) */
+#define IAC_OPT_BOX_CAP 6 /* LDS kernel: box capacity in voxels, 1024, 1920 (2048 selects it too) or 4096
+                               (0 = 1920 for AC_uw, 4096 for C(q)) */
 #define IAC_OPT_CELL_VOX_V 7 /* LDS kernel: cell side along the second k axis, 0 = as IAC_OPT_CELL_VOX */
 #define IAC_OPT_TASK_REGION 8 /* LDS kernel: first cells per task region side, 0 = the whole grid */
 #define IAC_OPT_PAIR_MODE 9 /* LDS kernel, AC_uw nearest-neighbour identity-rotation path: 0 default (1), 1 lanes

@@ -292,8 +292,8 @@ int iac_ctx_set_option(iac_ctx *ctx, int option, int64_t value)
         ctx->pair_mode = (int)value;
         return IAC_OK;
     case IAC_OPT_BOX_CAP:
-        if (value != 0 && value != 1024 && value != 2048 && value != 4096)
-            return fail(IAC_ERR_INVALID_ARG, "IAC_OPT_BOX_CAP is 0, 1024, 2048 or 4096");
+        if (value != 0 && value != 1024 && value != 1920 && value != 2048 && value != 4096)
+            return fail(IAC_ERR_INVALID_ARG, "IAC_OPT_BOX_CAP is 0, 1024, 1920 (or 2048, the same) or 4096");
         ctx->box_cap = (int)value;
         return IAC_OK;
     case IAC_OPT_PAIR_PREFETCH:

@@ -37,11 +37,18 @@ namespace {
 
 constexpr int kBlock = 256;
 // LDS box capacity (uint64 voxels): 4096 (32 KiB, 1024 items per scan group, 3 blocks
-// per CU), 2048 (16 KiB, 512 items, 5 blocks) or 1024 (8 KiB, 512 items, 8 blocks);
-// chosen per plan.
+// per CU), 1920 (15 KiB, 512 items: 25.9 KB in all, so 6 blocks per CU) or 1024 (8 KiB,
+// 512 items); chosen per plan.  1920 holds the tight boxes of 6-voxel cells (at most
+// 1,859 voxels on a 1024^2 detector).
 constexpr int kBoxMax = 4096;
-constexpr int kBoxSmall = 2048;
+constexpr int kBoxSmall = 1920;
 constexpr int kBoxTiny = 1024;
+// waves per SIMD the specialised-path kernels are compiled for: 6 keeps them within
+// 85 VGPRs (no spills) so that 6 blocks fit a CU
+template <bool FAST, int BOX>
+struct Occupancy {
+    static constexpr int blocks = FAST && BOX != kBoxMax ? 6 : 1;
+};
 template <int BOX>
 struct BoxCfg {
     static constexpr int group = BOX == kBoxMax ? 1024 : 512;  // items per scan group
@@ -526,8 +533,8 @@ __device__ __forceinline__ void tri_decode(uint32_t r, uint32_t n, uint32_t &ia,
 // non-triangle item carries 1/nb (correctly rounded) in place of na, so (ia, ib)
 // is a float-reciprocal division with one correction step while the group's
 // pair total is below 2^24.
-// a volatile LDS word: cross-lane flags stay LDS accesses, in program order
-typedef volatile __attribute__((address_space(3))) uint32_t lds_flag_t;
+// a volatile LDS byte: cross-lane flags stay LDS accesses, in program order
+typedef volatile __attribute__((address_space(3))) uint8_t lds_flag_t;
 
 template <bool SMALL, bool INBOX>
 __device__ __forceinline__ void fast_walk(const float4 *__restrict__ src, const uint4 *item, const uint32_t *pre,
@@ -614,7 +621,7 @@ __device__ __forceinline__ void fast_walk_w(bool small, bool inbox, const float4
 }
 
 template <int KIND, bool ORD, int PF, bool FAST, int BOX>
-__global__ void __launch_bounds__(kBlock)
+__global__ void __launch_bounds__(kBlock, (Occupancy<FAST, BOX>::blocks))
     pairs_kernel(const float4 *__restrict__ srec, const uint64_t *__restrict__ off,
                  const uint32_t *__restrict__ cs, const Chunk *__restrict__ chunks,
                  const Task *__restrict__ tasks, uint32_t ntasks, CellGeom g, PairSpec S,
@@ -628,7 +635,7 @@ __global__ void __launch_bounds__(kBlock)
     __shared__ uint32_t pre[kGroup + 1];  // exclusive prefix of pair counts
     __shared__ uint32_t wtot[kBlock / 64];
     __shared__ uint32_t wnz[kBlock / 64];       // walk: items with pairs per wave
-    __shared__ uint32_t wflags[kBlock / 64][64];  // walk: item-start flags per wave
+    __shared__ uint8_t wflags[kBlock / 64][64];  // walk: item-start flags per wave
     __shared__ uint32_t s_nnz;
 
     const Task T = tasks[blockIdx.x % ntasks];
@@ -1314,7 +1321,7 @@ int enqueue(iac_ctx *ctx, hipStream_t st, const PairRun &R, const Bufs &b, bool 
 }
 
 // defaults from the tuner on the 1024^2 sparse and lognormal workloads: cells of 6 voxels
-// in 2048-voxel boxes on the specialised path (lanes on consecutive pairs), cells of 4
+// in 1920-voxel boxes on the specialised path (lanes on consecutive pairs), cells of 4
 // voxels for the other AC_uw modes (a pair range per thread), and dense C(q) cells of
 // 4 voxels in 4096-voxel boxes
 int cell_vox_of(iac_ctx *ctx, const PairRun &R)
@@ -1325,7 +1332,8 @@ int cell_vox_of(iac_ctx *ctx, const PairRun &R)
 }
 int box_cap_of(iac_ctx *ctx, const PairSpec &S)
 {
-    if (ctx->box_cap == kBoxMax || ctx->box_cap == kBoxSmall || ctx->box_cap == kBoxTiny) return ctx->box_cap;
+    if (ctx->box_cap == kBoxMax || ctx->box_cap == kBoxTiny) return ctx->box_cap;
+    if (ctx->box_cap == kBoxSmall || ctx->box_cap == 2048) return kBoxSmall;
     return S.kind == kKindAc3d ? kBoxSmall : kBoxMax;
 }
 uint32_t fchunk_of(iac_ctx *ctx) { return ctx->frame_chunk > 0 ? ctx->frame_chunk : 32; }

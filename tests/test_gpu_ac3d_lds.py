@@ -50,9 +50,9 @@ def test_lds_cfg2_subset(lds, dev):
 
 @pytest.mark.parametrize("cell_vox,chunk,prefetch,box", [(2, 1, 8, 4096), (3, 7, 4, 4096), (4, 128, 1, 4096),
                                                          (8, 64, 8, 4096), (16, 300, 4, 4096), (6, 64, 1, 4096),
-                                                         (5, 32, 1, 2048), (6, 64, 1, 2048), (9, 16, 1, 2048),
+                                                         (5, 32, 1, 1920), (6, 64, 1, 1920), (9, 16, 1, 1920),
                                                          (3, 32, 1, 1024), (4, 8, 1, 1024), (4, 32, 1, 2048),
-                                                         (3008, 32, 1, 2048), (12004, 16, 1, 4096)])
+                                                         (3008, 32, 1, 1920), (12004, 16, 1, 4096)])
 @pytest.mark.parametrize("region,mode", [(0, 1), (0, 2), (5, 1)])
 def test_lds_tuning_invariance(lds, dev, cell_vox, chunk, prefetch, box, region, mode):
     """Every cell size, frame chunk, pair batch depth, task region and pair walk
@@ -133,7 +133,7 @@ def test_lds_accumulates_into_existing_mesh(lds, dev):
     assert np.array_equal(mesh_np(m), ref)
 
 
-@pytest.mark.parametrize("box,mode", [(2048, 1), (4096, 1), (2048, 2)])
+@pytest.mark.parametrize("box,mode", [(1920, 1), (4096, 1), (1920, 2)])
 def test_lds_dense_and_sparse_frames(lds, dev, box, mode):
     """Dense frames (groups of more than 2^24 pairs, items of ~15k pairs) between
     sparse and empty ones, in one batch."""
