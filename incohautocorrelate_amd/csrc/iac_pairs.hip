@@ -1103,7 +1103,8 @@ int get_plan(iac_ctx *ctx, hipStream_t st, const PairRun &R, int cell_vox, int b
     hipLaunchKernelGGL(fill_u32_kernel, dim3(1), dim3(64), 0, st, d_ext, 0xffffffffu, (size_t)3);
     hipLaunchKernelGGL(fill_u32_kernel, dim3(1), dim3(64), 0, st, d_ext + 3, 0u, (size_t)3);
     const unsigned eb = std::max(1u, std::min<unsigned>(1024, (R.npix + 255) / 256));
-    hipLaunchKernelGGL(kext_kernel, dim3(eb), dim3(256), 0, st, R.d_kmap, R.npix, d_ext);
+    // few blocks: the six extent words take one atomic per wave, all on the same addresses
+    hipLaunchKernelGGL(kext_kernel, dim3(std::min(eb, 64u)), dim3(256), 0, st, R.d_kmap, R.npix, d_ext);
     IAC_HIP(hipGetLastError());
     uint32_t hext[6];
     IAC_HIP(hipMemcpyAsync(hext, d_ext, sizeof(hext), hipMemcpyDeviceToHost, st));
