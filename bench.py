@@ -67,6 +67,8 @@ def parse():
                     "2 range per thread)")
     ap.add_argument("--no-sim", action="store_true", help="skip the simulator leg")
     ap.add_argument("--no-uneven", action="store_true", help="skip the uneven-batch leg")
+    ap.add_argument("--no-ac1d", action="store_true", help="skip the AC1D leg")
+    ap.add_argument("--no-cq", action="store_true", help="skip the dense C(q) leg")
     ap.add_argument("--no-probes", action="store_true", help="skip the copy / atomic ceiling probes")
     ap.add_argument("--uneven-only", action="store_true", help="run only the uneven-batch leg")
     ap.add_argument("--sim-only", action="store_true", help="run only the simulator leg (prints its JSON object)")
@@ -166,8 +168,10 @@ def main():
     if not args.no_cpu:
         cpu = cpu_baseline(kmap, data, shape, ctx, iac, d_kmap, dev, args)
 
+    ac1d = None if args.no_ac1d else ac1d_leg(args, iac, W, dev, d_kmap, d_pix, d_cnt, d_off, data, max_q)
     del d_pix, d_cnt, d_off, d_mesh
     torch.cuda.empty_cache()
+    cq = None if args.no_cq else cq_dense_leg(args, iac, W, dev)
     probes = None if args.no_probes else probe_leg(iac, dev, nvox)
     uneven = None if args.no_uneven else uneven_leg(args, iac, W, dev)
     sim = None if args.no_sim else sim_leg(args, iac, W, dev)
@@ -219,6 +223,8 @@ def main():
         "cpu_baseline": cpu,
         "probes": probes,
         "uneven_batch": uneven,
+        "ac1d": ac1d,
+        "cq_dense": cq,
         "simulator": sim,
     }
     print(json.dumps(out), flush=True)
@@ -252,6 +258,10 @@ TUNE_GRID = [dict(mode=m, box=b, cell_vox=c, cell_vox_v=c, frame_chunk=fc)
 
 UNEVEN_GRID = [dict(mode=m, box=b, cell_vox=c, cell_vox_v=c, frame_chunk=32)
                for m, b, c in ((1, 1920, 4), (1, 1920, 5), (1, 1920, 6), (1, 4096, 8))]
+
+
+CQ_GRID = [dict(mode=m, box=b, cell_vox=c, cell_vox_v=c, frame_chunk=fc, region=r)
+           for m in (1, 2) for b, c in ((4096, 4), (1920, 4), (1920, 3)) for fc in (1, 32) for r in (0, 4)]
 
 
 def autotune(ctx, iac, d_kmap, d_pix, d_cnt, data, shape, d_mesh, dev):
@@ -335,6 +345,22 @@ def cpu_baseline(kmap, data, shape, ctx, iac, d_kmap, dev, args):
                           torch.from_numpy(off[: nf + 1].view(np.int64)).to(dev), n, shape, m)
     upd = int(m.sum().item()) // 1000
     log(f"[bench] CPU oracle: {nf} frames, {upd:.3e} updates in {dt:.2f}s on {cores} threads")
+    # the same restatement on one thread, calibrated on 2 frames, over a prefix of about a
+    # third of that time
+    t = time.perf_counter()
+    O.ac3d_frames(kmap, data["pix"][: off[2]], data["cnt"][: off[2]], off[:3], shape, nthreads=1)
+    per1 = (time.perf_counter() - t) / 2
+    nf1 = int(max(2, min(nf, (args.cpu_seconds / 3) / max(per1, 1e-6))))
+    n1 = int(off[nf1])
+    t = time.perf_counter()
+    O.ac3d_frames(kmap, data["pix"][:n1], data["cnt"][:n1], off[: nf1 + 1], shape, nthreads=1)
+    dt1 = time.perf_counter() - t
+    m.zero_()
+    ctx.ac3d_sparse_batch(d_kmap, torch.from_numpy(data["pix"][:n1].view(np.int32)).to(dev),
+                          torch.ones(n1, dtype=torch.int16, device=dev),
+                          torch.from_numpy(off[: nf1 + 1].view(np.int64)).to(dev), n1, shape, m)
+    upd1 = int(m.sum().item()) // 1000
+    log(f"[bench] CPU oracle, 1 thread: {nf1} frames, {upd1:.3e} updates in {dt1:.2f}s")
     return {
         "value": upd / dt,
         "unit": "pair-updates/s",
@@ -342,7 +368,96 @@ def cpu_baseline(kmap, data, shape, ctx, iac, d_kmap, dev, args):
         "kind": "port",
         "sample": f"first {nf} of the {len(off) - 1} frames ({upd} pair-updates), "
                   f"OpenMP {cores} threads, -O3 -fopenmp -ffp-contract=off",
+        "single_thread": {"value": upd1 / dt1, "cores": 1,
+                          "sample": f"first {nf1} frames ({upd1} pair-updates), 1 thread"},
     }
+
+
+def ac1d_leg(args, iac, W, dev, d_kmap, d_pix, d_cnt, d_off, data, max_q_vec):
+    """AC1D AC_uw on the headline workload: the radial |q| histogram of every frame's
+    ordered photon pairs (4096 bins, linear split as the evaluation forces), timed
+    against the measured LDS-atomic ceiling."""
+    import torch
+
+    size = 4096
+    max_q, dq = iac.ac1d_shape(max_q_vec, size)
+    ctx = iac.Context(0)
+    n_total = int(data["frame_off"][-1])
+    sizes = np.diff(data["frame_off"].astype(np.int64))
+    pairs = int((sizes.astype(np.float64) ** 2).sum())  # ordered pairs, a = b included
+    h = torch.zeros(size, dtype=torch.float64, device=dev)
+
+    def run():
+        h.zero_()
+        ctx.ac1d_acuw(d_kmap, d_pix, d_cnt, d_off, n_total, size, max_q, dq, 1, h)
+
+    run()
+    torch.cuda.synchronize()
+    steps = max(1, args.steps)
+    t = time.perf_counter()
+    for _ in range(steps):
+        run()
+    torch.cuda.synchronize()
+    ms = (time.perf_counter() - t) * 1e3 / steps
+    ctx.close()
+    log(f"[ac1d] {ms:.2f} ms/step, {pairs:.4e} ordered pairs, {pairs / (ms * 1e-3):.4e} pairs/s")
+    return {"workload": f"AC1D AC_uw of the headline frames: {len(sizes)} frames, {size} bins, linear split, "
+                        f"self pairs weighted 2",
+            "value": pairs / (ms * 1e-3), "unit": "ordered photon pairs/s", "ms_per_step": ms, "steps": steps,
+            "pairs_per_step": pairs}
+
+
+def cq_dense_leg(args, iac, W, dev):
+    """Dense pixel-pair C(q) (cfg3 shape): 256^2 FP32 Gamma(1, 0.5) frames into the
+    129^3 small C(q) mesh, every ordered pixel pair of each frame."""
+    import torch
+
+    n, nf = 256, 8
+    _, kmap, mq = W.detector(n)
+    s = iac.mesh_shape_small_cofq(mq, 127)
+    frames = W.gamma_frames(n * n, nf, 1.0, 0.5, seed=args.seed + 7)
+    ctx = iac.Context(0)
+    d_kmap = torch.from_numpy(kmap).to(dev)
+    d_frames = torch.from_numpy(frames).to(dev)
+    mesh = torch.zeros(s.size_ab ** 2 * s.size_c, dtype=torch.int64, device=dev)
+    goff = np.array([0, nf], np.uint32)
+    mult = iac.cq_small_multiplicator(frames[0])
+
+    def run():
+        mesh.zero_()
+        ctx.cq_dense_frames(d_kmap, d_frames, goff, s, mesh, iac.NEAREST, mult)
+
+    tuned = None
+    if not args.no_tune:
+        best = None
+        ctx.enable_timing(True)
+        for cfg in CQ_GRID:
+            apply_lds_config(ctx, iac, cfg)
+            run()
+            ctx.kernel_time(reset=True)
+            run()
+            ms, _ = ctx.kernel_time(reset=True)
+            log(f"[cq tune] {cfg_str(cfg)}: {ms:.2f} ms")
+            if best is None or ms < best[0]:
+                best = (ms, cfg)
+        ctx.enable_timing(False)
+        apply_lds_config(ctx, iac, best[1])
+        tuned = dict(best[1])
+    run()
+    torch.cuda.synchronize()
+    steps = max(1, min(args.steps, 3))
+    t = time.perf_counter()
+    for _ in range(steps):
+        run()
+    torch.cuda.synchronize()
+    ms = (time.perf_counter() - t) * 1e3 / steps
+    ctx.close()
+    pairs = float(n * n) ** 2 * nf
+    log(f"[cq] dense {n}^2 x {nf} frames: {ms:.2f} ms/step, {pairs / (ms * 1e-3):.4e} pixel pairs/s")
+    return {"workload": f"dense C(q) small (AutoCorr_CQ_small): {n}x{n} detector, {nf} FP32 Gamma(1, 0.5) frames, "
+                        f"{s.size_ab}x{s.size_ab}x{s.size_c} uint64 mesh, nearest neighbour",
+            "value": pairs / (ms * 1e-3), "unit": "pixel pairs/s", "ms_per_step": ms, "steps": steps,
+            "pairs_per_step": pairs, "tuned": tuned}
 
 
 def probe_leg(iac, dev, nvox):

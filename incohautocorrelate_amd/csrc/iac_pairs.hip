@@ -536,7 +536,7 @@ __device__ __forceinline__ void tri_decode(uint32_t r, uint32_t n, uint32_t &ia,
 // a volatile LDS byte: cross-lane flags stay LDS accesses, in program order
 typedef volatile __attribute__((address_space(3))) uint8_t lds_flag_t;
 
-template <bool SMALL, bool INBOX>
+template <int KIND, bool SMALL, bool INBOX>
 __device__ __forceinline__ void fast_walk(const float4 *__restrict__ src, const uint4 *item, const uint32_t *pre,
                                           uint32_t nnz, uint32_t w0, uint32_t w1, bool tri, bool big,
                                           const PairSpec &S, unsigned long long *box, const Task &T, BoxSink &sink,
@@ -579,7 +579,14 @@ __device__ __forceinline__ void fast_walk(const float4 *__restrict__ src, const 
             ib = r - ia * d.w;
         }
         ub = __builtin_amdgcn_readlane(u, 63);
-        if (active) {
+        if (active && KIND == kKindCq) {
+            // C(q): its own value and axis order, through the checked box
+            const float4 ca = src[d.x + ia];
+            const float4 cb = src[d.z + ib];
+            int fs, ms, ss;
+            uint64_t v;
+            if (bin_sym<kKindCq>(S, ca, cb, fs, ms, ss, v)) sink.add(fs, ms, ss, v);
+        } else if (active) {
             const float4 ca = src[d.x + ia];
             const float4 cb = src[d.z + ib];
             asm volatile("" ::"v"(cb.w), "v"(ca.w));  // 16-byte loads
@@ -606,17 +613,20 @@ __device__ __forceinline__ void fast_walk(const float4 *__restrict__ src, const 
     }
 }
 
+template <int KIND>
 __device__ __forceinline__ void fast_walk_w(bool small, bool inbox, const float4 *__restrict__ src, const uint4 *item,
                                             const uint32_t *pre, uint32_t nnz, uint32_t w0, uint32_t w1, bool tri,
                                             bool big, const PairSpec &S, unsigned long long *box, const Task &T,
                                             BoxSink &sink, lds_flag_t *fl)
 {
-    if (inbox) {
-        if (small) fast_walk<true, true>(src, item, pre, nnz, w0, w1, tri, big, S, box, T, sink, fl);
-        else fast_walk<false, true>(src, item, pre, nnz, w0, w1, tri, big, S, box, T, sink, fl);
+    if (KIND == kKindCq) {
+        fast_walk<kKindCq, false, false>(src, item, pre, nnz, w0, w1, tri, big, S, box, T, sink, fl);
+    } else if (inbox) {
+        if (small) fast_walk<KIND, true, true>(src, item, pre, nnz, w0, w1, tri, big, S, box, T, sink, fl);
+        else fast_walk<KIND, false, true>(src, item, pre, nnz, w0, w1, tri, big, S, box, T, sink, fl);
     } else {
-        if (small) fast_walk<true, false>(src, item, pre, nnz, w0, w1, tri, big, S, box, T, sink, fl);
-        else fast_walk<false, false>(src, item, pre, nnz, w0, w1, tri, big, S, box, T, sink, fl);
+        if (small) fast_walk<KIND, true, false>(src, item, pre, nnz, w0, w1, tri, big, S, box, T, sink, fl);
+        else fast_walk<KIND, false, false>(src, item, pre, nnz, w0, w1, tri, big, S, box, T, sink, fl);
     }
 }
 
@@ -660,7 +670,8 @@ __global__ void __launch_bounds__(kBlock, (Occupancy<FAST, BOX>::blocks))
     const uint32_t nitems = (CH.f1 - CH.f0) * cells;
     const float inv_cells = 1.0f / (float)cells, inv_nu = 1.0f / (float)T.nu;
     const uint32_t lane = threadIdx.x & 63u, wave = threadIdx.x >> 6;
-    const bool walking = FAST && PF == 1 && walk;
+    // lanes on consecutive pairs: the AC_uw specialised path and symmetric C(q)
+    const bool walking = (FAST || (KIND == kKindCq && !ORD)) && PF == 1 && walk;
     for (uint32_t ub = 0; ub < nitems; ub += kGroup) {
         // phase 1: item descriptors and pair counts (contiguous items per thread)
         uint32_t mycnt[kItemsPerThread];
@@ -753,7 +764,7 @@ __global__ void __launch_bounds__(kBlock, (Occupancy<FAST, BOX>::blocks))
             const uint32_t w0 = (uint32_t)(((uint64_t)total * wave) / (kBlock / 64));
             const uint32_t w1 = (uint32_t)(((uint64_t)total * (wave + 1)) / (kBlock / 64));
             if (w0 < w1)
-                fast_walk_w(small, inbox, srec, item, pre, s_nnz, w0, w1, tri, total >= (1u << 24), S, box, T, sink,
+                fast_walk_w<KIND>(small, inbox, srec, item, pre, s_nnz, w0, w1, tri, total >= (1u << 24), S, box, T, sink,
                             (lds_flag_t *)wflags[wave]);
             __syncthreads();
             continue;
@@ -1094,7 +1105,8 @@ iac_pair_plan *find_plan(iac_ctx *ctx, const float *kmap, uint32_t npix, int cel
 
 // Cell grid of the k-map (two host round trips for the extents) and the task
 // table, from the context's cache when the geometry matches.
-int get_plan(iac_ctx *ctx, hipStream_t st, const PairRun &R, int cell_vox, int box_cap, iac_pair_plan **out)
+int get_plan(iac_ctx *ctx, hipStream_t st, const PairRun &R, int cell_vox, int box_cap, int region,
+             iac_pair_plan **out)
 {
     const PairSpec &S = R.spec;
     void *ext = nullptr;
@@ -1146,7 +1158,7 @@ int get_plan(iac_ctx *ctx, hipStream_t st, const PairRun &R, int cell_vox, int b
     std::vector<uint32_t> cext(6 * (size_t)nc);
     IAC_HIP(hipMemcpyAsync(cext.data(), d_ext + 6, cext.size() * 4, hipMemcpyDeviceToHost, st));
     IAC_HIP(hipStreamSynchronize(st));
-    iac_pair_plan *pl = find_plan(ctx, R.d_kmap, R.npix, cell_vox, box_cap, ctx->task_region, S);
+    iac_pair_plan *pl = find_plan(ctx, R.d_kmap, R.npix, cell_vox, box_cap, region, S);
     if (pl && (std::memcmp(&pl->g, &g, sizeof(g)) != 0 || pl->cell_ext != cext)) pl = nullptr;
     if (!pl) {
         // reuse the oldest slot beyond four cached plans
@@ -1162,7 +1174,7 @@ int get_plan(iac_ctx *ctx, hipStream_t st, const PairRun &R, int cell_vox, int b
         pl->npix = R.npix;
         pl->cell_vox = cell_vox;
         pl->box_cap = box_cap;
-        pl->region = ctx->task_region;
+        pl->region = region;
         pl->spec = S;
         pl->g = g;
         pl->cell_ext.swap(cext);
@@ -1191,7 +1203,9 @@ std::vector<Chunk> make_chunks(const std::vector<uint64_t> &off, const uint32_t 
         const double n = (double)(off[f + 1] - off[f]);
         total += n * n;
     }
-    const double budget = std::max(1.0, total / std::max(1u, nframes) * fchunk);
+    // fchunk mean frames of pair work, at most 2^28 pairs: dense frames (C(q) images hold
+    // P^2 pairs) become chunks of one frame, so a run still spreads over many blocks
+    const double budget = std::max(1.0, std::min(total / std::max(1u, nframes) * fchunk, 268435456.0));
     const uint32_t fmax = std::min<uint32_t>(4 * fchunk, 4096);  // items per chunk < 2^24 (pairs_kernel)
     std::vector<Chunk> chunks;
     for (uint32_t gi = 0; gi < ngroups; gi++) {
@@ -1321,21 +1335,29 @@ int enqueue(iac_ctx *ctx, hipStream_t st, const PairRun &R, const Bufs &b, bool 
     return IAC_OK;
 }
 
-// defaults from the tuner on the 1024^2 sparse and lognormal workloads: cells of 6 voxels
-// in 1920-voxel boxes on the specialised path (lanes on consecutive pairs), cells of 4
-// voxels for the other AC_uw modes (a pair range per thread), and dense C(q) cells of
-// 4 voxels in 4096-voxel boxes
+// defaults from the tuner: on the 1024^2 sparse and lognormal AC_uw workloads cells of 6
+// voxels in 1920-voxel boxes on the specialised path (lanes on consecutive pairs), cells
+// of 4 for the other AC_uw modes (a pair range per thread); on dense 256^2 C(q) images
+// (nearest neighbour) cells of 3 in 1920-voxel boxes, 4 x 4-cell task regions; linear
+// C(q) keeps cells of 4 in 4096-voxel boxes
 int cell_vox_of(iac_ctx *ctx, const PairRun &R)
 {
-    const bool walk = fast_path(R.spec, R.ordered) && ctx->pair_mode != 2;
-    const int u = ctx->cell_vox > 0 ? ctx->cell_vox : (walk ? 6 : 4);
+    const bool cq = R.spec.kind == kKindCq;
+    const bool walk = (fast_path(R.spec, R.ordered) || (cq && !R.ordered)) && ctx->pair_mode != 2;
+    const int u = ctx->cell_vox > 0 ? ctx->cell_vox : (walk ? (cq ? 3 : 6) : 4);
     return u + 1000 * (ctx->cell_vox_v > 0 ? ctx->cell_vox_v : u);
 }
-int box_cap_of(iac_ctx *ctx, const PairSpec &S)
+int box_cap_of(iac_ctx *ctx, const PairRun &R)
 {
     if (ctx->box_cap == kBoxMax || ctx->box_cap == kBoxTiny) return ctx->box_cap;
     if (ctx->box_cap == kBoxSmall || ctx->box_cap == 2048) return kBoxSmall;
-    return S.kind == kKindAc3d ? kBoxSmall : kBoxMax;
+    return R.spec.kind == kKindCq && R.ordered ? kBoxMax : kBoxSmall;
+}
+// dense C(q) images: task regions of 4 x 4 cells (more, smaller blocks); AC_uw: whole grid
+int region_of(iac_ctx *ctx, const PairRun &R)
+{
+    if (ctx->task_region > 0) return ctx->task_region;
+    return R.spec.kind == kKindCq && !R.ordered ? 4 : 0;
 }
 uint32_t fchunk_of(iac_ctx *ctx) { return ctx->frame_chunk > 0 ? ctx->frame_chunk : 32; }
 
@@ -1371,7 +1393,7 @@ int pairs_run(iac_ctx *ctx, const PairRun &R)
     hipStream_t st = ctx->stream;
     if (R.n_total >= (1ull << 32)) return fail(IAC_ERR_LIMIT, "pair engine: at most 2^32 - 1 records per call");
     iac_pair_plan *pl = nullptr;
-    IAC_TRY(get_plan(ctx, st, R, cell_vox_of(ctx, R), box_cap_of(ctx, R.spec), &pl));
+    IAC_TRY(get_plan(ctx, st, R, cell_vox_of(ctx, R), box_cap_of(ctx, R), region_of(ctx, R), &pl));
     std::vector<uint64_t> off(R.nframes + 1);
     IAC_HIP(hipMemcpyAsync(off.data(), R.d_frame_off, off.size() * 8, hipMemcpyDeviceToHost, st));
     IAC_HIP(hipStreamSynchronize(st));
@@ -1419,7 +1441,7 @@ int pairs_prepare(iac_ctx *ctx, const PairRun &R0, PairBatch **out)
     hipStream_t st = ctx->stream;
     if (R0.n_total >= (1ull << 32)) return fail(IAC_ERR_LIMIT, "pair engine: at most 2^32 - 1 records per call");
     iac_pair_plan *pl = nullptr;
-    IAC_TRY(get_plan(ctx, st, R0, cell_vox_of(ctx, R0), box_cap_of(ctx, R0.spec), &pl));
+    IAC_TRY(get_plan(ctx, st, R0, cell_vox_of(ctx, R0), box_cap_of(ctx, R0), region_of(ctx, R0), &pl));
     std::vector<uint64_t> off(R0.nframes + 1);
     IAC_HIP(hipMemcpyAsync(off.data(), R0.d_frame_off, off.size() * 8, hipMemcpyDeviceToHost, st));
     IAC_HIP(hipStreamSynchronize(st));

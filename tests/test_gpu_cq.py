@@ -156,3 +156,34 @@ def test_dense_float_rotation_merge_vs_fp64(ctx, dev):
     rel = np.abs(got[pop] - ref[pop]) / ref[pop]
     assert rel.max() <= 1e-6, rel.max()
     assert abs(got.sum() - ref.sum()) <= 1e-6 * ref.sum()
+
+
+@pytest.mark.parametrize("mode,cell,box", [(1, 4, 4096), (1, 8, 4096), (1, 6, 1920), (2, 4, 4096), (2, 6, 1920)])
+def test_cq_dense_pair_modes(ctx, dev, mode, cell, box):
+    """Nearest-neighbour C(q) of dense frames gives the oracle's bits with lanes on
+    consecutive pairs and with a pair range per thread, at several cell sizes and
+    box classes (a checked box: pairs outside it go to global memory)."""
+    import incohautocorrelate_amd as iac
+
+    n = 96
+    _, km, mq = W.detector(n)
+    s = iac.mesh_shape_small_cofq(mq, 47)
+    frames = W.gamma_frames(n * n, 3, 1.0, 0.5, seed=21)
+    frames[1, ::7] = 0.0  # zero pixels are dropped by the cell sort
+    goff = np.array([0, 3], np.uint32)
+    mult = iac.cq_small_multiplicator(frames.mean(axis=0))
+    ref = np.zeros(s.size_ab ** 2 * s.size_c, np.uint64)
+    for f in range(3):
+        ref, _ = O.cq_small(frames[f], km, s, 0, mult, mesh=ref)
+    ctx.set_option(iac.OPT_PAIR_MODE, mode)
+    ctx.set_option(iac.OPT_CELL_VOX, cell)
+    ctx.set_option(iac.OPT_BOX_CAP, box)
+    try:
+        mesh = _zeros(s.size_ab ** 2 * s.size_c, dev)
+        ctx.cq_dense_frames(to_dev(km, dev), to_dev(frames, dev), goff, s, mesh, 0, mult)
+        got = mesh_np(mesh)
+    finally:
+        ctx.set_option(iac.OPT_PAIR_MODE, 0)
+        ctx.set_option(iac.OPT_CELL_VOX, 0)
+        ctx.set_option(iac.OPT_BOX_CAP, 0)
+    assert np.array_equal(got, ref)
