@@ -166,21 +166,29 @@ __global__ void __launch_bounds__(kBlock)
                      int interp, float self_factor, unsigned long long *__restrict__ hist_u,
                      double *__restrict__ hist_d, unsigned long long *__restrict__ dropped)
 {
-    __shared__ unsigned long long hu[kExact ? kMaxBins : 1];
-    __shared__ double hd[kExact ? 1 : kMaxBins];
-    __shared__ float4 sb[kTile];
+    // dynamic LDS: the block's histogram (size bins of 8 bytes), then the staged b tile
+    extern __shared__ float4 smem[];
+    unsigned long long *hu = reinterpret_cast<unsigned long long *>(smem);
+    double *hd = reinterpret_cast<double *>(smem);
+    float4 *sb = smem + (size + 1) / 2;
     for (uint32_t i = threadIdx.x; i < size; i += kBlock) {
         if (kExact) hu[i] = 0ull; else hd[i] = 0.0;
     }
     unsigned long long drop = 0;
     const uint64_t total = tile_off[nframes];
     const int lane_a = threadIdx.x & (kTile - 1), bgrp = threadIdx.x / kTile;
-    for (uint64_t t = blockIdx.x; t < total; t += gridDim.x) {
-        uint32_t lo = 0, hi = nframes;
+    // a contiguous range of tiles per block: the frame is searched once, then advanced
+    const uint64_t t0 = total * blockIdx.x / gridDim.x, t1 = total * (blockIdx.x + 1) / gridDim.x;
+    uint32_t lo = 0;
+    {
+        uint32_t hi = nframes;  // the frame of t0: largest f with tile_off[f] <= t0
         while (hi - lo > 1) {
             const uint32_t mid = (lo + hi) >> 1;
-            if (tile_off[mid] <= t) lo = mid; else hi = mid;
+            if (tile_off[mid] <= t0) lo = mid; else hi = mid;
         }
+    }
+    for (uint64_t t = t0; t < t1; t++) {
+        while (tile_off[lo + 1] <= t) lo++;  // frames without tiles are skipped
         const uint64_t base = frame_off[lo];
         const uint32_t n = (uint32_t)(frame_off[lo + 1] - base);
         const uint32_t nt = (n + kTile - 1) / kTile;
@@ -256,7 +264,9 @@ __global__ void calc_ac_kernel(const double *__restrict__ ac_uw, const double *_
     }
 }
 
-unsigned persistent_blocks(iac_ctx *ctx) { return (unsigned)std::max(1, ctx->num_cus * 2); }
+unsigned persistent_blocks(iac_ctx *ctx) { return (unsigned)std::max(1, ctx->num_cus * 4); }
+// dynamic LDS of the AC_uw kernel: the histogram, then one staged tile of records
+size_t acuw_lds(uint32_t size) { return (size_t)(size + 1) / 2 * sizeof(float4) + kTile * sizeof(float4); }
 
 }  // namespace
 
@@ -281,10 +291,12 @@ int acuw_run(iac_ctx *ctx, const float4 *rec, const uint64_t *d_frame_off, uint3
     IAC_TRY(iac::timer_begin(ctx));
     if (exact) {
         hipLaunchKernelGGL(zero_u64_kernel, dim3(1), dim3(256), 0, ctx->stream, hu, size);
-        hipLaunchKernelGGL(ac1d_acuw_kernel<true>, dim3(persistent_blocks(ctx)), dim3(kBlock), 0, ctx->stream, rec,
+        hipLaunchKernelGGL(ac1d_acuw_kernel<true>, dim3(persistent_blocks(ctx)), dim3(kBlock), acuw_lds(size),
+                           ctx->stream, rec,
                            d_frame_off, toff, nframes, size, max_q, dq, interp, sf, hu, d_hist, ctx->d_counters);
     } else {
-        hipLaunchKernelGGL(ac1d_acuw_kernel<false>, dim3(persistent_blocks(ctx)), dim3(kBlock), 0, ctx->stream, rec,
+        hipLaunchKernelGGL(ac1d_acuw_kernel<false>, dim3(persistent_blocks(ctx)), dim3(kBlock), acuw_lds(size),
+                           ctx->stream, rec,
                            d_frame_off, toff, nframes, size, max_q, dq, interp, sf, hu, d_hist, ctx->d_counters);
     }
     IAC_HIP(hipGetLastError());
