@@ -212,34 +212,44 @@ __global__ void __launch_bounds__(256)
     if (live) out[(size_t)f * npix + p] = result;
 }
 
-// SimulatePart's post-processing (Simulator.cpp:868-888), one block per frame:
-// the frame sum in double over 1024-pixel chunks in order, then over the chunks;
-// x (float)(n_expected / sum); Poisson draw from stream 0; white noise
+// SimulatePart's post-processing (Simulator.cpp:868-888): the frame sum in double
+// over 1024-pixel chunks in order (one thread per chunk), then over the chunks in
+// order; x (float)(n_expected / sum); Poisson draw from stream 0; white noise
 // (2u - 1) * adu_noise in double from stream 1; x value_per_photon in float.
 __global__ void __launch_bounds__(256)
-    postprocess_kernel(float *__restrict__ frames, uint32_t npix, double n_expected, int do_poisson, float adu_noise,
-                       float value_per_photon, uint64_t seed, uint32_t frame0)
+    chunk_sums_kernel(const float *__restrict__ frames, uint32_t npix, uint32_t nch, double *__restrict__ part)
 {
-    __shared__ double part[4096];
-    __shared__ float fac_s;
-    const uint32_t f = blockIdx.x;
+    const uint32_t f = blockIdx.y, c = blockIdx.x * blockDim.x + threadIdx.x;
+    if (c >= nch) return;
+    const float *I = frames + (size_t)f * npix;
+    double s = 0.0;
+    const uint32_t e = min(npix, (c + 1) * 1024);
+    for (uint32_t l = c * 1024; l < e; l++) s += I[l];
+    part[(size_t)f * nch + c] = s;
+}
+
+// the frame's factor n_expected / (sum of its chunk sums, in order)
+__global__ void frame_factor_kernel(const double *__restrict__ part, uint32_t nch, uint32_t nframes,
+                                    double n_expected, float *__restrict__ fac)
+{
+    const uint32_t f = blockIdx.x * blockDim.x + threadIdx.x;
+    if (f >= nframes) return;
+    double integ = 0.0;
+    for (uint32_t c = 0; c < nch; c++) integ += part[(size_t)f * nch + c];
+    fac[f] = (float)(n_expected / integ);
+}
+
+constexpr uint32_t kPostSlice = 4096;  // pixels per post-processing block
+
+__global__ void __launch_bounds__(256)
+    postprocess_kernel(float *__restrict__ frames, uint32_t npix, const float *__restrict__ fac_of, int do_poisson,
+                       float adu_noise, float value_per_photon, uint64_t seed, uint32_t frame0)
+{
+    const uint32_t f = blockIdx.y;
+    const float fac = fac_of[f];
     float *I = frames + (size_t)f * npix;
-    const uint32_t nch = (npix + 1023) / 1024;
-    for (uint32_t c = threadIdx.x; c < nch; c += blockDim.x) {
-        double s = 0.0;
-        const uint32_t e = min(npix, (c + 1) * 1024);
-        for (uint32_t l = c * 1024; l < e; l++) s += I[l];
-        part[c] = s;
-    }
-    __syncthreads();
-    if (threadIdx.x == 0) {
-        double integ = 0.0;
-        for (uint32_t c = 0; c < nch; c++) integ += part[c];
-        fac_s = (float)(n_expected / integ);
-    }
-    __syncthreads();
-    const float fac = fac_s;
-    for (uint32_t l = threadIdx.x; l < npix; l += blockDim.x) {
+    const uint32_t l1 = min(npix, (blockIdx.x + 1) * kPostSlice);
+    for (uint32_t l = blockIdx.x * kPostSlice + threadIdx.x; l < l1; l += blockDim.x) {
         float v = I[l] * fac;
         double u[2];
         if (do_poisson) {
@@ -455,8 +465,20 @@ int iac_sim_postprocess(iac_ctx *ctx, float *d_frames, uint32_t npix, uint32_t n
     if (!d_frames || npix == 0) return iac::fail(IAC_ERR_INVALID_ARG, "bad post-processing request");
     if (npix > 4096u * 1024u) return iac::fail(IAC_ERR_LIMIT, "at most 4 Mi pixels per frame");
     if (nframes == 0) return IAC_OK;
-    hipLaunchKernelGGL(postprocess_kernel, dim3(nframes), dim3(256), 0, ctx->stream, d_frames, npix, n_expected,
-                       do_poisson, adu_noise, value_per_photon, seed, frame0);
+    if (nframes > 65535) return iac::fail(IAC_ERR_LIMIT, "at most 65535 frames per call");
+    // chunk sums (1024 pixels each, in order), the frame factor over them in order, then
+    // rescale + Poisson + noise over 4096-pixel slices
+    const uint32_t nch = (npix + 1023) / 1024;
+    void *sp = nullptr;
+    IAC_TRY(iac::scratch(ctx, 2, (size_t)nframes * nch * sizeof(double) + (size_t)nframes * sizeof(float), &sp));
+    double *part = static_cast<double *>(sp);
+    float *fac = reinterpret_cast<float *>(part + (size_t)nframes * nch);
+    hipLaunchKernelGGL(chunk_sums_kernel, dim3((nch + 255) / 256, nframes), dim3(256), 0, ctx->stream, d_frames,
+                       npix, nch, part);
+    hipLaunchKernelGGL(frame_factor_kernel, dim3((nframes + 255) / 256), dim3(256), 0, ctx->stream, part, nch,
+                       nframes, n_expected, fac);
+    hipLaunchKernelGGL(postprocess_kernel, dim3((npix + kPostSlice - 1) / kPostSlice, nframes), dim3(256), 0,
+                       ctx->stream, d_frames, npix, fac, do_poisson, adu_noise, value_per_photon, seed, frame0);
     IAC_HIP(hipGetLastError());
     return IAC_OK;
 }
