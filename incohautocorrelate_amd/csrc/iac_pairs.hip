@@ -10,13 +10,19 @@
 //   2. cell_sort: per frame, counting sort of its records by cell, dropping
 //      zero-valued records (they add nothing in any mode); in symmetric mode
 //      the frame's self-pair value is summed for its group.
-//   3. pairs: one block per (delta, chunk of frames of one group).  The block
-//      privatises delta's voxel box in LDS across the chunk and bins
+//   3. pairs: one block per (delta [, region of cells], chunk of frames of one
+//      group).  The block privatises delta's voxel box in LDS across the chunk
+//      and bins
 //        symmetric modes: each unordered pair of distinct records once
 //          (deltas of the positive half plane; within-cell pairs for 0),
 //        ordered modes: every ordered pair (all deltas; self pairs in 0),
 //      flushing the box with one atomic per nonzero voxel.  A pair outside
-//      the box (the box is only a bound) adds to global memory directly.
+//      the box adds to global memory directly, except on in-box tasks of the
+//      AC_uw specialised path, whose exact box provably holds every pair.
+//      Work inside a block goes by groups of (frame, cell) items: the items'
+//      pair counts are scanned, then symmetric nearest-neighbour runs walk the
+//      group's pairs with the lanes of a wave on consecutive pairs (coalesced
+//      record loads), other runs give each thread a contiguous pair range.
 //   4. symmetric modes: mesh[v] += H[v] + H[mirror v], centre += self.
 // Nearest-neighbour binning is odd in q (iac_binning.h): the (b,a) pair lands
 // on the point reflection of (a,b) with the same value, so step 4 restores
