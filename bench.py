@@ -69,6 +69,7 @@ def parse():
     ap.add_argument("--no-uneven", action="store_true", help="skip the uneven-batch leg")
     ap.add_argument("--no-ac1d", action="store_true", help="skip the AC1D leg")
     ap.add_argument("--no-cq", action="store_true", help="skip the dense C(q) leg")
+    ap.add_argument("--no-eval", action="store_true", help="skip the end-to-end iac_eval leg")
     ap.add_argument("--no-probes", action="store_true", help="skip the copy / atomic ceiling probes")
     ap.add_argument("--uneven-only", action="store_true", help="run only the uneven-batch leg")
     ap.add_argument("--sim-only", action="store_true", help="run only the simulator leg (prints its JSON object)")
@@ -172,6 +173,7 @@ def main():
     del d_pix, d_cnt, d_off, d_mesh
     torch.cuda.empty_cache()
     cq = None if args.no_cq else cq_dense_leg(args, iac, W, dev)
+    ev = None if args.no_eval else eval_leg(args, W)
     probes = None if args.no_probes else probe_leg(iac, dev, nvox)
     uneven = None if args.no_uneven else uneven_leg(args, iac, W, dev)
     sim = None if args.no_sim else sim_leg(args, iac, W, dev)
@@ -225,6 +227,7 @@ def main():
         "uneven_batch": uneven,
         "ac1d": ac1d,
         "cq_dense": cq,
+        "eval_e2e": ev,
         "simulator": sim,
     }
     print(json.dumps(out), flush=True)
@@ -458,6 +461,48 @@ def cq_dense_leg(args, iac, W, dev):
                         f"{s.size_ab}x{s.size_ab}x{s.size_c} uint64 mesh, nearest neighbour",
             "value": pairs / (ms * 1e-3), "unit": "pixel pairs/s", "ms_per_step": ms, "steps": steps,
             "pairs_per_step": pairs, "tuned": tuned}
+
+
+def eval_leg(args, W):
+    """RunIAC::Run_AutoCorr_DataEval end to end through the C++ host mirror (the
+    iac_eval driver): a raw float32 stack of photon frames is read event by event,
+    averaged, sparsified, C(q) small and its merge computed, the frames binned by
+    batched AC_uw, the final mesh normalised and written."""
+    import shutil
+    import subprocess
+    import tempfile
+
+    exe = ROOT / "incohautocorrelate_amd" / "iac_eval"
+    n, nf, edge = 512, 1000, 255
+    pm, _, _ = W.detector(n)
+    tmp = Path(tempfile.mkdtemp(prefix="iac_eval_bench_"))
+    try:
+        pm.astype(np.float32).tofile(tmp / "pm.bin")
+        rng = np.random.default_rng(args.seed + 11)
+        with open(tmp / "frames.bin", "wb") as fh:  # photons of 6.4 ADU with 0.8 ADU noise, Poisson(0.01)
+            for f0 in range(0, nf, 100):
+                ph = rng.poisson(0.01, (min(100, nf - f0), n * n))
+                fr = ph * 6.4 + rng.normal(0.0, 0.8, ph.shape) * (ph > 0)
+                fh.write(fr.astype(np.float32).tobytes())
+        cmd = [str(exe), "--pixelmap", str(tmp / "pm.bin"), "--size", str(n), str(n), "--frames",
+               str(tmp / "frames.bin"), "--nframes", str(nf), "--mesh-size", str(edge), "--photon-offset", "3.2",
+               "--photon-step", "6.4", "--out-final", str(tmp / "final.bin"), "--out-acuw", str(tmp / "acuw.bin"),
+               "--out-cq", str(tmp / "cq.bin")]
+        t = time.perf_counter()
+        r = subprocess.run(cmd, capture_output=True, text=True, timeout=600)
+        dt = time.perf_counter() - t
+        if r.returncode != 0:
+            raise RuntimeError("iac_eval failed: " + r.stderr[-2000:])
+        report = json.loads(r.stdout.strip().splitlines()[-1])
+    finally:
+        shutil.rmtree(tmp, ignore_errors=True)
+    log(f"[eval] {nf} frames of {n}^2 end to end in {dt:.2f} s ({nf / dt:.1f} frames/s), report {report}")
+    return {"workload": f"Run_AutoCorr_DataEval (3D) through iac_eval: {nf} float32 frames of {n}x{n} "
+                        f"(Poisson(0.01) photons of 6.4 ADU + 0.8 ADU noise) from a raw stack, identity "
+                        f"orientations, {edge + 2}^3 mesh, averaged intensity, C(q) small + merge, AC_uw, "
+                        f"normalisation, three raw float64 outputs",
+            "value": nf / dt, "unit": "frames/s (process wall time, file reads and writes included)",
+            "seconds": dt, "report": report}
 
 
 def probe_leg(iac, dev, nvox):
