@@ -86,18 +86,18 @@ int iac_ctx_enable_timing(iac_ctx *ctx, int enable);
 int iac_ctx_timing(iac_ctx *ctx, double *ms, uint64_t *launches, int reset);
 /* Context options (tuning and kernel selection for measurement and tests). */
 #define IAC_OPT_AC3D_KERNEL 1 /* 0 automatic, 1 global-atomic tile kernel, 2 LDS cell-box kernel */
-#define IAC_OPT_CELL_VOX 2    /* LDS kernel: cell side (along the widest k axis) in voxels, 0 = default (6 on the
-                                 AC_uw nearest-neighbour identity-rotation walk, 3 for nearest-neighbour C(q),
-                                 else 4) */
+#define IAC_OPT_CELL_VOX 2    /* LDS kernel: cell side (along the widest k axis) in voxels, 0 = default (6 for
+                                 AC_uw nearest neighbour, 5 for AC_uw linear, 3 for C(q) nearest neighbour, 4
+                                 for C(q) linear) */
 #define IAC_OPT_FRAME_CHUNK 3 /* LDS kernel: mean frames per block, at most 2^28 pairs of work, 0 = default (32) */
 #define IAC_OPT_PAIR_PREFETCH 5 /* LDS kernel: record pairs loaded per batch, 1, 4 or 8 (0 = 1) */
 #define IAC_OPT_BOX_CAP 6 /* LDS kernel: box capacity in voxels, 1024, 1920 (2048 selects it too) or 4096
-                               (0 = 1920; 4096 for linear C(q)) */
+                               (0 = 1920 for nearest neighbour, 4096 for linear) */
 #define IAC_OPT_CELL_VOX_V 7 /* LDS kernel: cell side along the second k axis, 0 = as IAC_OPT_CELL_VOX */
-#define IAC_OPT_TASK_REGION 8 /* LDS kernel: first cells per task region side, 0 = default (4 for nearest-neighbour
-                                   C(q), else the whole grid) */
-#define IAC_OPT_PAIR_MODE 9 /* LDS kernel, AC_uw nearest-neighbour identity-rotation path: 0 default (1), 1 lanes
-                                on consecutive pairs, 2 a contiguous pair range per thread */
+#define IAC_OPT_TASK_REGION 8 /* LDS kernel: first cells per task region side, 0 = default (4 for C(q), else the
+                                   whole grid) */
+#define IAC_OPT_PAIR_MODE 9 /* LDS kernel: 0 default (1, except 2 for linear C(q)), 1 lanes on consecutive pairs,
+                                2 a contiguous pair range per thread */
 #define IAC_OPT_SIM_CHUNK_ROWS 4 /* simulator: detector rows (pixel x subpixel) per GEMM, 0 = auto (1 GiB of FP32 field) */
 int iac_ctx_set_option(iac_ctx *ctx, int option, int64_t value);
 /* Updates whose voxel fell outside the mesh since the context was created.

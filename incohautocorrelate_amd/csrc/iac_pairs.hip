@@ -542,7 +542,9 @@ __device__ __forceinline__ void tri_decode(uint32_t r, uint32_t n, uint32_t &ia,
 // a volatile LDS byte: cross-lane flags stay LDS accesses, in program order
 typedef volatile __attribute__((address_space(3))) uint8_t lds_flag_t;
 
-template <int KIND, bool SMALL, bool INBOX>
+// The binning: FASTBIN, the AC_uw specialised inline path (SMALL and INBOX apply
+// there); otherwise bin_ordered for ordered runs and bin_sym for symmetric ones.
+template <int KIND, bool ORD, bool FASTBIN, bool SMALL, bool INBOX>
 __device__ __forceinline__ void fast_walk(const float4 *__restrict__ src, const uint4 *item, const uint32_t *pre,
                                           uint32_t nnz, uint32_t w0, uint32_t w1, bool tri, bool big,
                                           const PairSpec &S, unsigned long long *box, const Task &T, BoxSink &sink,
@@ -585,13 +587,24 @@ __device__ __forceinline__ void fast_walk(const float4 *__restrict__ src, const 
             ib = r - ia * d.w;
         }
         ub = __builtin_amdgcn_readlane(u, 63);
-        if (active && KIND == kKindCq) {
+        if (active && ORD) {
+            // ordered runs (linear second level): every corner through the checked box
+            const float4 ca = src[d.x + ia];
+            const float4 cb = src[d.z + ib];
+            bin_ordered<KIND>(S, ca, cb, sink);
+        } else if (active && KIND == kKindCq) {
             // C(q): its own value and axis order, through the checked box
             const float4 ca = src[d.x + ia];
             const float4 cb = src[d.z + ib];
             int fs, ms, ss;
             uint64_t v;
             if (bin_sym<kKindCq>(S, ca, cb, fs, ms, ss, v)) sink.add(fs, ms, ss, v);
+        } else if (active && !FASTBIN) {
+            const float4 ca = src[d.x + ia];
+            const float4 cb = src[d.z + ib];
+            int fs, ms, ss;
+            uint64_t v;
+            if (bin_sym<KIND>(S, ca, cb, fs, ms, ss, v)) sink.add(fs, ms, ss, v);
         } else if (active) {
             const float4 ca = src[d.x + ia];
             const float4 cb = src[d.z + ib];
@@ -619,20 +632,20 @@ __device__ __forceinline__ void fast_walk(const float4 *__restrict__ src, const 
     }
 }
 
-template <int KIND>
+template <int KIND, bool ORD, bool FAST>
 __device__ __forceinline__ void fast_walk_w(bool small, bool inbox, const float4 *__restrict__ src, const uint4 *item,
                                             const uint32_t *pre, uint32_t nnz, uint32_t w0, uint32_t w1, bool tri,
                                             bool big, const PairSpec &S, unsigned long long *box, const Task &T,
                                             BoxSink &sink, lds_flag_t *fl)
 {
-    if (KIND == kKindCq) {
-        fast_walk<kKindCq, false, false>(src, item, pre, nnz, w0, w1, tri, big, S, box, T, sink, fl);
+    if (!FAST) {
+        fast_walk<KIND, ORD, false, false, false>(src, item, pre, nnz, w0, w1, tri, big, S, box, T, sink, fl);
     } else if (inbox) {
-        if (small) fast_walk<KIND, true, true>(src, item, pre, nnz, w0, w1, tri, big, S, box, T, sink, fl);
-        else fast_walk<KIND, false, true>(src, item, pre, nnz, w0, w1, tri, big, S, box, T, sink, fl);
+        if (small) fast_walk<KIND, ORD, true, true, true>(src, item, pre, nnz, w0, w1, tri, big, S, box, T, sink, fl);
+        else fast_walk<KIND, ORD, true, false, true>(src, item, pre, nnz, w0, w1, tri, big, S, box, T, sink, fl);
     } else {
-        if (small) fast_walk<KIND, true, false>(src, item, pre, nnz, w0, w1, tri, big, S, box, T, sink, fl);
-        else fast_walk<KIND, false, false>(src, item, pre, nnz, w0, w1, tri, big, S, box, T, sink, fl);
+        if (small) fast_walk<KIND, ORD, true, true, false>(src, item, pre, nnz, w0, w1, tri, big, S, box, T, sink, fl);
+        else fast_walk<KIND, ORD, true, false, false>(src, item, pre, nnz, w0, w1, tri, big, S, box, T, sink, fl);
     }
 }
 
@@ -676,8 +689,8 @@ __global__ void __launch_bounds__(kBlock, (Occupancy<FAST, BOX>::blocks))
     const uint32_t nitems = (CH.f1 - CH.f0) * cells;
     const float inv_cells = 1.0f / (float)cells, inv_nu = 1.0f / (float)T.nu;
     const uint32_t lane = threadIdx.x & 63u, wave = threadIdx.x >> 6;
-    // lanes on consecutive pairs: the AC_uw specialised path and symmetric C(q)
-    const bool walking = (FAST || (KIND == kKindCq && !ORD)) && PF == 1 && walk;
+    // lanes on consecutive pairs (every mode unless a pair range per thread is asked for)
+    const bool walking = PF == 1 && walk;
     for (uint32_t ub = 0; ub < nitems; ub += kGroup) {
         // phase 1: item descriptors and pair counts (contiguous items per thread)
         uint32_t mycnt[kItemsPerThread];
@@ -770,7 +783,8 @@ __global__ void __launch_bounds__(kBlock, (Occupancy<FAST, BOX>::blocks))
             const uint32_t w0 = (uint32_t)(((uint64_t)total * wave) / (kBlock / 64));
             const uint32_t w1 = (uint32_t)(((uint64_t)total * (wave + 1)) / (kBlock / 64));
             if (w0 < w1)
-                fast_walk_w<KIND>(small, inbox, srec, item, pre, s_nnz, w0, w1, tri, total >= (1u << 24), S, box, T, sink,
+                fast_walk_w<KIND, ORD, FAST>(small, inbox, srec, item, pre, s_nnz, w0, w1, tri, total >= (1u << 24), S,
+                                             box, T, sink,
                             (lds_flag_t *)wflags[wave]);
             __syncthreads();
             continue;
@@ -1264,14 +1278,12 @@ void launch_pairs_box(int pf, bool fast, int walk, dim3 grid, hipStream_t st, co
                            R.d_frame_off, b.cs, b.d_chunks, b.d_tasks, b.ntasks, b.g, R.spec, target, nvox, dropped, b.d_flag, walk);
 }
 
-// mode: 0 or 1 lanes on consecutive pairs (the specialised path), 2 a contiguous
-// pair range per thread
+// walk: lanes on consecutive pairs (1) or a contiguous pair range per thread (0)
 template <int K, bool O>
-void launch_pairs(int pf, int mode, dim3 grid, hipStream_t st, const PairRun &R, const Bufs &b,
+void launch_pairs(int pf, int walk, dim3 grid, hipStream_t st, const PairRun &R, const Bufs &b,
                   unsigned long long *target, size_t nvox, unsigned long long *dropped)
 {
     const bool fast = fast_path(R.spec, O);
-    const int walk = mode == 2 ? 0 : 1;
     if (b.box_cap == kBoxSmall)
         launch_pairs_box<K, O, kBoxSmall>(pf, fast, walk, grid, st, R, b, target, nvox, dropped);
     else if (b.box_cap == kBoxTiny)
@@ -1282,6 +1294,8 @@ void launch_pairs(int pf, int mode, dim3 grid, hipStream_t st, const PairRun &R,
 
 // Everything after planning, enqueue only (no host synchronisation, no
 // allocation, fills by kernel): capturable into a graph.
+bool walks(iac_ctx *ctx, const PairRun &R);
+
 int enqueue(iac_ctx *ctx, hipStream_t st, const PairRun &R, const Bufs &b, bool timed)
 {
     const PairSpec &S = R.spec;
@@ -1315,18 +1329,18 @@ int enqueue(iac_ctx *ctx, hipStream_t st, const PairRun &R, const Bufs &b, bool 
         // ordered (linear) binning holds eight corner adds per pair in registers: no batching there
         const int pf = ordered || ctx->pair_prefetch == 0 || ctx->pair_prefetch == 1 ? 1 : (ctx->pair_prefetch == 4 ? 4 : 8);
         // block = chunk * ntasks + task; whole chunks per launch, the slice's chunks by pointer
-        const int mode = ctx->pair_mode;
+        const int walk = walks(ctx, R) ? 1 : 0;
         const uint32_t per = std::max<uint32_t>(1, kMaxGridBlocks / b.ntasks);
         for (uint32_t c0 = 0; c0 < b.nchunks; c0 += per) {
             Bufs bs = b;
             bs.d_chunks = b.d_chunks + c0;
             const dim3 grid(b.ntasks * std::min(per, b.nchunks - c0));
             if (S.kind == kKindAc3d) {
-                if (ordered) launch_pairs<kKindAc3d, true>(pf, mode, grid, st, R, bs, target, nvox, ctx->d_counters);
-                else launch_pairs<kKindAc3d, false>(pf, mode, grid, st, R, bs, target, nvox, ctx->d_counters);
+                if (ordered) launch_pairs<kKindAc3d, true>(pf, walk, grid, st, R, bs, target, nvox, ctx->d_counters);
+                else launch_pairs<kKindAc3d, false>(pf, walk, grid, st, R, bs, target, nvox, ctx->d_counters);
             } else {
-                if (ordered) launch_pairs<kKindCq, true>(pf, mode, grid, st, R, bs, target, nvox, ctx->d_counters);
-                else launch_pairs<kKindCq, false>(pf, mode, grid, st, R, bs, target, nvox, ctx->d_counters);
+                if (ordered) launch_pairs<kKindCq, true>(pf, walk, grid, st, R, bs, target, nvox, ctx->d_counters);
+                else launch_pairs<kKindCq, false>(pf, walk, grid, st, R, bs, target, nvox, ctx->d_counters);
             }
             IAC_HIP(hipGetLastError());
         }
@@ -1341,29 +1355,40 @@ int enqueue(iac_ctx *ctx, hipStream_t st, const PairRun &R, const Bufs &b, bool 
     return IAC_OK;
 }
 
-// defaults from the tuner: on the 1024^2 sparse and lognormal AC_uw workloads cells of 6
-// voxels in 1920-voxel boxes on the specialised path (lanes on consecutive pairs), cells
-// of 4 for the other AC_uw modes (a pair range per thread); on dense 256^2 C(q) images
-// (nearest neighbour) cells of 3 in 1920-voxel boxes, 4 x 4-cell task regions; linear
-// C(q) keeps cells of 4 in 4096-voxel boxes
+// Defaults from the tuner (bench.py sweeps and the ordered-mode measurements):
+//   AC_uw nearest neighbour, identity rotation: lanes on consecutive pairs, 6-voxel cells,
+//     1920-voxel boxes (1024^2 sparse and lognormal workloads);
+//   AC_uw linear (ordered): lanes on consecutive pairs, 5-voxel cells, 4096-voxel boxes
+//     (64 LDS adds per pair with a linear first level: LDS-atomic bound);
+//   C(q) nearest neighbour: lanes on consecutive pairs, 3-voxel cells, 1920-voxel boxes,
+//     4 x 4-cell task regions (dense 256^2 images);
+//   C(q) linear: a pair range per thread, 4-voxel cells, 4096-voxel boxes, regions of 4.
+bool walks(iac_ctx *ctx, const PairRun &R)
+{
+    if (ctx->pair_mode == 1) return true;
+    if (ctx->pair_mode == 2) return false;
+    return !(R.spec.kind == kKindCq && R.ordered);
+}
 int cell_vox_of(iac_ctx *ctx, const PairRun &R)
 {
-    const bool cq = R.spec.kind == kKindCq;
-    const bool walk = (fast_path(R.spec, R.ordered) || (cq && !R.ordered)) && ctx->pair_mode != 2;
-    const int u = ctx->cell_vox > 0 ? ctx->cell_vox : (walk ? (cq ? 3 : 6) : 4);
+    int dflt = 4;
+    if (walks(ctx, R)) {
+        if (R.spec.kind == kKindCq) dflt = 3;
+        else dflt = R.ordered ? 5 : (fast_path(R.spec, R.ordered) ? 6 : 4);
+    }
+    const int u = ctx->cell_vox > 0 ? ctx->cell_vox : dflt;
     return u + 1000 * (ctx->cell_vox_v > 0 ? ctx->cell_vox_v : u);
 }
 int box_cap_of(iac_ctx *ctx, const PairRun &R)
 {
     if (ctx->box_cap == kBoxMax || ctx->box_cap == kBoxTiny) return ctx->box_cap;
     if (ctx->box_cap == kBoxSmall || ctx->box_cap == 2048) return kBoxSmall;
-    return R.spec.kind == kKindCq && R.ordered ? kBoxMax : kBoxSmall;
+    return R.ordered ? kBoxMax : kBoxSmall;
 }
-// dense C(q) images: task regions of 4 x 4 cells (more, smaller blocks); AC_uw: whole grid
 int region_of(iac_ctx *ctx, const PairRun &R)
 {
     if (ctx->task_region > 0) return ctx->task_region;
-    return R.spec.kind == kKindCq && !R.ordered ? 4 : 0;
+    return R.spec.kind == kKindCq ? 4 : 0;
 }
 uint32_t fchunk_of(iac_ctx *ctx) { return ctx->frame_chunk > 0 ? ctx->frame_chunk : 32; }
 

@@ -171,3 +171,34 @@ def test_uneven_lognormal_full_size(ctx, dev):
     b6.launch(m6, zero_mesh=True)
     assert np.array_equal(mesh_np(m6), ref)
     b6.destroy()
+
+
+@pytest.mark.parametrize("mode,cell,box", [(1, 5, 4096), (1, 4, 1920), (2, 4, 4096), (1, 3, 1024)])
+def test_linear_modes_pair_walks(ctx, dev, mode, cell, box):
+    """Linear AC_uw (ordered pairs, 8 or 64 corner adds each) through the engine with
+    lanes on consecutive pairs and with a pair range per thread: the oracle's bits."""
+    import torch
+
+    import incohautocorrelate_amd as iac
+
+    n = 96
+    _, km, mq = W.detector(n)
+    shape = iac.mesh_shape_big(mq, 47)
+    data = W.lognormal_photon_frames(n * n, 12, 50, 1.0, 1, 400, seed=31)
+    n_total = int(data["frame_off"][-1])
+    ref = {}
+    for i1, i2 in ((1, 1), (0, 1)):
+        ref[i1, i2], _ = O.ac3d_frames(km, data["pix"], data["cnt"], data["frame_off"], shape, None, i1, i2, True)
+    ctx.set_option(iac.OPT_AC3D_KERNEL, 2)
+    ctx.set_option(iac.OPT_PAIR_MODE, mode)
+    ctx.set_option(iac.OPT_CELL_VOX, cell)
+    ctx.set_option(iac.OPT_BOX_CAP, box)
+    try:
+        pix, cnt, off = _dev_frames(data, dev)
+        for (i1, i2), r in ref.items():
+            m = torch.zeros(shape.size_ab ** 2 * shape.size_c, dtype=torch.int64, device=dev)
+            ctx.ac3d_sparse_batch(to_dev(km, dev), pix, cnt, off, n_total, shape, m, interp1=i1, interp2=i2)
+            assert np.array_equal(mesh_np(m), r), (i1, i2)
+    finally:
+        for o in (iac.OPT_AC3D_KERNEL, iac.OPT_PAIR_MODE, iac.OPT_CELL_VOX, iac.OPT_BOX_CAP):
+            ctx.set_option(o, 0)
