@@ -402,12 +402,38 @@ def ac1d_leg(args, iac, W, dev, d_kmap, d_pix, d_cnt, d_off, data, max_q_vec):
         run()
     torch.cuda.synchronize()
     ms = (time.perf_counter() - t) * 1e3 / steps
-    ctx.close()
     log(f"[ac1d] {ms:.2f} ms/step, {pairs:.4e} ordered pairs, {pairs / (ms * 1e-3):.4e} pairs/s")
-    return {"workload": f"AC1D AC_uw of the headline frames: {len(sizes)} frames, {size} bins, linear split, "
-                        f"self pairs weighted 2",
-            "value": pairs / (ms * 1e-3), "unit": "ordered photon pairs/s", "ms_per_step": ms, "steps": steps,
-            "pairs_per_step": pairs}
+    out = {"workload": f"AC1D AC_uw of the headline frames: {len(sizes)} frames, {size} bins, linear split, "
+                       f"self pairs weighted 2",
+           "value": pairs / (ms * 1e-3), "unit": "ordered photon pairs/s", "ms_per_step": ms, "steps": steps,
+           "pairs_per_step": pairs}
+
+    # AC1D C(q): every unordered pixel pair of one averaged-intensity image of the same detector
+    npix = int(d_kmap.shape[0])
+    img = W.gamma_frames(npix, 1, 1.0, 0.5, seed=args.seed + 11)[0]
+    mult = iac.ac1d_cq_multiplicator(img)
+    d_img = torch.from_numpy(img).to(dev)
+    hq = torch.zeros(size, dtype=torch.int64, device=dev)
+
+    def run_cq():
+        hq.zero_()
+        ctx.ac1d_cq(d_kmap, d_img, size, max_q, dq, 1, mult, hq)
+
+    run_cq()
+    torch.cuda.synchronize()
+    cq_steps = max(1, min(args.steps, 3))
+    t = time.perf_counter()
+    for _ in range(cq_steps):
+        run_cq()
+    torch.cuda.synchronize()
+    cq_ms = (time.perf_counter() - t) * 1e3 / cq_steps
+    ctx.close()
+    cq_pairs = npix * (npix + 1) // 2
+    log(f"[ac1d-cq] {cq_ms:.2f} ms/step, {cq_pairs:.4e} pixel pairs, {cq_pairs / (cq_ms * 1e-3):.4e} pairs/s")
+    out["cq"] = {"workload": f"AC1D C(q) of one {npix}-pixel Gamma(1, 0.5) image: {size} bins, linear split",
+                 "value": cq_pairs / (cq_ms * 1e-3), "unit": "unordered pixel pairs/s", "ms_per_step": cq_ms,
+                 "steps": cq_steps, "pairs_per_step": cq_pairs}
+    return out
 
 
 def cq_dense_leg(args, iac, W, dev):

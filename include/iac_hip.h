@@ -317,6 +317,10 @@ int iac_probe_atomics(iac_ctx *ctx, uint64_t *d_mesh, uint64_t nvox, uint64_t n_
                       float *ms);
 /* A plain 16-byte-vector copy of `bytes` (a multiple of 16); *ms = kernel time. */
 int iac_probe_copy(iac_ctx *ctx, const void *d_src, void *d_dst, size_t bytes, float *ms);
+/* Self-check of the short correctly rounded square root the AC1D kernels use for |q|:
+ * every float bit pattern u in [lo, hi) (hi <= 2^32) against the library sqrtf; *mismatches
+ * = the count of results that differ bit for bit.  Synchronous. */
+int iac_selfcheck_sqrt(iac_ctx *ctx, uint64_t lo, uint64_t hi, uint64_t *mismatches);
 
 #ifdef __cplusplus
 }

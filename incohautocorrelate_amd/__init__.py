@@ -303,6 +303,13 @@ class Context:
         check(lib().iac_probe_copy(self._h, _ptr(src), _ptr(dst), nbytes, C.byref(ms)), "iac_probe_copy")
         return ms.value
 
+    def selfcheck_sqrt(self, lo: int = 0, hi: int = 1 << 32) -> int:
+        """Mismatches of the AC1D kernels' short square root against sqrtf over the
+        float bit patterns [lo, hi)."""
+        n = C.c_uint64()
+        check(lib().iac_selfcheck_sqrt(self._h, int(lo), int(hi), C.byref(n)), "iac_selfcheck_sqrt")
+        return n.value
+
     # ---- simulator (Simulator::SimulatePart, IAC/Simulator.cpp:662-944) ----
     def sim_plan(self, pixmap, emitters, wavelength: float, mode: int = SIM_BF16X3, subsampling: int = 0,
                  u_step=(0.0, 0.0, 0.0), v_step=(0.0, 0.0, 0.0)) -> "SimPlan":

@@ -102,6 +102,7 @@ SIGNATURES: dict[str, tuple] = {
                                   U32, U32, I32, P]),
     "iac_probe_atomics": (I32, [P, P, U64, U64, I32, U64, C.POINTER(C.c_float)]),
     "iac_probe_copy": (I32, [P, P, P, C.c_size_t, C.POINTER(C.c_float)]),
+    "iac_selfcheck_sqrt": (I32, [P, U64, U64, C.POINTER(U64)]),
     "iac_sim_postprocess": (I32, [P, P, U32, U32, C.c_double, I32, C.c_float, C.c_float, C.c_uint64, U32]),
 }
 

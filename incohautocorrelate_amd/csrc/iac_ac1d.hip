@@ -16,7 +16,9 @@
 // for bit (all partial sums are integers below 2^53).
 #include <algorithm>
 #include <cmath>
+#include <numeric>
 
+#include "iac_fpexact.h"
 #include "iac_internal.h"
 
 #pragma clang fp contract(off)
@@ -50,24 +52,72 @@ __device__ __forceinline__ void decode_tri(uint64_t l, uint32_t nt, uint32_t &ta
 // differences, then a correctly rounded float sqrt.
 __device__ __forceinline__ float qnorm(const float4 &a, const float4 &b)
 {
-    return sqrtf((a.x - b.x) * (a.x - b.x) + (a.y - b.y) * (a.y - b.y) + (a.z - b.z) * (a.z - b.z));
+    return iac::sqrt_rn((a.x - b.x) * (a.x - b.x) + (a.y - b.y) * (a.y - b.y) + (a.z - b.z) * (a.z - b.z));
+}
+
+// q / dq rounded to float.  kFast: q * RN(1/dq) in double, then rounded to
+// float.  The exact quotient of two floats is never within 2^-49 (relative) of a
+// float rounding midpoint (a midpoint m has an odd 25-bit significand, so q - m dq
+// is a nonzero multiple of a power of two at least q 2^-49), and the double
+// product errs by at most 2^-52, so both round to the same float.  The host takes
+// this path only while every quotient of a binned pair is 0 or a normal float.
+template <bool kFast>
+__device__ __forceinline__ float div_dq(float q, float dq, double inv_dq)
+{
+    return kFast ? (float)((double)q * inv_dq) : q / dq;
+}
+
+// (unsigned long long)(long long)x for x >= 0 as the reference truncates its
+// weighted values.  Values below 2^32 (the usual case: the multiplicator scales
+// I_i I_j / <I>^2 to about 1e5..1e6) take the one-instruction u32 conversion;
+// a wave with any larger (or NaN) value takes the full 64-bit conversion on a
+// branch of its own (the empty volatile asm keeps it from being if-converted).
+__device__ __forceinline__ unsigned long long trunc_u64(double x)
+{
+    if (__builtin_amdgcn_ballot_w64(!(x < 4294967296.0)) != 0) {
+        asm volatile("");
+        return (unsigned long long)(long long)x;
+    }
+    return (uint32_t)x;
+}
+
+__device__ __forceinline__ void trunc_u64x2(double x1, double x2, unsigned long long &v1, unsigned long long &v2)
+{
+    if (__builtin_amdgcn_ballot_w64(!(x1 < 4294967296.0 && x2 < 4294967296.0)) != 0) {
+        asm volatile("");
+        v1 = (unsigned long long)(long long)x1;
+        v2 = (unsigned long long)(long long)x2;
+    } else {
+        v1 = (uint32_t)x1;
+        v2 = (uint32_t)x2;
+    }
 }
 
 // AutoCorr_CQ_AV over one image: records are {k, I} of every pixel.
+template <bool kFastDiv>
 __global__ void __launch_bounds__(kBlock)
-    ac1d_cq_kernel(const float4 *__restrict__ rec, uint32_t n, uint32_t size, float max_q, float dq, int interp,
-                   double mult, unsigned long long *__restrict__ hist, unsigned long long *__restrict__ dropped)
+    ac1d_cq_kernel(const float4 *__restrict__ rec, uint32_t n, uint32_t size, float max_q, float dq, double inv_dq,
+                   int interp, double mult, unsigned long long *__restrict__ hist,
+                   unsigned long long *__restrict__ dropped)
 {
-    __shared__ unsigned long long h[kMaxBins];
-    __shared__ float4 sb[kTile];
+    // dynamic LDS: the block's histogram (size bins), then the staged b tile
+    extern __shared__ float4 smem[];
+    unsigned long long *h = reinterpret_cast<unsigned long long *>(smem);
+    float4 *sb = smem + (size + 1) / 2;
     for (uint32_t i = threadIdx.x; i < size; i += kBlock) h[i] = 0ull;
     unsigned long long drop = 0;
     const uint64_t total = tri_tiles(n);
     const uint32_t nt = (n + kTile - 1) / kTile;
     const int lane_a = threadIdx.x & (kTile - 1), bgrp = threadIdx.x / kTile;
-    for (uint64_t t = blockIdx.x; t < total; t += gridDim.x) {
-        uint32_t ta, tb;
-        decode_tri(t, nt, ta, tb);
+    // a contiguous range of tiles per block, row-major over the triangle (ta <= tb)
+    const uint64_t t0 = total * blockIdx.x / gridDim.x, t1 = total * (blockIdx.x + 1) / gridDim.x;
+    uint32_t ta = 0, tb = 0;
+    if (t0 < t1) decode_tri(t0, nt, ta, tb);
+    for (uint64_t t = t0; t < t1; t++) {
+        if (t > t0 && ++tb == nt) {
+            ta++;
+            tb = ta;
+        }
         __syncthreads();
         if (threadIdx.x < kTile) {
             const uint32_t b = tb * kTile + threadIdx.x;
@@ -77,29 +127,30 @@ __global__ void __launch_bounds__(kBlock)
         const uint32_t a = ta * kTile + lane_a;
         if (a >= n) continue;
         const float4 ra = rec[a];
+#pragma unroll 4
         for (int jj = 0; jj < kBPerThread; jj++) {
             const int bl = bgrp * kBPerThread + jj;
             const uint32_t b = tb * kTile + bl;
-            if (b >= n) break;
-            if (ta == tb && b < a) continue;
             const float4 rb = sb[bl];
-            const double val = (double)(ra.w * rb.w);
-            if (val < (double)1e-37f) continue;
+            const float w = ra.w * rb.w;
             float q = qnorm(ra, rb);
-            if (q > max_q) continue;
-            q = q / dq;
-            const unsigned long long rep = (a == b) ? 1ull : 2ull;
+            // pairs j >= i of the image with I_i I_j >= 1e-37 and |q| <= Max_Q
+            if (b >= n || (ta == tb && b < a) || w < 1e-37f || q > max_q) continue;
+            const double val = (double)w;
+            q = div_dq<kFastDiv>(q, dq, inv_dq);
+            const unsigned sh = a != b;  // off-diagonal pairs count twice
             if (interp == 0) {
                 const uint32_t sc = (uint32_t)floor((double)q + 0.5);
-                const unsigned long long v = (unsigned long long)(long long)(val * mult);
-                if (sc < size) { if (v) atomicAdd(&h[sc], v * rep); } else drop += rep;
+                const unsigned long long v = trunc_u64(val * mult);
+                if (sc < size) atomicAdd(&h[sc], v << sh); else drop += 1u << sh;
             } else {
                 const uint32_t sc1 = (uint32_t)floorf(q), sc2 = sc1 + 1;
                 const double sep = (double)(q - floorf(q));
-                const unsigned long long v1 = (unsigned long long)(long long)((val * (1 - sep)) * mult);
-                const unsigned long long v2 = (unsigned long long)(long long)((val * sep) * mult);
-                if (sc1 < size) { if (v1) atomicAdd(&h[sc1], v1 * rep); } else drop += rep;
-                if (sc2 < size) { if (v2) atomicAdd(&h[sc2], v2 * rep); } else drop += rep;
+                const double x1 = (val * (1 - sep)) * mult, x2 = (val * sep) * mult;
+                unsigned long long v1, v2;
+                trunc_u64x2(x1, x2, v1, v2);
+                if (sc1 < size) atomicAdd(&h[sc1], v1 << sh); else drop += 1u << sh;
+                if (sc2 < size) atomicAdd(&h[sc2], v2 << sh); else drop += 1u << sh;
             }
         }
     }
@@ -150,20 +201,26 @@ __global__ void gather_kernel(const float *__restrict__ kmap, uint32_t npix, con
     }
 }
 
+// Record i holds pixel (i * stride) mod npix, stride coprime to npix: the 64
+// records of a tile lie far apart on the detector, so the 64 lanes of a wave
+// pairing them with one b pixel bin into different |q| and seldom hit the same
+// histogram word.  The integer histogram does not depend on the pixel order.
 __global__ void gather_image_kernel(const float *__restrict__ kmap, const float *__restrict__ I, uint32_t npix,
-                                    float4 *__restrict__ rec)
+                                    uint32_t stride, float4 *__restrict__ rec)
 {
-    for (uint32_t p = blockIdx.x * blockDim.x + threadIdx.x; p < npix; p += gridDim.x * blockDim.x)
-        rec[p] = make_float4(kmap[3 * (size_t)p], kmap[3 * (size_t)p + 1], kmap[3 * (size_t)p + 2], I[p]);
+    for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < npix; i += gridDim.x * blockDim.x) {
+        const size_t p = (size_t)i * stride % npix;
+        rec[i] = make_float4(kmap[3 * p], kmap[3 * p + 1], kmap[3 * p + 2], I[p]);
+    }
 }
 
 // AC_uw: photon pairs j <= k of every frame.  kExact: integer-valued weights
 // accumulate in uint64 (nearest neighbour); otherwise double.
-template <bool kExact>
+template <bool kExact, bool kFastDiv>
 __global__ void __launch_bounds__(kBlock)
     ac1d_acuw_kernel(const float4 *__restrict__ rec, const uint64_t *__restrict__ frame_off,
                      const uint64_t *__restrict__ tile_off, uint32_t nframes, uint32_t size, float max_q, float dq,
-                     int interp, float self_factor, unsigned long long *__restrict__ hist_u,
+                     double inv_dq, int interp, float self_factor, unsigned long long *__restrict__ hist_u,
                      double *__restrict__ hist_d, unsigned long long *__restrict__ dropped)
 {
     // dynamic LDS: the block's histogram (size bins of 8 bytes), then the staged b tile
@@ -187,13 +244,19 @@ __global__ void __launch_bounds__(kBlock)
             if (tile_off[mid] <= t0) lo = mid; else hi = mid;
         }
     }
+    uint32_t ta = 0, tb = 0, cur = 0xffffffffu;  // tile (ta, tb) of frame cur
     for (uint64_t t = t0; t < t1; t++) {
         while (tile_off[lo + 1] <= t) lo++;  // frames without tiles are skipped
         const uint64_t base = frame_off[lo];
         const uint32_t n = (uint32_t)(frame_off[lo + 1] - base);
         const uint32_t nt = (n + kTile - 1) / kTile;
-        uint32_t ta, tb;
-        decode_tri(t - tile_off[lo], nt, ta, tb);
+        if (lo != cur) {  // first tile of the block in this frame
+            decode_tri(t - tile_off[lo], nt, ta, tb);
+            cur = lo;
+        } else if (++tb == nt) {  // next tile, row-major over the triangle
+            ta++;
+            tb = ta;
+        }
         __syncthreads();
         if (threadIdx.x < kTile) {
             const uint32_t b = tb * kTile + threadIdx.x;
@@ -203,15 +266,14 @@ __global__ void __launch_bounds__(kBlock)
         const uint32_t a = ta * kTile + lane_a;
         if (a >= n) continue;
         const float4 ra = rec[base + a];
+#pragma unroll 4
         for (int jj = 0; jj < kBPerThread; jj++) {
             const int bl = bgrp * kBPerThread + jj;
             const uint32_t b = tb * kTile + bl;
-            if (b >= n) break;
-            if (ta == tb && b < a) continue;
             const float4 rb = sb[bl];
             float q = qnorm(ra, rb);
-            if (q > max_q) continue;
-            q = q / dq;
+            if (b >= n || (ta == tb && b < a) || q > max_q) continue;
+            q = div_dq<kFastDiv>(q, dq, inv_dq);
             const float f2 = (a == b) ? self_factor : 2.0f;
             if (interp == 0) {
                 const uint32_t sc = (uint32_t)floor((double)q + 0.5);
@@ -265,6 +327,14 @@ __global__ void calc_ac_kernel(const double *__restrict__ ac_uw, const double *_
 }
 
 unsigned persistent_blocks(iac_ctx *ctx) { return (unsigned)std::max(1, ctx->num_cus * 4); }
+
+// div_dq<true> is exact while dq is a normal float, every binned |q| <= max_q
+// keeps q / dq <= 2^126, and the smallest nonzero |q| (2^-75, above the square
+// root of the smallest subnormal) keeps q / dq >= 2^-126.
+bool fast_div_ok(float max_q, float dq)
+{
+    return std::isnormal(dq) && dq > 0.f && dq <= 0x1p51f && (double)max_q <= (double)dq * 0x1p126;
+}
 // dynamic LDS of the AC_uw kernel: the histogram, then one staged tile of records
 size_t acuw_lds(uint32_t size) { return (size_t)(size + 1) / 2 * sizeof(float4) + kTile * sizeof(float4); }
 
@@ -288,17 +358,14 @@ int acuw_run(iac_ctx *ctx, const float4 *rec, const uint64_t *d_frame_off, uint3
     hipLaunchKernelGGL(tile_scan_kernel, dim3(1), dim3(1024), 0, ctx->stream, d_frame_off, nframes, toff);
     IAC_HIP(hipGetLastError());
     const float sf = (float)self_pair_factor;
+    const bool fdiv = fast_div_ok(max_q, dq);
+    const double inv_dq = 1.0 / (double)dq;
+    auto kern = exact ? (fdiv ? ac1d_acuw_kernel<true, true> : ac1d_acuw_kernel<true, false>)
+                      : (fdiv ? ac1d_acuw_kernel<false, true> : ac1d_acuw_kernel<false, false>);
     IAC_TRY(iac::timer_begin(ctx));
-    if (exact) {
-        hipLaunchKernelGGL(zero_u64_kernel, dim3(1), dim3(256), 0, ctx->stream, hu, size);
-        hipLaunchKernelGGL(ac1d_acuw_kernel<true>, dim3(persistent_blocks(ctx)), dim3(kBlock), acuw_lds(size),
-                           ctx->stream, rec,
-                           d_frame_off, toff, nframes, size, max_q, dq, interp, sf, hu, d_hist, ctx->d_counters);
-    } else {
-        hipLaunchKernelGGL(ac1d_acuw_kernel<false>, dim3(persistent_blocks(ctx)), dim3(kBlock), acuw_lds(size),
-                           ctx->stream, rec,
-                           d_frame_off, toff, nframes, size, max_q, dq, interp, sf, hu, d_hist, ctx->d_counters);
-    }
+    if (exact) hipLaunchKernelGGL(zero_u64_kernel, dim3(1), dim3(256), 0, ctx->stream, hu, size);
+    hipLaunchKernelGGL(kern, dim3(persistent_blocks(ctx)), dim3(kBlock), acuw_lds(size), ctx->stream, rec,
+                       d_frame_off, toff, nframes, size, max_q, dq, inv_dq, interp, sf, hu, d_hist, ctx->d_counters);
     IAC_HIP(hipGetLastError());
     IAC_TRY(iac::timer_end(ctx));
     if (exact) {
@@ -346,12 +413,15 @@ int iac_ac1d_cq(iac_ctx *ctx, const float *d_kmap, uint32_t npix, const float *d
     if (interp != IAC_NEAREST && interp != IAC_LINEAR) return iac::fail(IAC_ERR_INVALID_ARG, "bad interpolation");
     void *rec = nullptr;
     IAC_TRY(iac::scratch(ctx, 0, (size_t)npix * sizeof(float4), &rec));
+    uint32_t stride = npix / kTile + 1;  // about npix / 64, coprime to npix
+    while (std::gcd(stride, npix) != 1) stride++;
     hipLaunchKernelGGL(gather_image_kernel, dim3(std::min<uint32_t>(4096, (npix + 255) / 256)), dim3(256), 0,
-                       ctx->stream, d_kmap, d_intensity, npix, static_cast<float4 *>(rec));
+                       ctx->stream, d_kmap, d_intensity, npix, stride, static_cast<float4 *>(rec));
     IAC_HIP(hipGetLastError());
     IAC_TRY(iac::timer_begin(ctx));
-    hipLaunchKernelGGL(ac1d_cq_kernel, dim3(persistent_blocks(ctx)), dim3(kBlock), 0, ctx->stream,
-                       static_cast<const float4 *>(rec), npix, size, max_q, dq, interp, mult,
+    auto kern = fast_div_ok(max_q, dq) ? ac1d_cq_kernel<true> : ac1d_cq_kernel<false>;
+    hipLaunchKernelGGL(kern, dim3(persistent_blocks(ctx)), dim3(kBlock), acuw_lds(size), ctx->stream,
+                       static_cast<const float4 *>(rec), npix, size, max_q, dq, 1.0 / (double)dq, interp, mult,
                        reinterpret_cast<unsigned long long *>(d_hist), ctx->d_counters);
     IAC_HIP(hipGetLastError());
     IAC_TRY(iac::timer_end(ctx));

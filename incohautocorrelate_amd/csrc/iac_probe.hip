@@ -2,8 +2,10 @@
 // engine is compared against.  uint64 atomic adds spread uniformly over a mesh,
 // concentrated on a hot region (the occupied centre slab of a sparse run), or
 // into an LDS box (the engine's privatised path), and a plain streaming copy.
+// Also the exhaustive self-check of the short correctly rounded square root.
 #include <algorithm>
 
+#include "iac_fpexact.h"
 #include "iac_internal.h"
 
 namespace {
@@ -49,9 +51,41 @@ __global__ void __launch_bounds__(256) copy_kernel(const uint4 *__restrict__ src
         dst[i] = src[i];
 }
 
+// every float bit pattern in [lo, hi): iac::sqrt_rn against the library sqrtf,
+// bit for bit (NaN results compared as NaN)
+__global__ void __launch_bounds__(256) sqrt_check_kernel(uint64_t lo, uint64_t hi, unsigned long long *bad)
+{
+    unsigned long long nbad = 0;
+    for (uint64_t u = lo + blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; u < hi;
+         u += (uint64_t)gridDim.x * blockDim.x) {
+        const float x = __uint_as_float((uint32_t)u);
+        const float a = iac::sqrt_rn(x), b = sqrtf(x);
+        if (__float_as_uint(a) != __float_as_uint(b) && !(a != a && b != b)) nbad++;
+    }
+    if (nbad) atomicAdd(bad, nbad);
+}
+
 }  // namespace
 
 extern "C" {
+
+int iac_selfcheck_sqrt(iac_ctx *ctx, uint64_t lo, uint64_t hi, uint64_t *mismatches)
+{
+    IAC_TRY(iac::bind(ctx));
+    if (!mismatches || lo > hi || hi > (1ull << 32)) return iac::fail(IAC_ERR_INVALID_ARG, "bad sqrt check range");
+    void *d = nullptr;
+    IAC_TRY(iac::scratch(ctx, 0, sizeof(uint64_t), &d));
+    IAC_HIP(hipMemsetAsync(d, 0, sizeof(uint64_t), ctx->stream));
+    if (hi > lo) {
+        const unsigned blocks = (unsigned)std::max(1, ctx->num_cus * 16);
+        hipLaunchKernelGGL(sqrt_check_kernel, dim3(blocks), dim3(256), 0, ctx->stream, lo, hi,
+                           static_cast<unsigned long long *>(d));
+        IAC_HIP(hipGetLastError());
+    }
+    IAC_HIP(hipMemcpyAsync(mismatches, d, sizeof(uint64_t), hipMemcpyDeviceToHost, ctx->stream));
+    IAC_HIP(hipStreamSynchronize(ctx->stream));
+    return IAC_OK;
+}
 
 int iac_probe_atomics(iac_ctx *ctx, uint64_t *d_mesh, uint64_t nvox, uint64_t n_updates, int mode, uint64_t hot,
                       float *ms)

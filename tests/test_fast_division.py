@@ -1,0 +1,62 @@
+"""Pin of the AC1D kernels' division by the bin width (IAC/AC1D.cpp:212, 272:
+q = q / dq_per_Step in float).
+
+The kernels compute (float)((double)q * RN(1/(double)dq)) instead of the float
+division.  The claim: the exact quotient of two floats is never within 2^-49
+(relative) of a float rounding midpoint, while the double product errs by at
+most 2^-52, so both round to the same float whenever the quotient is 0 or a
+normal float.  numpy's float32 division is IEEE (correctly rounded) and is the
+reference; a plausible slip (a float reciprocal, a float product, a truncating
+conversion) fails these checks.
+"""
+import numpy as np
+
+import incohautocorrelate_amd as iac
+from incohautocorrelate_amd import workloads as W
+
+f32, f64 = np.float32, np.float64
+
+
+def fast(q, dq):
+    return (q.astype(f64) * (f64(1.0) / f64(dq))).astype(f32)
+
+
+def exact(q, dq):
+    out = q / f32(dq)
+    assert out.dtype == np.float32
+    return out
+
+
+def test_random_quotients():
+    rng = np.random.default_rng(0)
+    for _ in range(20):
+        dq = f32(2.0 ** rng.uniform(-40, 40))
+        q = (rng.uniform(0.5, 1.0, 1_000_000) * 2.0 ** rng.integers(-60, 60, 1_000_000)).astype(f32)
+        assert np.array_equal(fast(q, dq).view(np.uint32), exact(q, dq).view(np.uint32))
+
+
+def test_near_midpoints():
+    """q = RN(m dq) and its float neighbours, m a float rounding midpoint: the
+    quotients that land closest to where the float rounding flips."""
+    rng = np.random.default_rng(1)
+    for _ in range(40):
+        dq = f32(rng.uniform(1e-4, 1e-2))
+        f = rng.uniform(1.0, 4096.0, 200_000).astype(f32)
+        m = f.astype(f64) + np.spacing(f).astype(f64) / 2  # midpoints above f
+        q0 = (m * f64(dq)).astype(f32)
+        for k in range(-2, 3):
+            q = (q0.view(np.int32) + k).view(f32)
+            assert np.array_equal(fast(q, dq).view(np.uint32), exact(q, dq).view(np.uint32))
+
+
+def test_bin_widths_of_real_detectors():
+    """Every pair distance of a 96^2 detector against the AC1D bin widths the
+    kernels see (4096, 1000 and 600 bins)."""
+    _, km, mq = W.detector(96)
+    k = km.reshape(-1, 3)[::7]
+    d = k[:, None, :] - k[None, :, :]
+    q = np.sqrt(((d[..., 0] * d[..., 0]) + (d[..., 1] * d[..., 1])) + (d[..., 2] * d[..., 2])).ravel()
+    assert q.dtype == np.float32
+    for size in (4096, 1000, 600):
+        _, dq = iac.ac1d_shape(mq, size)
+        assert np.array_equal(fast(q, dq).view(np.uint32), exact(q, dq).view(np.uint32))

@@ -10,25 +10,32 @@ from incohautocorrelate_amd import workloads as W
 pytestmark = pytest.mark.gpu
 
 
-@pytest.mark.parametrize("interp", [0, 1])
-def test_ac1d_cq_vs_oracle(ctx, dev, interp):
+@pytest.mark.parametrize("interp,n,bright", [(0, 96, False), (1, 96, False), (1, 77, False), (0, 77, True),
+                                             (1, 64, True)])
+def test_ac1d_cq_vs_oracle(ctx, dev, interp, n, bright):
+    """n = 77: 5929 pixels, a ragged last tile row and column; the per-block tile
+    ranges cross triangle rows.  bright: a few pixels 3e4 times the mean, so that
+    weighted values pass 2^32 and the full 64-bit truncation is taken."""
     import torch
 
     import incohautocorrelate_amd as iac
 
-    n = 96
     _, km, mq = W.detector(n)
     size = 600
     max_q, dq = iac.ac1d_shape(mq, size)
     assert (max_q, dq) == O.ac1d_shape(mq, size)
     rng = np.random.default_rng(interp)
     I = rng.gamma(1.0, 0.5, n * n).astype(np.float32)
+    if bright:
+        I[rng.integers(0, n * n, 5)] = 1.5e4
     mult = iac.ac1d_cq_multiplicator(I)
     assert mult == O.ac1d_cq_multiplicator(I)
     ref, dref = O.ac1d_cq(I, km, size, max_q, dq, interp, mult)
     h = torch.zeros(size, dtype=torch.int64, device=dev)
     ctx.ac1d_cq(to_dev(km, dev), to_dev(I, dev), size, max_q, dq, interp, mult, h)
     assert dref == 0
+    if bright:  # the bright self pairs weigh more than 2^32
+        assert float(I.max()) ** 2 * mult >= 2 ** 32
     assert np.array_equal(mesh_np(h), ref)
 
 
@@ -132,3 +139,9 @@ def test_sparsified_frames_through_ac3d(ctx, dev):
     ctx.ac3d_records_batch(to_dev(km, dev), rec, off, nt, shape, mesh)
     ref, _ = O.ac3d_frames(km, d["pix"], d["cnt"], d["frame_off"], shape)
     assert np.array_equal(mesh_np(mesh), ref)
+
+
+def test_short_sqrt_exhaustive(ctx):
+    """The AC1D kernels' |q| square root against the library sqrtf on every one of
+    the 2^32 float bit patterns (zero, subnormals, the small-input branch, inf, NaN)."""
+    assert ctx.selfcheck_sqrt(0, 1 << 32) == 0
