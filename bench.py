@@ -264,7 +264,8 @@ UNEVEN_GRID = [dict(mode=m, box=b, cell_vox=c, cell_vox_v=c, frame_chunk=32)
 
 
 CQ_GRID = [dict(mode=m, box=b, cell_vox=c, cell_vox_v=c, frame_chunk=fc, region=r)
-           for m in (1, 2) for b, c in ((4096, 4), (1920, 4), (1920, 3)) for fc in (1, 32) for r in (0, 4)]
+           for m in (1, 2) for b, c in ((4096, 4), (4096, 5), (4096, 6), (1920, 3), (1920, 4), (1920, 5), (1024, 3))
+           for fc in (1, 8, 32) for r in (0, 2, 4)]
 
 
 def autotune(ctx, iac, d_kmap, d_pix, d_cnt, data, shape, d_mesh, dev):

@@ -94,9 +94,9 @@ int iac_ctx_timing(iac_ctx *ctx, double *ms, uint64_t *launches, int reset);
 #define IAC_OPT_BOX_CAP 6 /* LDS kernel: box capacity in voxels, 1024, 1920 (2048 selects it too) or 4096
                                (0 = 1920 for nearest neighbour, 4096 for linear) */
 #define IAC_OPT_CELL_VOX_V 7 /* LDS kernel: cell side along the second k axis, 0 = as IAC_OPT_CELL_VOX */
-#define IAC_OPT_TASK_REGION 8 /* LDS kernel: first cells per task region side, 0 = default (4 for C(q), else the
-                                   whole grid) */
-#define IAC_OPT_PAIR_MODE 9 /* LDS kernel: 0 default (1, except 2 for linear C(q)), 1 lanes on consecutive pairs,
+#define IAC_OPT_TASK_REGION 8 /* LDS kernel: first cells per task region side, 0 = default (2 for C(q) nearest
+                                   neighbour, 4 for C(q) linear, else the whole grid) */
+#define IAC_OPT_PAIR_MODE 9 /* LDS kernel: 0 default (1 for AC_uw, 2 for C(q)), 1 lanes on consecutive pairs,
                                 2 a contiguous pair range per thread */
 #define IAC_OPT_SIM_CHUNK_ROWS 4 /* simulator: detector rows (pixel x subpixel) per GEMM, 0 = auto (1 GiB of FP32 field) */
 int iac_ctx_set_option(iac_ctx *ctx, int option, int64_t value);

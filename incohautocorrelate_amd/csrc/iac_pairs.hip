@@ -554,6 +554,10 @@ __device__ __forceinline__ void fast_walk(const float4 *__restrict__ src, const 
     const int fox = S.P.c0 - T.bx0, foy = S.P.c1 - T.by0, foz = S.P.c2 - T.bz0;
     const int bx8 = 8 * T.bx, bxby8 = 8 * T.bx * T.by;
     const int c8 = 8 * fox + bx8 * foy + bxby8 * foz;
+    // C(q): byte stride of the mesh axis that record component i bins along (k_a -> fs, k_b -> ms, k_c -> ss)
+    const int cst0 = S.ka == 0 ? 8 : (S.kb == 0 ? bx8 : bxby8);
+    const int cst1 = S.ka == 1 ? 8 : (S.kb == 1 ? bx8 : bxby8);
+    const int cst2 = S.ka == 2 ? 8 : (S.kb == 2 ? bx8 : bxby8);
     const float vox = S.P.vox, cut_sq = S.P.cut_sq;
     char *const boxb = reinterpret_cast<char *>(box);
     uint32_t lo = 0, hi = nnz;  // the item holding w0: largest u with pre[u] <= w0
@@ -592,6 +596,23 @@ __device__ __forceinline__ void fast_walk(const float4 *__restrict__ src, const 
             const float4 ca = src[d.x + ia];
             const float4 cb = src[d.z + ib];
             bin_ordered<KIND>(S, ca, cb, sink);
+        } else if (active && KIND == kKindCq && INBOX) {
+            // C(q) of a task whose box holds every voxel its pairs reach: the bin of
+            // bin_sym (its value and axis order: round(q[ka] vox) + c0 is round_away
+            // of the same product plus c0) as one byte offset, component i scaled by
+            // the box stride of the mesh axis it maps to
+            const float4 ca = src[d.x + ia];
+            const float4 cb = src[d.z + ib];
+            asm volatile("" ::"v"(cb.w), "v"(ca.w));  // 16-byte loads
+            const float w = ca.w * cb.w;
+            const float q0 = ca.x - cb.x, q1 = ca.y - cb.y, q2 = ca.z - cb.z;
+            const float sq = (q0 * q0 + q1 * q1) + q2 * q2;
+            if (sq < cut_sq && !(w < 1e-37f)) {
+                const int o = __mul24(round_away(q2 * vox), cst2) +
+                              (__mul24(round_away(q1 * vox), cst1) + (__mul24(round_away(q0 * vox), cst0) + c8));
+                atomicAdd(reinterpret_cast<unsigned long long *>(boxb + o),
+                          (unsigned long long)(uint64_t)((double)w * S.mult));
+            }
         } else if (active && KIND == kKindCq) {
             // C(q): its own value and axis order, through the checked box
             const float4 ca = src[d.x + ia];
@@ -639,7 +660,10 @@ __device__ __forceinline__ void fast_walk_w(bool small, bool inbox, const float4
                                             BoxSink &sink, lds_flag_t *fl)
 {
     if (!FAST) {
-        fast_walk<KIND, ORD, false, false, false>(src, item, pre, nnz, w0, w1, tri, big, S, box, T, sink, fl);
+        if (KIND == kKindCq && !ORD && S.interp == 0 && inbox)
+            fast_walk<KIND, ORD, false, false, true>(src, item, pre, nnz, w0, w1, tri, big, S, box, T, sink, fl);
+        else
+            fast_walk<KIND, ORD, false, false, false>(src, item, pre, nnz, w0, w1, tri, big, S, box, T, sink, fl);
     } else if (inbox) {
         if (small) fast_walk<KIND, ORD, true, true, true>(src, item, pre, nnz, w0, w1, tri, big, S, box, T, sink, fl);
         else fast_walk<KIND, ORD, true, false, true>(src, item, pre, nnz, w0, w1, tri, big, S, box, T, sink, fl);
@@ -678,8 +702,17 @@ __global__ void __launch_bounds__(kBlock, (Occupancy<FAST, BOX>::blocks))
     const int vbox = T.bx * T.by * T.bz;
     const int fox = S.P.c0 - T.bx0, foy = S.P.c1 - T.by0, foz = S.P.c2 - T.bz0;  // fast path offsets
     const uint32_t fbxby = (uint32_t)(T.bx * T.by);
-    const bool inbox = FAST && T.inbox != 0;
+    // in-box tasks: the AC_uw specialised path, or symmetric nearest-neighbour C(q) (tight boxes)
+    const bool inbox = (FAST || (KIND == kKindCq && !ORD && S.interp == 0)) && T.inbox != 0;
     const bool small = FAST && wflag[0] != 0u;
+    // in-box C(q) on a pair range per thread: byte strides of the box axis each record
+    // component bins along, and the centre's byte offset in the box
+    const int cbx8 = 8 * T.bx, cbxby8 = 8 * T.bx * T.by;
+    const int cc8 = 8 * fox + cbx8 * foy + cbxby8 * foz;
+    const int cst0 = S.ka == 0 ? 8 : (S.kb == 0 ? cbx8 : cbxby8);
+    const int cst1 = S.ka == 1 ? 8 : (S.kb == 1 ? cbx8 : cbxby8);
+    const int cst2 = S.ka == 2 ? 8 : (S.kb == 2 ? cbx8 : cbxby8);
+    char *const cboxb = reinterpret_cast<char *>(box);
     for (int v = threadIdx.x; v < vbox; v += kBlock) box[v] = 0ull;
     __syncthreads();
 
@@ -812,14 +845,36 @@ __global__ void __launch_bounds__(kBlock, (Occupancy<FAST, BOX>::blocks))
                 ia = r / d.w;
                 ib = r % d.w;
             }
-            if (inbox) {
+            if (FAST && inbox) {
                 fast_pairs_w<true>(small, srec, item, pre, u, d, ia, ib, p1 - p0, tri, S, box, T, sink);
             } else if (PF == 1) {
                 // one record load per pair: the a record stays in a register along its row
                 float4 ra = srec[d.x + ia];
+                // in-box C(q): consecutive pairs of a row fall in runs on one voxel, so
+                // a run's values add in a register and reach the box in one atomic
+                int co = -1;
+                unsigned long long cacc = 0;
                 for (uint32_t p = p0;;) {
                     const float4 rb = srec[d.z + ib];
-                    if (FAST) {
+                    if (KIND == kKindCq && !ORD && inbox) {
+                        asm volatile("" ::"v"(rb.w));  // one 16-byte load
+                        const float w = ra.w * rb.w;
+                        const float q0 = ra.x - rb.x, q1 = ra.y - rb.y, q2 = ra.z - rb.z;
+                        const float sq = (q0 * q0 + q1 * q1) + q2 * q2;
+                        if (sq < S.P.cut_sq && !(w < 1e-37f)) {
+                            const int o = __mul24(round_away(q2 * S.P.vox), cst2) +
+                                          (__mul24(round_away(q1 * S.P.vox), cst1) +
+                                           (__mul24(round_away(q0 * S.P.vox), cst0) + cc8));
+                            const unsigned long long v = (uint64_t)((double)w * S.mult);
+                            if (o == co) {
+                                cacc += v;
+                            } else {
+                                if (co >= 0) atomicAdd(reinterpret_cast<unsigned long long *>(cboxb + co), cacc);
+                                co = o;
+                                cacc = v;
+                            }
+                        }
+                    } else if (FAST) {
                         // identity rotation, nearest neighbour: round(round(x)) == round(x), so every
                         // symmetric mode bins at (int)roundf(q * vox); box index in 24-bit multiplies
                         asm volatile("" ::"v"(rb.w));  // one 16-byte load, not xyz now and w later
@@ -861,6 +916,7 @@ __global__ void __launch_bounds__(kBlock, (Occupancy<FAST, BOX>::blocks))
                         ra = srec[d.x + ia];
                     }
                 }
+                if (co >= 0) atomicAdd(reinterpret_cast<unsigned long long *>(cboxb + co), cacc);
             } else
             for (uint32_t p = p0; p < p1;) {
                 const uint32_t nb = min((uint32_t)PF, p1 - p);
@@ -925,6 +981,13 @@ __global__ void __launch_bounds__(kBlock, (Occupancy<FAST, BOX>::blocks))
 inline bool fast_path(const PairSpec &S, bool ordered)
 {
     return S.kind == kKindAc3d && !ordered && S.P.rot_identity && std::sqrt((double)S.P.cut_sq) * S.P.vox < 4194304.0;
+}
+
+// symmetric nearest-neighbour C(q): its voxels are round-half-away of the same
+// fl(fl(ka - kb) * vox) products, so the tight boxes (and in-box tasks) apply
+inline bool cq_tight(const PairSpec &S, bool ordered)
+{
+    return S.kind == kKindCq && !ordered && S.interp == 0 && std::sqrt((double)S.P.cut_sq) * S.P.vox < 4194304.0;
 }
 
 // interval of round(x) / of floor(x)+{0,1} for x in [lo, hi]
@@ -1039,7 +1102,8 @@ void build_tasks(iac_pair_plan &pl, bool ordered)
     const int cen[3] = {S.P.c0, S.P.c1, S.P.c2};
     const int dim[3] = {S.P.sab, S.P.sab, S.P.sc};
     const int reg = pl.region > 0 ? pl.region : std::max(g.ncu, g.ncv);
-    const bool fast = fast_path(S, ordered);
+    const bool fast = fast_path(S, ordered), tight_cq = cq_tight(S, ordered);
+    const int cax[3] = {S.ka, S.kb, S.kc};  // C(q): the record component each mesh axis bins
     pl.tasks.clear();
     for (int dv = ordered ? -(g.ncv - 1) : 0; dv < g.ncv; dv++) {
         for (int du = -(g.ncu - 1); du < g.ncu; du++) {
@@ -1076,12 +1140,13 @@ void build_tasks(iac_pair_plan &pl, bool ordered)
                     }
                     if (std::sqrt(dist2) > maxq * (1.0 + 1e-5) + 1e-7) continue;
                     double lo[3], hi[3];
-                    if (fast) {
+                    if (fast || tight_cq) {
                         // the pair bins at round_away(fl(fl(ka - kb) * vox)): within eps of x in [l, h]
                         // (two float roundings, |x| < 2^22), so round-half-away lands in
                         // [ceil(l - eps - 1/2), floor(h + eps + 1/2)]
                         for (int d = 0; d < 3; d++) {
-                            const double l = qlo[d] * S.P.vox, h = qhi[d] * S.P.vox;
+                            const int c = tight_cq ? cax[d] : d;
+                            const double l = qlo[c] * S.P.vox, h = qhi[c] * S.P.vox;
                             const double eps = 1e-3 + 1e-6 * std::max(std::fabs(l), std::fabs(h));
                             lo[d] = std::ceil(l - eps - 0.5);
                             hi[d] = std::floor(h + eps + 0.5);
@@ -1099,6 +1164,7 @@ void build_tasks(iac_pair_plan &pl, bool ordered)
                         t.inbox = 1;
                         for (int d = 0; d < 3; d++)
                             if (lo[d] + cen[d] < 0.0 || hi[d] + cen[d] > (double)dim[d] - 1) t.inbox = 0;
+                        if (S.kind == kKindCq && !tight_cq) t.inbox = 0;  // in-box C(q) bins need the tight boxes
                         t.bx0 = ilo[0];
                         t.by0 = ilo[1];
                         t.bz0 = ilo[2];
@@ -1367,15 +1433,14 @@ bool walks(iac_ctx *ctx, const PairRun &R)
 {
     if (ctx->pair_mode == 1) return true;
     if (ctx->pair_mode == 2) return false;
-    return !(R.spec.kind == kKindCq && R.ordered);
+    // C(q) of dense frames: a pair range per thread, whose runs of one voxel add in a register
+    return R.spec.kind != kKindCq;
 }
 int cell_vox_of(iac_ctx *ctx, const PairRun &R)
 {
     int dflt = 4;
-    if (walks(ctx, R)) {
-        if (R.spec.kind == kKindCq) dflt = 3;
-        else dflt = R.ordered ? 5 : (fast_path(R.spec, R.ordered) ? 6 : 4);
-    }
+    if (R.spec.kind == kKindCq && !R.ordered) dflt = 3;
+    else if (walks(ctx, R)) dflt = R.ordered ? 5 : (fast_path(R.spec, R.ordered) ? 6 : 4);
     const int u = ctx->cell_vox > 0 ? ctx->cell_vox : dflt;
     return u + 1000 * (ctx->cell_vox_v > 0 ? ctx->cell_vox_v : u);
 }
@@ -1388,7 +1453,8 @@ int box_cap_of(iac_ctx *ctx, const PairRun &R)
 int region_of(iac_ctx *ctx, const PairRun &R)
 {
     if (ctx->task_region > 0) return ctx->task_region;
-    return R.spec.kind == kKindCq ? 4 : 0;
+    if (R.spec.kind != kKindCq) return 0;
+    return R.ordered ? 4 : 2;
 }
 uint32_t fchunk_of(iac_ctx *ctx) { return ctx->frame_chunk > 0 ? ctx->frame_chunk : 32; }
 

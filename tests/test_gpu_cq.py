@@ -158,11 +158,15 @@ def test_dense_float_rotation_merge_vs_fp64(ctx, dev):
     assert abs(got.sum() - ref.sum()) <= 1e-6 * ref.sum()
 
 
-@pytest.mark.parametrize("mode,cell,box", [(1, 4, 4096), (1, 8, 4096), (1, 6, 1920), (2, 4, 4096), (2, 6, 1920)])
-def test_cq_dense_pair_modes(ctx, dev, mode, cell, box):
+@pytest.mark.parametrize("mode,cell,box,region", [(1, 4, 4096, 0), (1, 8, 4096, 0), (1, 6, 1920, 0), (2, 4, 4096, 0),
+                                                (2, 6, 1920, 0), (1, 3, 1920, 2), (2, 3, 1920, 2), (2, 3, 1024, 0),
+                                                (2, 5, 4096, 3)])
+def test_cq_dense_pair_modes(ctx, dev, mode, cell, box, region):
     """Nearest-neighbour C(q) of dense frames gives the oracle's bits with lanes on
-    consecutive pairs and with a pair range per thread, at several cell sizes and
-    box classes (a checked box: pairs outside it go to global memory)."""
+    consecutive pairs and with a pair range per thread, at several cell sizes, box
+    classes and task regions: in-box tasks bin inline (mode 2 adding runs of one
+    voxel in a register), the others through a checked box whose misses go to
+    global memory."""
     import incohautocorrelate_amd as iac
 
     n = 96
@@ -178,6 +182,7 @@ def test_cq_dense_pair_modes(ctx, dev, mode, cell, box):
     ctx.set_option(iac.OPT_PAIR_MODE, mode)
     ctx.set_option(iac.OPT_CELL_VOX, cell)
     ctx.set_option(iac.OPT_BOX_CAP, box)
+    ctx.set_option(iac.OPT_TASK_REGION, region)
     try:
         mesh = _zeros(s.size_ab ** 2 * s.size_c, dev)
         ctx.cq_dense_frames(to_dev(km, dev), to_dev(frames, dev), goff, s, mesh, 0, mult)
@@ -186,4 +191,5 @@ def test_cq_dense_pair_modes(ctx, dev, mode, cell, box):
         ctx.set_option(iac.OPT_PAIR_MODE, 0)
         ctx.set_option(iac.OPT_CELL_VOX, 0)
         ctx.set_option(iac.OPT_BOX_CAP, 0)
+        ctx.set_option(iac.OPT_TASK_REGION, 0)
     assert np.array_equal(got, ref)
