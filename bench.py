@@ -169,7 +169,7 @@ def main():
     if not args.no_cpu:
         cpu = cpu_baseline(kmap, data, shape, ctx, iac, d_kmap, dev, args)
 
-    ac1d = None if args.no_ac1d else ac1d_leg(args, iac, W, dev, d_kmap, d_pix, d_cnt, d_off, data, max_q)
+    ac1d = None if args.no_ac1d else ac1d_leg(args, iac, W, dev, kmap, d_kmap, d_pix, d_cnt, d_off, data, max_q)
     del d_pix, d_cnt, d_off, d_mesh
     torch.cuda.empty_cache()
     cq = None if args.no_cq else cq_dense_leg(args, iac, W, dev)
@@ -377,7 +377,66 @@ def cpu_baseline(kmap, data, shape, ctx, iac, d_kmap, dev, args):
     }
 
 
-def ac1d_leg(args, iac, W, dev, d_kmap, d_pix, d_cnt, d_off, data, max_q_vec):
+def _oracle():
+    sys.path.insert(0, str(ROOT / "oracle"))
+    import oracle as O
+
+    return O
+
+
+def _cpu_cores():
+    return min(len(os.sched_getaffinity(0)), 16)
+
+
+def ac1d_acuw_cpu(args, kmap, data, size, max_q, dq):
+    """The oracle's AC1D AC_uw restatement (the reference CPU branch, one thread, linear
+    split, self pairs weighted 2) on a prefix of the same frames, about a fifth of
+    --cpu-seconds; hit lists built before the clock starts."""
+    O = _oracle()
+    off = data["frame_off"]
+    hls = []
+
+    def run(nf):
+        while len(hls) < nf:
+            f = len(hls)
+            hls.append(O.hit_list_from_photons(kmap, data["pix"][off[f]:off[f + 1]], data["cnt"][off[f]:off[f + 1]]))
+        ac = np.zeros(size)
+        t = time.perf_counter()
+        for f in range(nf):
+            O.ac1d_acuw(hls[f], size, max_q, dq, 1, 2, ac=ac)
+        return time.perf_counter() - t
+
+    per = run(2) / 2
+    nf = int(max(2, min(len(off) - 1, args.cpu_seconds / 5 / max(per, 1e-6))))
+    dt = run(nf)
+    pairs = float((np.diff(off[: nf + 1].astype(np.int64)).astype(np.float64) ** 2).sum())
+    log(f"[ac1d] CPU oracle: {nf} frames, {pairs:.3e} ordered pairs in {dt:.2f}s on 1 thread")
+    return {"value": pairs / dt, "unit": "ordered photon pairs/s", "cores": 1, "kind": "port",
+            "sample": f"first {nf} frames ({pairs:.4e} ordered pairs) in {dt:.2f} s"}
+
+
+def dense_cpu(args, label, run_k, npix, unordered):
+    """Time an oracle pass over the first K pixels of an image (K^2 ordered pixel
+    pairs, OpenMP over the outer pixel), K sized to about a fifth of --cpu-seconds.
+    `unordered`: report the job-equivalent K(K+1)/2 unordered pairs per second (the
+    unit of the GPU leg, whose symmetric kernel visits each unordered pair once)."""
+    cores = _cpu_cores()
+    k = min(npix, 4096)
+    t = time.perf_counter()
+    run_k(k, cores)
+    dt = time.perf_counter() - t
+    k = int(min(npix, max(k, k * (args.cpu_seconds / 5 / max(dt, 1e-6)) ** 0.5)))
+    t = time.perf_counter()
+    run_k(k, cores)
+    dt = time.perf_counter() - t
+    pairs = k * (k + 1) / 2 if unordered else float(k) * k
+    log(f"[{label}] CPU oracle: {k} pixels, {pairs:.3e} pairs in {dt:.2f}s on {cores} threads")
+    return {"value": pairs / dt, "unit": ("unordered" if unordered else "ordered") + " pixel pairs/s",
+            "cores": cores, "kind": "port",
+            "sample": f"the first {k} pixels of the image ({k}^2 ordered pairs visited) in {dt:.2f} s"}
+
+
+def ac1d_leg(args, iac, W, dev, kmap, d_kmap, d_pix, d_cnt, d_off, data, max_q_vec):
     """AC1D AC_uw on the headline workload: the radial |q| histogram of every frame's
     ordered photon pairs (4096 bins, linear split as the evaluation forces), timed
     against the measured LDS-atomic ceiling."""
@@ -407,7 +466,8 @@ def ac1d_leg(args, iac, W, dev, d_kmap, d_pix, d_cnt, d_off, data, max_q_vec):
     out = {"workload": f"AC1D AC_uw of the headline frames: {len(sizes)} frames, {size} bins, linear split, "
                        f"self pairs weighted 2",
            "value": pairs / (ms * 1e-3), "unit": "ordered photon pairs/s", "ms_per_step": ms, "steps": steps,
-           "pairs_per_step": pairs}
+           "pairs_per_step": pairs,
+           "cpu_baseline": None if args.no_cpu else ac1d_acuw_cpu(args, kmap, data, size, max_q, dq)}
 
     # AC1D C(q): every unordered pixel pair of one averaged-intensity image of the same detector
     npix = int(d_kmap.shape[0])
@@ -431,9 +491,15 @@ def ac1d_leg(args, iac, W, dev, d_kmap, d_pix, d_cnt, d_off, data, max_q_vec):
     ctx.close()
     cq_pairs = npix * (npix + 1) // 2
     log(f"[ac1d-cq] {cq_ms:.2f} ms/step, {cq_pairs:.4e} pixel pairs, {cq_pairs / (cq_ms * 1e-3):.4e} pairs/s")
+    cq_cpu = None
+    if not args.no_cpu:
+        O = _oracle()
+        km = kmap.reshape(-1, 3)
+        cq_cpu = dense_cpu(args, "ac1d-cq", lambda k, c: O.ac1d_cq(img[:k], km[:k], size, max_q, dq, 1, mult,
+                                                                    nthreads=c), npix, True)
     out["cq"] = {"workload": f"AC1D C(q) of one {npix}-pixel Gamma(1, 0.5) image: {size} bins, linear split",
                  "value": cq_pairs / (cq_ms * 1e-3), "unit": "unordered pixel pairs/s", "ms_per_step": cq_ms,
-                 "steps": cq_steps, "pairs_per_step": cq_pairs}
+                 "steps": cq_steps, "pairs_per_step": cq_pairs, "cpu_baseline": cq_cpu}
     return out
 
 
@@ -484,10 +550,16 @@ def cq_dense_leg(args, iac, W, dev):
     ctx.close()
     pairs = float(n * n) ** 2 * nf
     log(f"[cq] dense {n}^2 x {nf} frames: {ms:.2f} ms/step, {pairs / (ms * 1e-3):.4e} pixel pairs/s")
+    cpu = None
+    if not args.no_cpu:
+        O = _oracle()
+        km = kmap.reshape(-1, 3)
+        cpu = dense_cpu(args, "cq", lambda k, c: O.cq_small(frames[0][:k], km[:k], s, 0, mult, nthreads=c),
+                        n * n, False)
     return {"workload": f"dense C(q) small (AutoCorr_CQ_small): {n}x{n} detector, {nf} FP32 Gamma(1, 0.5) frames, "
                         f"{s.size_ab}x{s.size_ab}x{s.size_c} uint64 mesh, nearest neighbour",
-            "value": pairs / (ms * 1e-3), "unit": "pixel pairs/s", "ms_per_step": ms, "steps": steps,
-            "pairs_per_step": pairs, "tuned": tuned}
+            "value": pairs / (ms * 1e-3), "unit": "ordered pixel pairs/s", "ms_per_step": ms, "steps": steps,
+            "pairs_per_step": pairs, "tuned": tuned, "cpu_baseline": cpu}
 
 
 def eval_leg(args, W):
