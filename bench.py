@@ -263,9 +263,10 @@ UNEVEN_GRID = [dict(mode=m, box=b, cell_vox=c, cell_vox_v=c, frame_chunk=32)
                for m, b, c in ((1, 1920, 4), (1, 1920, 5), (1, 1920, 6), (1, 4096, 8))]
 
 
-CQ_GRID = [dict(mode=m, box=b, cell_vox=c, cell_vox_v=c, frame_chunk=fc, region=r)
-           for m in (1, 2) for b, c in ((4096, 4), (4096, 5), (4096, 6), (1920, 3), (1920, 4), (1920, 5), (1024, 3))
-           for fc in (1, 8, 32) for r in (0, 2, 4)]
+# around the defaults (mode 2, 3-voxel cells, 1920-voxel boxes, 2x2-cell regions); frames per
+# block do not matter here (a dense frame is one chunk)
+CQ_GRID = [dict(mode=m, box=b, cell_vox=c, cell_vox_v=c, frame_chunk=32, region=r)
+           for m in (1, 2) for b, c in ((1920, 3), (1024, 3), (1920, 4), (4096, 4), (1920, 5)) for r in (2, 4)]
 
 
 def autotune(ctx, iac, d_kmap, d_pix, d_cnt, data, shape, d_mesh, dev):
