@@ -224,12 +224,29 @@ int make_params(const iac_mesh_shape *s, int interp1, int interp2, int double_ma
     P->interp2 = interp2;
     P->double_map = double_map ? 1 : 0;
     P->rot_identity = 1;
+    P->dval = 0;
     return IAC_OK;
 }
 
 bool symmetric_mode(const iac::BinParams &P)
 {
     return P.interp2 == IAC_NEAREST && (!P.double_map || P.interp1 == IAC_NEAREST);
+}
+
+// The LDS engine's binning (identity rotation).  A double map with a nearest first
+// level hands the second level an integral voxel: its linear split has one corner,
+// of weight 1, the voxel itself (Atomic_Add_rotated_q_Entry, ACMesh.cpp:337-384,
+// with Sep = 1), valued FloatToInt((double)v).  That is nearest-neighbour binning
+// with the double fixed point, point symmetric like the nearest modes.
+iac::BinParams engine_params(const iac::BinParams &P)
+{
+    iac::BinParams E = P;
+    E.rot_identity = 1;
+    if (P.double_map && P.interp1 == IAC_NEAREST && P.interp2 == IAC_LINEAR) {
+        E.interp2 = IAC_NEAREST;
+        E.dval = 1;
+    }
+    return E;
 }
 
 int launch_tiles(iac_ctx *ctx, const float4 *rec, const uint64_t *frame_off, const uint64_t *tile_off,
@@ -271,9 +288,8 @@ int batch_records(iac_ctx *ctx, const float *d_kmap, uint32_t npix, const float4
         std::memset(&R.spec, 0, sizeof(R.spec));
         R.spec.kind = iac::kKindAc3d;
         for (int i = 0; i < 9; i++) R.spec.R[i] = (i % 4) == 0 ? 1.0f : 0.0f;
-        R.spec.P = P;
-        R.spec.P.rot_identity = 1;
-        R.ordered = !symmetric_mode(P);
+        R.spec.P = engine_params(P);
+        R.ordered = !symmetric_mode(R.spec.P);
         R.d_kmap = d_kmap;
         R.npix = npix;
         R.rec = rec;
@@ -383,9 +399,8 @@ int iac_ac3d_batch_create(iac_ctx *ctx, const float *d_kmap, uint32_t npix, cons
     std::memset(&R.spec, 0, sizeof(R.spec));
     R.spec.kind = iac::kKindAc3d;
     for (int i = 0; i < 9; i++) R.spec.R[i] = (i % 4) == 0 ? 1.0f : 0.0f;
-    R.spec.P = P;
-    R.spec.P.rot_identity = 1;
-    R.ordered = !symmetric_mode(P);
+    R.spec.P = engine_params(P);
+    R.ordered = !symmetric_mode(R.spec.P);
     R.d_kmap = d_kmap;
     R.npix = npix;
     R.rec = b->rec;

@@ -22,6 +22,10 @@ struct BinParams {
     int sab, sc;         // Size_AB, Size_C
     int interp1, interp2, double_map;
     int rot_identity;    // rotation is exactly the identity matrix
+    // symmetric binning valued by FloatToInt(double): a linear second level over
+    // the integral first-level voxel of a double map under the identity rotation
+    // puts the whole pair on one corner, weight 1 (the LDS engine's pair spec only)
+    int dval;
 };
 
 // FloatToInt(float) then the implicit long -> unsigned int of ACMesh.cpp:390-391.
@@ -95,14 +99,18 @@ __device__ __forceinline__ void add_rotated(Sink &sink, float q0, float q1, floa
         int ms = (int)floorf(q1) + P.c1;
         int ss = (int)floorf(q2) + P.c2;
         double v = (double)value;
-        // corner (ex, ey, ez) gets v * ((wx * wy) * wz), w = s or 1 - s per axis
+        // corner (ex, ey, ez) gets v * ((wx * wy) * wz), w = s or 1 - s per axis.  A corner of
+        // weight 0 adds FloatToInt(0) = 0 (finite v) and is skipped: an integral q -- the first
+        // level's voxel under the identity rotation -- leaves one corner of eight
 #pragma unroll 1
         for (int c = 0; c < 8; c++) {
             const int ex = c & 1, ey = (c >> 1) & 1, ez = (c >> 2) & 1;
             const double wx = ex ? (1 - sfs) : sfs;
             const double wy = ey ? (1 - sms) : sms;
             const double wz = ez ? (1 - sss) : sss;
-            sink.add(fs + ex, ms + ey, ss + ez, pair_val_d(v * ((wx * wy) * wz)));
+            const double w = (wx * wy) * wz;
+            if (w == 0.0) continue;
+            sink.add(fs + ex, ms + ey, ss + ez, pair_val_d(v * w));
         }
     }
 }
@@ -142,6 +150,7 @@ __device__ __forceinline__ void add_q_entry(Sink &sink, float ax, float ay, floa
                 double wx = ex ? (1 - sfs) : sfs;
                 double wy = ey ? (1 - sms) : sms;
                 double wz = ez ? (1 - sss) : sss;
+                if (wx == 0.0 || wy == 0.0 || wz == 0.0) continue;  // its value is 0: every add is 0
                 add_rotated(sink, x, y, z, (float)(((v * wx) * wy) * wz), P);
             }
         }

@@ -143,6 +143,13 @@ __device__ __forceinline__ uint32_t pair_val_small(float p)
     return pair_val(p);
 }
 
+// the value of a symmetric AC_uw pair: FloatToInt(float), or FloatToInt(double)
+// for a linear second level over an integral first-level voxel (BinParams::dval)
+__device__ __forceinline__ uint32_t sym_val(const BinParams &P, float p)
+{
+    return P.dval ? pair_val_d((double)p) : pair_val_small(p);
+}
+
 // ---- binning policies -------------------------------------------------------
 
 // C(q) fixed point of AutoCorr_CQ_small (kernel.cl:93-101): (ulong)((double)(Ia*Ib) * M),
@@ -161,7 +168,7 @@ __device__ __forceinline__ bool bin_sym(const PairSpec &S, const float4 &a, cons
     if (KIND == kKindAc3d) {
         int dx, dy, dz;
         if (!bin_nn(a.x, a.y, a.z, b.x, b.y, b.z, S.P, S.R, dx, dy, dz)) return false;
-        v = pair_val(a.w * b.w);
+        v = sym_val(S.P, a.w * b.w);
         fs = dx + S.P.c0;
         ms = dy + S.P.c1;
         ss = dz + S.P.c2;
@@ -186,7 +193,7 @@ __device__ __forceinline__ bool bin_sym(const PairSpec &S, const float4 &a, cons
 template <int KIND>
 __device__ __forceinline__ uint64_t self_value(const PairSpec &S, float c)
 {
-    if (KIND == kKindAc3d) return pair_val(c * c);
+    if (KIND == kKindAc3d) return sym_val(S.P, c * c);
     double val;
     return cq_value(S, c * c, val) ? (uint64_t)(val * S.mult) : 0ull;
 }
@@ -980,7 +987,9 @@ __global__ void __launch_bounds__(kBlock, (Occupancy<FAST, BOX>::blocks))
 // rotation; round_away needs |q * vox| < 2^23 and the cut bounds |q| by Max_Q.
 inline bool fast_path(const PairSpec &S, bool ordered)
 {
-    return S.kind == kKindAc3d && !ordered && S.P.rot_identity && std::sqrt((double)S.P.cut_sq) * S.P.vox < 4194304.0;
+    // (its inline value is FloatToInt(float): the double-valued symmetric mode takes bin_sym)
+    return S.kind == kKindAc3d && !ordered && S.P.rot_identity && !S.P.dval &&
+           std::sqrt((double)S.P.cut_sq) * S.P.vox < 4194304.0;
 }
 
 // symmetric nearest-neighbour C(q): its voxels are round-half-away of the same

@@ -202,3 +202,46 @@ def test_linear_modes_pair_walks(ctx, dev, mode, cell, box):
     finally:
         for o in (iac.OPT_AC3D_KERNEL, iac.OPT_PAIR_MODE, iac.OPT_CELL_VOX, iac.OPT_BOX_CAP):
             ctx.set_option(o, 0)
+
+
+@pytest.mark.parametrize("mode", [1, 2])
+def test_nn_linear_double_map_symmetric(ctx, dev, mode):
+    """Double map, nearest first level, linear second level, identity rotation: the
+    engine bins it point-symmetrically on the voxel itself, valued FloatToInt(double).
+    Float weights and integer counts whose products pass 10,527 (where the double and
+    float fixed points part) pin that value against the oracle's ordered 8-corner loop."""
+    import torch
+
+    import incohautocorrelate_amd as iac
+
+    n = 96
+    _, km, mq = W.detector(n)
+    shape = iac.mesh_shape_big(mq, 47)
+    rng = np.random.default_rng(77)
+    hls = []
+    for f in range(6):
+        npho = int(rng.integers(20, 120))
+        pix = rng.choice(n * n, npho, replace=False)
+        w = (rng.gamma(1.0, 3.0, npho) if f % 2 == 0 else rng.integers(1, 300, npho)).astype(np.float32)
+        hl = np.empty((npho, 4), np.float32)
+        hl[:, :3] = km[pix]
+        hl[:, 3] = w
+        hls.append(hl)
+    ref = np.zeros(shape.size_ab ** 3, np.uint64)
+    ref_nn = np.zeros(shape.size_ab ** 3, np.uint64)
+    for hl in hls:
+        ref, _ = O.ac3d_sparse(hl, shape, None, 0, 1, True, mesh=ref)
+        ref_nn, _ = O.ac3d_sparse(hl, shape, None, 0, 0, True, mesh=ref_nn)
+    assert not np.array_equal(ref, ref_nn)  # the value function matters for these weights
+    rec = np.concatenate(hls)
+    off = np.concatenate([[0], np.cumsum([len(h) for h in hls])]).astype(np.uint64)
+    ctx.set_option(iac.OPT_AC3D_KERNEL, 2)
+    ctx.set_option(iac.OPT_PAIR_MODE, mode)
+    try:
+        m = torch.zeros(shape.size_ab ** 3, dtype=torch.int64, device=dev)
+        ctx.ac3d_records_batch(to_dev(km, dev), to_dev(rec, dev), to_dev(off, dev), len(rec), shape, m,
+                               interp1=0, interp2=1)
+        assert np.array_equal(mesh_np(m), ref)
+    finally:
+        ctx.set_option(iac.OPT_AC3D_KERNEL, 0)
+        ctx.set_option(iac.OPT_PAIR_MODE, 0)
