@@ -133,9 +133,12 @@ int iac_ac3d_sparse(iac_ctx *ctx, const float *d_hitlist4, uint32_t n, const flo
  * the photon entries [d_frame_off[f], d_frame_off[f+1]) of (pixel index,
  * count) lists d_pix/d_cnt; k vectors are gathered from d_kmap ([npix][3]).
  * d_rots is [nframes][9] floats on the device or NULL for the identity.
- * n_total = d_frame_off[nframes] (host copy, sizes the scratch).  One pass of
- * a persistent, load-balanced kernel covers all frames; the mesh is identical
- * to calling iac_ac3d_sparse once per frame. */
+ * n_total = d_frame_off[nframes] (host copy, sizes the scratch).  The mesh is
+ * identical to calling iac_ac3d_sparse once per frame.  Runs on the LDS
+ * cell-box engine; per-frame rotations too when the double map has nearest-
+ * neighbour levels and no rotated voxel can leave the mesh (the frames bin
+ * under the identity and each box voxel is rotated as it flushes), otherwise
+ * on the global-atomic tile kernel. */
 int iac_ac3d_sparse_batch(iac_ctx *ctx, const float *d_kmap, uint32_t npix,
                           const uint32_t *d_pix, const uint16_t *d_cnt,
                           const uint64_t *d_frame_off, uint32_t nframes, uint64_t n_total,

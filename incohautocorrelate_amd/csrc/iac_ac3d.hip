@@ -12,8 +12,8 @@
 // add; the self pair lands on the centre once.  Linear modes bin both orders.
 #include <algorithm>
 #include <cmath>
-
 #include <cstring>
+#include <vector>
 
 #include "iac_binning.h"
 #include "iac_internal.h"
@@ -273,17 +273,61 @@ int launch_tiles(iac_ctx *ctx, const float4 *rec, const uint64_t *frame_off, con
     return IAC_OK;
 }
 
-// Frames of records through the LDS cell-box engine (one shared rotation, here
-// the identity) or, for per-frame rotations, the tile kernel.
+// Per-frame rotations on the LDS engine.  With a double map and nearest-neighbour
+// levels a pair's voxel is round(R_f round(q vox)): it depends on the pair only
+// through its first-level voxel, so the frames bin under the identity and each box
+// voxel is rotated when it flushes.  Taken when no rotated voxel can leave the mesh
+// (the tile kernel counts such updates pair by pair): a first-level voxel is at most
+// r1 = Max_Q vox (1 + 1e-6) + sqrt(3)/2 long, its rotated component i at most
+// r1 |R_i|_2 plus the float rounding of the three-term row product.
+bool rotations_on_engine(const iac::BinParams &P, const std::vector<float> &h, uint32_t nframes)
+{
+    if (!(P.double_map && P.interp1 == IAC_NEAREST && P.interp2 == IAC_NEAREST)) return false;
+    const double r1 = std::sqrt((double)P.cut_sq) * (double)P.vox * (1.0 + 1e-6) + 0.8660254038;
+    const int lim = std::min({P.c0, P.c1, P.c2, P.sab - 1 - P.c0, P.sab - 1 - P.c1, P.sc - 1 - P.c2});
+    for (uint32_t f = 0; f < nframes; f++) {
+        for (int i = 0; i < 3; i++) {
+            double l1 = 0, l2 = 0;
+            for (int j = 0; j < 3; j++) {
+                const double a = h[9 * (size_t)f + 3 * i + j];
+                if (!std::isfinite(a)) return false;
+                l1 += std::fabs(a);
+                l2 += a * a;
+            }
+            if (!(r1 * std::sqrt(l2) + 4e-7 * r1 * l1 < lim + 0.5)) return false;
+        }
+    }
+    return true;
+}
+
+// Frames of records through the LDS cell-box engine (the identity, or per-frame
+// rotations applied at the box flush) or the tile kernel.
 int batch_records(iac_ctx *ctx, const float *d_kmap, uint32_t npix, const float4 *rec,
                   const uint64_t *d_frame_off, uint32_t nframes, uint64_t n_total, const float *d_rots,
                   const iac::BinParams &P, uint64_t *d_mesh)
 {
-    const bool engine_ok = d_rots == nullptr && n_total < (1ull << 32);
+    bool engine_ok = n_total < (1ull << 32) && ctx->ac3d_kernel != 1;
+    std::vector<float> hrot;
+    if (engine_ok && d_rots) {
+        hrot.resize(9 * (size_t)nframes);
+        IAC_HIP(hipMemcpyAsync(hrot.data(), d_rots, hrot.size() * sizeof(float), hipMemcpyDeviceToHost,
+                               ctx->stream));
+        IAC_HIP(hipStreamSynchronize(ctx->stream));
+        engine_ok = rotations_on_engine(P, hrot, nframes);
+        bool all_identity = true;
+        for (uint32_t f = 0; f < nframes && all_identity; f++)
+            for (int i = 0; i < 9; i++)
+                if (hrot[9 * (size_t)f + i] != ((i % 4) == 0 ? 1.0f : 0.0f)) all_identity = false;
+        if (engine_ok && all_identity) {
+            d_rots = nullptr;
+            hrot.clear();
+        }
+    }
     if (ctx->ac3d_kernel == 2 && !engine_ok)
         return iac::fail(IAC_ERR_PRECONDITION,
-                         "the LDS engine needs one rotation shared by all frames and < 2^32 photons");
-    if (engine_ok && ctx->ac3d_kernel != 1) {
+                         "the LDS engine needs < 2^32 photons and, for per-frame rotations, a double map with "
+                         "nearest-neighbour levels whose rotated voxels stay in the mesh");
+    if (engine_ok) {
         iac::PairRun R;
         std::memset(&R.spec, 0, sizeof(R.spec));
         R.spec.kind = iac::kKindAc3d;
@@ -300,6 +344,10 @@ int batch_records(iac_ctx *ctx, const float *d_kmap, uint32_t npix, const float4
         R.h_group_off = goff;
         R.ngroups = 1;
         R.d_mesh = d_mesh;
+        if (d_rots) {
+            R.d_rots = d_rots;
+            R.h_rots = hrot.data();
+        }
         return iac::pairs_run(ctx, R);
     }
     void *toff = nullptr;

@@ -35,6 +35,12 @@ struct PairRun {
     const uint32_t *h_group_off;  // host [ngroups + 1]: frames of group g are [off[g], off[g+1])
     uint32_t ngroups;
     uint64_t *d_mesh;             // ngroups meshes of size_ab^2 * size_c uint64, accumulated into
+    // AC_uw with a double map and nearest-neighbour levels: per-frame row-major rotations
+    // ([nframes][9], device, and the same on the host for chunking), applied to the
+    // first-level voxel when a box flushes; the pairs bin under the identity (spec.R).
+    // The caller has checked that no rotated voxel leaves the mesh.
+    const float *d_rots = nullptr;
+    const float *h_rots = nullptr;
 };
 
 int pairs_run(iac_ctx *ctx, const PairRun &run);

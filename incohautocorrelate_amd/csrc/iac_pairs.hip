@@ -423,6 +423,18 @@ __global__ void symmetrize_kernel(const unsigned long long *__restrict__ H, unsi
     }
 }
 
+// A first-level voxel (mesh coordinates of the identity binning) under the frame's
+// rotation: Atomic_Add_rotated_q_Entry's nearest second level (ACMesh.cpp:319-336)
+// applied to the float voxel offsets, which equal the first level's roundf(q vox).
+__device__ __forceinline__ void rotate_voxel(const float *r, int c0, int c1, int c2, int &fs, int &ms, int &ss)
+{
+    float q0 = (float)(fs - c0), q1 = (float)(ms - c1), q2 = (float)(ss - c2);
+    rotate3(q0, q1, q2, r);
+    fs = (int)roundf(q0) + c0;
+    ms = (int)roundf(q1) + c1;
+    ss = (int)roundf(q2) + c2;
+}
+
 struct BoxSink {
     unsigned long long *box;  // LDS
     unsigned long long *out;  // global mesh (group's H or mesh)
@@ -430,16 +442,20 @@ struct BoxSink {
     int bx0, by0, bz0, bx, by, bz;
     int S, SC;
     unsigned long long drop_weight;
+    const float *rot;  // the chunk's rotation of first-level voxels, or null
+    int c0, c1, c2;
     __device__ __forceinline__ void add(int fs, int ms, int ss, uint64_t v)
     {
         if (!v) return;
         const unsigned lx = (unsigned)(fs - bx0), ly = (unsigned)(ms - by0), lz = (unsigned)(ss - bz0);
         if (lx < (unsigned)bx && ly < (unsigned)by && lz < (unsigned)bz) {
             atomicAdd(box + lx + (size_t)bx * (ly + (size_t)by * lz), (unsigned long long)v);
-        } else if ((unsigned)fs < (unsigned)S && (unsigned)ms < (unsigned)S && (unsigned)ss < (unsigned)SC) {
-            atomicAdd(out + (size_t)fs + (size_t)ms * S + (size_t)ss * S * S, (unsigned long long)v);
         } else {
-            atomicAdd(dropped, drop_weight);
+            if (rot) rotate_voxel(rot, c0, c1, c2, fs, ms, ss);
+            if ((unsigned)fs < (unsigned)S && (unsigned)ms < (unsigned)S && (unsigned)ss < (unsigned)SC)
+                atomicAdd(out + (size_t)fs + (size_t)ms * S + (size_t)ss * S * S, (unsigned long long)v);
+            else
+                atomicAdd(dropped, drop_weight);
         }
     }
 };
@@ -686,7 +702,8 @@ __global__ void __launch_bounds__(kBlock, (Occupancy<FAST, BOX>::blocks))
                  const uint32_t *__restrict__ cs, const Chunk *__restrict__ chunks,
                  const Task *__restrict__ tasks, uint32_t ntasks, CellGeom g, PairSpec S,
                  unsigned long long *__restrict__ target, size_t nvox,
-                 unsigned long long *__restrict__ dropped, const uint32_t *__restrict__ wflag, int walk)
+                 unsigned long long *__restrict__ dropped, const uint32_t *__restrict__ wflag, int walk,
+                 const float *__restrict__ rots)
 {
     constexpr int kGroup = BoxCfg<BOX>::group;
     constexpr int kItemsPerThread = BoxCfg<BOX>::ipt;
@@ -704,8 +721,10 @@ __global__ void __launch_bounds__(kBlock, (Occupancy<FAST, BOX>::blocks))
     const bool zero_delta = (T.du == 0 && T.dv == 0);
     const bool tri = !ORD && zero_delta;  // distinct unordered pairs within a cell
 
+    // per-frame rotations: a chunk holds frames of one rotation, applied to first-level voxels
+    const float *crot = rots ? rots + 9 * (size_t)CH.f0 : nullptr;
     BoxSink sink{box, target + (size_t)CH.group * nvox, dropped, T.bx0, T.by0, T.bz0, T.bx, T.by, T.bz,
-                 S.P.sab, S.P.sc, ORD ? 1ull : 2ull};
+                 S.P.sab, S.P.sc, ORD ? 1ull : 2ull, crot, S.P.c0, S.P.c1, S.P.c2};
     const int vbox = T.bx * T.by * T.bz;
     const int fox = S.P.c0 - T.bx0, foy = S.P.c1 - T.by0, foz = S.P.c2 - T.bz0;  // fast path offsets
     const uint32_t fbxby = (uint32_t)(T.bx * T.by);
@@ -968,15 +987,26 @@ __global__ void __launch_bounds__(kBlock, (Occupancy<FAST, BOX>::blocks))
         __syncthreads();
     }
 
-    // flush the LDS box
+    // flush the LDS box (under the chunk's rotation, voxel by voxel: every pair of a box
+    // voxel shares its first-level voxel, so the rotated second level bins them alike)
     unsigned long long *out = target + (size_t)CH.group * nvox;
+    float rr[9];
+    if (crot)
+        for (int i = 0; i < 9; i++) rr[i] = crot[i];
     for (int v = threadIdx.x; v < vbox; v += kBlock) {
         const unsigned long long x = box[v];
         if (x) {
             const int lx = v % T.bx, rest = v / T.bx, ly = rest % T.by, lz = rest / T.by;
-            const size_t gi = (size_t)(T.bx0 + lx) + (size_t)(T.by0 + ly) * S.P.sab +
-                              (size_t)(T.bz0 + lz) * S.P.sab * S.P.sab;
-            atomicAdd(out + gi, x);
+            int fs = T.bx0 + lx, ms = T.by0 + ly, ss = T.bz0 + lz;
+            if (crot) {
+                rotate_voxel(rr, S.P.c0, S.P.c1, S.P.c2, fs, ms, ss);
+                if ((unsigned)fs >= (unsigned)S.P.sab || (unsigned)ms >= (unsigned)S.P.sab ||
+                    (unsigned)ss >= (unsigned)S.P.sc) {
+                    atomicAdd(dropped, 1ull);  // excluded by the caller's bound check
+                    continue;
+                }
+            }
+            atomicAdd(out + (size_t)fs + (size_t)ms * S.P.sab + (size_t)ss * S.P.sab * S.P.sab, x);
         }
     }
 }
@@ -1290,7 +1320,7 @@ int get_plan(iac_ctx *ctx, hipStream_t st, const PairRun &R, int cell_vox, int b
 // Frame chunks inside each group, balanced by pair work: a chunk closes once its
 // frames' n^2 reach fchunk times the mean per frame, or at 4 fchunk frames.
 std::vector<Chunk> make_chunks(const std::vector<uint64_t> &off, const uint32_t *group_off, uint32_t ngroups,
-                               uint32_t fchunk)
+                               uint32_t fchunk, const float *h_rots = nullptr)
 {
     const uint32_t nframes = (uint32_t)off.size() - 1;
     double total = 0;
@@ -1309,7 +1339,10 @@ std::vector<Chunk> make_chunks(const std::vector<uint64_t> &off, const uint32_t 
         for (uint32_t f = group_off[gi]; f < group_off[gi + 1]; f++) {
             const double n = (double)(off[f + 1] - off[f]);
             acc += n * n;
-            if (acc >= budget || f + 1 - f0 >= fmax) {
+            // per-frame rotations: a chunk ends where the next frame's rotation differs
+            const bool rot_break = h_rots && f + 1 < group_off[gi + 1] &&
+                                   std::memcmp(h_rots + 9 * (size_t)f0, h_rots + 9 * (size_t)(f + 1), 36) != 0;
+            if (acc >= budget || f + 1 - f0 >= fmax || rot_break) {
                 chunks.push_back(Chunk{f0, f + 1, gi});
                 f0 = f + 1;
                 acc = 0;
@@ -1341,16 +1374,16 @@ void launch_pairs_box(int pf, bool fast, int walk, dim3 grid, hipStream_t st, co
 {
     if ((O || pf == 1) && fast)
         hipLaunchKernelGGL((pairs_kernel<K, O, 1, K == kKindAc3d && !O, BOX>), grid, dim3(kBlock), 0, st, b.srec,
-                           R.d_frame_off, b.cs, b.d_chunks, b.d_tasks, b.ntasks, b.g, R.spec, target, nvox, dropped, b.d_flag, walk);
+                           R.d_frame_off, b.cs, b.d_chunks, b.d_tasks, b.ntasks, b.g, R.spec, target, nvox, dropped, b.d_flag, walk, R.d_rots);
     else if (O || pf == 1 || BOX != kBoxMax)
         hipLaunchKernelGGL((pairs_kernel<K, O, 1, false, BOX>), grid, dim3(kBlock), 0, st, b.srec, R.d_frame_off,
-                           b.cs, b.d_chunks, b.d_tasks, b.ntasks, b.g, R.spec, target, nvox, dropped, b.d_flag, walk);
+                           b.cs, b.d_chunks, b.d_tasks, b.ntasks, b.g, R.spec, target, nvox, dropped, b.d_flag, walk, R.d_rots);
     else if (pf == 4)
         hipLaunchKernelGGL((pairs_kernel<K, O, O ? 1 : 4, false, BOX>), grid, dim3(kBlock), 0, st, b.srec,
-                           R.d_frame_off, b.cs, b.d_chunks, b.d_tasks, b.ntasks, b.g, R.spec, target, nvox, dropped, b.d_flag, walk);
+                           R.d_frame_off, b.cs, b.d_chunks, b.d_tasks, b.ntasks, b.g, R.spec, target, nvox, dropped, b.d_flag, walk, R.d_rots);
     else
         hipLaunchKernelGGL((pairs_kernel<K, O, O ? 1 : 8, false, BOX>), grid, dim3(kBlock), 0, st, b.srec,
-                           R.d_frame_off, b.cs, b.d_chunks, b.d_tasks, b.ntasks, b.g, R.spec, target, nvox, dropped, b.d_flag, walk);
+                           R.d_frame_off, b.cs, b.d_chunks, b.d_tasks, b.ntasks, b.g, R.spec, target, nvox, dropped, b.d_flag, walk, R.d_rots);
 }
 
 // walk: lanes on consecutive pairs (1) or a contiguous pair range per thread (0)
@@ -1503,7 +1536,7 @@ int pairs_run(iac_ctx *ctx, const PairRun &R)
     std::vector<uint64_t> off(R.nframes + 1);
     IAC_HIP(hipMemcpyAsync(off.data(), R.d_frame_off, off.size() * 8, hipMemcpyDeviceToHost, st));
     IAC_HIP(hipStreamSynchronize(st));
-    const std::vector<Chunk> chunks = make_chunks(off, R.h_group_off, R.ngroups, fchunk_of(ctx));
+    const std::vector<Chunk> chunks = make_chunks(off, R.h_group_off, R.ngroups, fchunk_of(ctx), R.h_rots);
     const int nc = pl->g.ncu * pl->g.ncv;
     const Sizes z = sizes_of(R, nc, chunks.size());
     // scratch: sorted records + cell offsets (slot 1), H + group offsets + self + chunks (slot 2)

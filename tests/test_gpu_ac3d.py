@@ -144,3 +144,63 @@ def test_empty_and_single(ctx, dev):
     c = shape.center[0]
     assert got[c + c * shape.size_ab + c * shape.size_ab ** 2] == 9000
     assert got.sum() == 9000
+
+
+def test_batch_rotations_on_engine(ctx, dev):
+    """Per-frame rotations through the LDS engine (double map, nearest levels): frames
+    bin under the identity and each box voxel is rotated when it flushes.  A run of
+    one repeated rotation (chunks over several frames), identity frames, a distinct
+    rotation per frame, an empty and a one-photon frame: the oracle's bits, the tile
+    kernel's bits, nothing dropped."""
+    import torch
+
+    import incohautocorrelate_amd as iac
+    from incohautocorrelate_amd import workloads as W
+
+    _, km, mq = W.detector(256)
+    shape = iac.mesh_shape_big(mq, 127)
+    d = W.lognormal_photon_frames(256 * 256, 48, 300, 1.0, 10, 3000, seed=8)
+    pix, cnt, off = d["pix"], d["cnt"], d["frame_off"]
+    off = np.sort(np.concatenate([off, [off[20], off[21] + 1]])).astype(np.uint64)
+    nf = off.size - 1
+    rots = W.random_rotations(nf, 17).astype(np.float32)
+    rots[:10] = rots[0]
+    rots[10:15] = np.eye(3, dtype=np.float32).reshape(9)
+    ref, dref = O.ac3d_frames(km, pix, cnt, off, shape, rots=rots)
+    assert dref == 0
+    got = {}
+    try:
+        for kernel in (2, 1):
+            ctx.set_option(iac.OPT_AC3D_KERNEL, kernel)
+            before = ctx.dropped()
+            got[kernel] = mesh_np(gpu_batch(ctx, dev, km, pix, cnt, off, shape, rots=rots))
+            assert ctx.dropped() == before
+    finally:
+        ctx.set_option(iac.OPT_AC3D_KERNEL, 0)
+    assert np.array_equal(got[2], ref)
+    assert np.array_equal(got[1], ref)
+
+
+@pytest.mark.parametrize("i2,dm", [(1, True), (0, False)])
+def test_batch_rotations_engine_refuses(ctx, dev, i2, dm):
+    """Per-frame rotations whose voxel is not a function of the first-level voxel (a
+    linear second level, or no double map) stay on the tile kernel: forcing the LDS
+    engine is a precondition error, and the automatic choice matches the oracle."""
+    import incohautocorrelate_amd as iac
+    from incohautocorrelate_amd import workloads as W
+
+    _, km, mq = W.detector(64)
+    shape = iac.mesh_shape_big(mq, 31)
+    d = W.lognormal_photon_frames(64 * 64, 6, 40, 1.0, 2, 200, seed=3)
+    rots = W.random_rotations(6, 5)
+    ctx.set_option(iac.OPT_AC3D_KERNEL, 2)
+    try:
+        with pytest.raises(iac.IACError):
+            gpu_batch(ctx, dev, km, d["pix"], d["cnt"], d["frame_off"], shape, rots=rots, interp2=i2,
+                      double_map=dm)
+    finally:
+        ctx.set_option(iac.OPT_AC3D_KERNEL, 0)
+    got = mesh_np(gpu_batch(ctx, dev, km, d["pix"], d["cnt"], d["frame_off"], shape, rots=rots, interp2=i2,
+                            double_map=dm))
+    ref, _ = O.ac3d_frames(km, d["pix"], d["cnt"], d["frame_off"], shape, rots=rots, interp2=i2, double_map=dm)
+    assert np.array_equal(got, ref)
