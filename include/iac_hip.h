@@ -135,8 +135,8 @@ int iac_ac3d_sparse(iac_ctx *ctx, const float *d_hitlist4, uint32_t n, const flo
  * d_rots is [nframes][9] floats on the device or NULL for the identity.
  * n_total = d_frame_off[nframes] (host copy, sizes the scratch).  The mesh is
  * identical to calling iac_ac3d_sparse once per frame.  Runs on the LDS
- * cell-box engine; per-frame rotations too when the double map has nearest-
- * neighbour levels and no rotated voxel can leave the mesh (the frames bin
+ * cell-box engine; per-frame rotations too when the double map has a nearest-
+ * neighbour second level and no rotated voxel can leave the mesh (the frames bin
  * under the identity and each box voxel is rotated as it flushes), otherwise
  * on the global-atomic tile kernel. */
 int iac_ac3d_sparse_batch(iac_ctx *ctx, const float *d_kmap, uint32_t npix,

@@ -273,17 +273,19 @@ int launch_tiles(iac_ctx *ctx, const float4 *rec, const uint64_t *frame_off, con
     return IAC_OK;
 }
 
-// Per-frame rotations on the LDS engine.  With a double map and nearest-neighbour
-// levels a pair's voxel is round(R_f round(q vox)): it depends on the pair only
-// through its first-level voxel, so the frames bin under the identity and each box
-// voxel is rotated when it flushes.  Taken when no rotated voxel can leave the mesh
-// (the tile kernel counts such updates pair by pair): a first-level voxel is at most
-// r1 = Max_Q vox (1 + 1e-6) + sqrt(3)/2 long, its rotated component i at most
-// r1 |R_i|_2 plus the float rounding of the three-term row product.
+// Per-frame rotations on the LDS engine.  With a double map and a nearest second
+// level a pair adds, per first-level voxel c (one, or the eight linear corners),
+// FloatToInt(its value) at round(R_f c): c carries the pair, the rotation acts on c
+// alone.  So the frames bin under the identity and each box voxel is rotated when
+// it flushes.  Taken when no rotated voxel can leave the mesh (the tile kernel
+// counts such updates pair by pair): a first-level voxel is at most
+// r1 = Max_Q vox (1 + 1e-6) + sqrt(3)/2 long (sqrt(3) for linear corners), its
+// rotated component i at most r1 |R_i|_2 plus the float rounding of the row product.
 bool rotations_on_engine(const iac::BinParams &P, const std::vector<float> &h, uint32_t nframes)
 {
-    if (!(P.double_map && P.interp1 == IAC_NEAREST && P.interp2 == IAC_NEAREST)) return false;
-    const double r1 = std::sqrt((double)P.cut_sq) * (double)P.vox * (1.0 + 1e-6) + 0.8660254038;
+    if (!(P.double_map && P.interp2 == IAC_NEAREST)) return false;
+    const double r1 = std::sqrt((double)P.cut_sq) * (double)P.vox * (1.0 + 1e-6) +
+                      (P.interp1 == IAC_NEAREST ? 0.8660254038 : 1.7320508076);
     const int lim = std::min({P.c0, P.c1, P.c2, P.sab - 1 - P.c0, P.sab - 1 - P.c1, P.sc - 1 - P.c2});
     for (uint32_t f = 0; f < nframes; f++) {
         for (int i = 0; i < 3; i++) {

@@ -146,12 +146,13 @@ def test_empty_and_single(ctx, dev):
     assert got.sum() == 9000
 
 
-def test_batch_rotations_on_engine(ctx, dev):
-    """Per-frame rotations through the LDS engine (double map, nearest levels): frames
-    bin under the identity and each box voxel is rotated when it flushes.  A run of
-    one repeated rotation (chunks over several frames), identity frames, a distinct
-    rotation per frame, an empty and a one-photon frame: the oracle's bits, the tile
-    kernel's bits, nothing dropped."""
+@pytest.mark.parametrize("i1", [0, 1])
+def test_batch_rotations_on_engine(ctx, dev, i1):
+    """Per-frame rotations through the LDS engine (double map, nearest second level,
+    nearest or linear first level): frames bin under the identity and each box voxel
+    is rotated when it flushes.  A run of one repeated rotation (chunks over several
+    frames), identity frames, a distinct rotation per frame, an empty and a one-photon
+    frame: the oracle's bits, the tile kernel's bits, nothing dropped."""
     import torch
 
     import incohautocorrelate_amd as iac
@@ -166,14 +167,14 @@ def test_batch_rotations_on_engine(ctx, dev):
     rots = W.random_rotations(nf, 17).astype(np.float32)
     rots[:10] = rots[0]
     rots[10:15] = np.eye(3, dtype=np.float32).reshape(9)
-    ref, dref = O.ac3d_frames(km, pix, cnt, off, shape, rots=rots)
+    ref, dref = O.ac3d_frames(km, pix, cnt, off, shape, rots=rots, interp1=i1)
     assert dref == 0
     got = {}
     try:
         for kernel in (2, 1):
             ctx.set_option(iac.OPT_AC3D_KERNEL, kernel)
             before = ctx.dropped()
-            got[kernel] = mesh_np(gpu_batch(ctx, dev, km, pix, cnt, off, shape, rots=rots))
+            got[kernel] = mesh_np(gpu_batch(ctx, dev, km, pix, cnt, off, shape, rots=rots, interp1=i1))
             assert ctx.dropped() == before
     finally:
         ctx.set_option(iac.OPT_AC3D_KERNEL, 0)
