@@ -67,6 +67,7 @@ def parse():
                     "2 range per thread)")
     ap.add_argument("--no-sim", action="store_true", help="skip the simulator leg")
     ap.add_argument("--no-uneven", action="store_true", help="skip the uneven-batch leg")
+    ap.add_argument("--no-rotated", action="store_true", help="skip the per-frame rotation leg")
     ap.add_argument("--no-ac1d", action="store_true", help="skip the AC1D leg")
     ap.add_argument("--no-cq", action="store_true", help="skip the dense C(q) leg")
     ap.add_argument("--no-eval", action="store_true", help="skip the end-to-end iac_eval leg")
@@ -170,6 +171,8 @@ def main():
         cpu = cpu_baseline(kmap, data, shape, ctx, iac, d_kmap, dev, args)
 
     ac1d = None if args.no_ac1d else ac1d_leg(args, iac, W, dev, kmap, d_kmap, d_pix, d_cnt, d_off, data, max_q)
+    rotated = None if args.no_rotated else rotated_leg(args, iac, W, dev, d_kmap, d_pix, d_cnt, d_off, data, shape,
+                                                       updates)
     del d_pix, d_cnt, d_off, d_mesh
     torch.cuda.empty_cache()
     cq = None if args.no_cq else cq_dense_leg(args, iac, W, dev)
@@ -225,6 +228,7 @@ def main():
         "cpu_baseline": cpu,
         "probes": probes,
         "uneven_batch": uneven,
+        "rotated": rotated,
         "ac1d": ac1d,
         "cq_dense": cq,
         "eval_e2e": ev,
@@ -376,6 +380,44 @@ def cpu_baseline(kmap, data, shape, ctx, iac, d_kmap, dev, args):
         "single_thread": {"value": upd1 / dt1, "cores": 1,
                           "sample": f"first {nf1} frames ({upd1} pair-updates), 1 thread"},
     }
+
+
+def rotated_leg(args, iac, W, dev, d_kmap, d_pix, d_cnt, d_off, data, shape, updates):
+    """The headline frames with a distinct uniform random rotation each (the event
+    orientations of RunIAC's frame loop): the LDS engine rotates first-level voxels at
+    the box flush; the global-atomic tile kernel is timed once beside it."""
+    import torch
+
+    nf = len(data["frame_off"]) - 1
+    rots = torch.from_numpy(W.random_rotations(nf, args.seed + 21).astype(np.float32)).to(dev)
+    n_total = int(data["frame_off"][-1])
+    ctx = iac.Context(0)
+    mesh = torch.zeros(shape.size_ab ** 3, dtype=torch.int64, device=dev)
+
+    def run():
+        mesh.zero_()
+        ctx.ac3d_sparse_batch(d_kmap, d_pix, d_cnt, d_off, n_total, shape, mesh, rots=rots)
+
+    def timed(k):
+        run()
+        torch.cuda.synchronize()
+        t = time.perf_counter()
+        for _ in range(k):
+            run()
+        torch.cuda.synchronize()
+        return (time.perf_counter() - t) * 1e3 / k
+
+    ms = timed(max(1, min(args.steps, 3)))
+    total = int(mesh.sum().item())
+    ctx.set_option(iac.OPT_AC3D_KERNEL, 1)
+    ms_tiles = timed(1)
+    same = int(mesh.sum().item()) == total
+    ctx.close()
+    log(f"[rotated] {ms:.2f} ms/step on the engine ({updates / (ms * 1e-3):.4e} pair-updates/s), tile kernel "
+        f"{ms_tiles:.2f} ms, totals equal: {same}")
+    return {"workload": f"the headline frames ({nf}) with a distinct uniform random rotation each",
+            "value": updates / (ms * 1e-3), "unit": "pair-updates/s", "ms_per_step": ms,
+            "tile_kernel_ms": ms_tiles, "totals_equal": same}
 
 
 def _oracle():
