@@ -28,13 +28,21 @@ struct BinParams {
     int dval;
 };
 
+// 1 / (double)0.001f, correctly rounded (the compiler divides in double)
+constexpr double kInvStep = 1.0 / (double)0.001f;
+
 // FloatToInt(float) then the implicit long -> unsigned int of ACMesh.cpp:390-391.
 // For integer-valued p in [0, 21054) the result is exactly 1000*p (checked
-// exhaustively against the formula); other values take the formula itself.
+// exhaustively against the formula); other values take the formula itself, its
+// float quotient p / 0.001f as (float)((double)p * RN(1 / 0.001f)).  The exact
+// quotient of two floats is never within 2^-49 (relative) of a float rounding
+// midpoint and the double product errs by at most 2^-52, so the two round alike
+// whenever the quotient is a normal float; a smaller one (|p| < 1.2e-41) gives 0
+// after the + 0.5f either way.
 __device__ __forceinline__ uint32_t pair_val(float p)
 {
     if (p >= 0.0f && p < 21054.0f && p == truncf(p)) return (uint32_t)p * 1000u;
-    float t = (p / 0.001f) + 0.5f;
+    float t = (float)((double)p * kInvStep) + 0.5f;
     return (uint32_t)(int64_t)floorf(t);
 }
 

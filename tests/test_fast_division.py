@@ -60,3 +60,29 @@ def test_bin_widths_of_real_detectors():
     for size in (4096, 1000, 600):
         _, dq = iac.ac1d_shape(mq, size)
         assert np.array_equal(fast(q, dq).view(np.uint32), exact(q, dq).view(np.uint32))
+
+
+def test_fixed_point_step():
+    """The AC_uw fixed point FloatToInt(float) (IAC/Settings.h:168-175) divides by the
+    step 0.001f; the kernels take the same short division.  Random magnitudes, values
+    around every half-integer quotient (where floor(t + 0.5f) flips), and the full
+    formula against the oracle's C restatement."""
+    import oracle as O
+
+    step = f32(0.001)
+    rng = np.random.default_rng(2)
+    q = (rng.uniform(0.5, 1.0, 2_000_000) * 2.0 ** rng.integers(-40, 40, 2_000_000)).astype(f32)
+    q = np.concatenate([q, -q])
+    assert np.array_equal(fast(q, step).view(np.uint32), exact(q, step).view(np.uint32))
+    # p near (k + 1/2) * step and its neighbours: the quotient lands next to a rounding flip
+    k = rng.integers(0, 4_000_000, 300_000).astype(np.float64)
+    p0 = ((k + 0.5) * f64(step)).astype(f32)
+    for d in range(-3, 4):
+        p = (p0.view(np.int32) + d).view(f32)
+        assert np.array_equal(fast(p, step).view(np.uint32), exact(p, step).view(np.uint32))
+    L = O.lib()
+    sample = np.concatenate([p0[:2000], q[:2000], rng.gamma(1.0, 40.0, 2000).astype(f32)])
+    for p in sample:
+        t = fast(np.array([p], f32), step)[0] + f32(0.5)
+        short = int(np.floor(t).astype(np.int64)) & 0xFFFFFFFF
+        assert L.ora_pair_val(float(p)) == short, p
