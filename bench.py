@@ -68,6 +68,7 @@ def parse():
     ap.add_argument("--no-sim", action="store_true", help="skip the simulator leg")
     ap.add_argument("--no-uneven", action="store_true", help="skip the uneven-batch leg")
     ap.add_argument("--no-rotated", action="store_true", help="skip the per-frame rotation leg")
+    ap.add_argument("--no-cfg1", action="store_true", help="skip the small dense-frame pipeline leg")
     ap.add_argument("--no-ac1d", action="store_true", help="skip the AC1D leg")
     ap.add_argument("--no-cq", action="store_true", help="skip the dense C(q) leg")
     ap.add_argument("--no-eval", action="store_true", help="skip the end-to-end iac_eval leg")
@@ -176,6 +177,7 @@ def main():
     del d_pix, d_cnt, d_off, d_mesh
     torch.cuda.empty_cache()
     cq = None if args.no_cq else cq_dense_leg(args, iac, W, dev)
+    cfg1 = None if args.no_cfg1 else cfg1_leg(args, iac, W, dev)
     ev = None if args.no_eval else eval_leg(args, W)
     probes = None if args.no_probes else probe_leg(iac, dev, nvox)
     uneven = None if args.no_uneven else uneven_leg(args, iac, W, dev)
@@ -229,6 +231,7 @@ def main():
         "probes": probes,
         "uneven_batch": uneven,
         "rotated": rotated,
+        "cfg1": cfg1,
         "ac1d": ac1d,
         "cq_dense": cq,
         "eval_e2e": ev,
@@ -380,6 +383,68 @@ def cpu_baseline(kmap, data, shape, ctx, iac, d_kmap, dev, args):
         "single_thread": {"value": upd1 / dt1, "cores": 1,
                           "sample": f"first {nf1} frames ({upd1} pair-updates), 1 thread"},
     }
+
+
+def cfg1_leg(args, iac, W, dev):
+    """The small dense-frame pipeline (cfg1 shape): 100 dense 128^2 uint32 Poisson(0.01)
+    frames as float32 on the device, sparsified there (CreateSparseHitList, threshold 0,
+    step 1), AC_uw of the hit lists into the 65^3 mesh (NN/NN double map), and C(q) small
+    of the flat-field mask into 65x65x3.  Reported per step as the reference plain
+    method's dense pixel-pair visits per second (16,384^2 per frame): the job the
+    pipeline does, not the pairs it touches.  The mesh is checked against the oracle."""
+    import torch
+
+    n, nf = 128, 100
+    _, km, mq = W.detector(n)
+    shape = iac.mesh_shape_big(mq, 63)
+    s = iac.mesh_shape_small_cofq(mq, 63)
+    d = W.dense_poisson_frames(n * n, nf, 0.01, seed=args.seed + 31)
+    flat = np.ones(n * n, np.float32)
+    mult = iac.cq_small_multiplicator(flat)
+    ctx = iac.Context(0)
+    d_km = torch.from_numpy(km).to(dev)
+    frames = torch.from_numpy(d["frames"].astype(np.float32)).to(dev)
+    d_flat = torch.from_numpy(flat).to(dev)
+    rec = torch.zeros((nf * n * n, 4), dtype=torch.float32, device=dev)
+    off = torch.zeros(nf + 1, dtype=torch.int64, device=dev)
+    mesh = torch.zeros(shape.size_ab ** 3, dtype=torch.int64, device=dev)
+    cqm = torch.zeros(s.size_ab ** 2 * s.size_c, dtype=torch.int64, device=dev)
+
+    def run():
+        mesh.zero_()
+        cqm.zero_()
+        nt = ctx.sparse_hit_list(d_km, frames, None, 0.0, 1.0, rec, off)
+        ctx.ac3d_records_batch(d_km, rec, off, nt, shape, mesh)
+        ctx.cq_small(d_km, d_flat, s, cqm, 0, mult)
+
+    run()
+    torch.cuda.synchronize()
+    steps = max(1, args.steps)
+    t = time.perf_counter()
+    for _ in range(steps):
+        run()
+    torch.cuda.synchronize()
+    ms = (time.perf_counter() - t) * 1e3 / steps
+    ctx.close()
+    visits = float(n * n) ** 2 * nf
+    O = _oracle()
+    t = time.perf_counter()
+    ref, _ = O.ac3d_frames(km, d["pix"], d["cnt"], d["frame_off"], shape, nthreads=_cpu_cores())
+    ref_cq, _ = O.cq_small(flat, km, s, 0, mult, nthreads=_cpu_cores())
+    cpu_s = time.perf_counter() - t
+    parity = bool(np.array_equal(mesh.cpu().numpy().view(np.uint64), ref) and
+                  np.array_equal(cqm.cpu().numpy().view(np.uint64), ref_cq))
+    log(f"[cfg1] {ms:.2f} ms/step ({visits / (ms * 1e-3):.4e} dense pixel-pair visits/s), oracle {cpu_s * 1e3:.1f} ms "
+        f"on {_cpu_cores()} threads, bit exact: {parity}")
+    return {"workload": f"{nf} dense {n}x{n} uint32 Poisson(0.01) frames: device sparsify, AC_uw into "
+                        f"{shape.size_ab}^3 (NN/NN double map), C(q) small of the flat field into "
+                        f"{s.size_ab}x{s.size_ab}x{s.size_c}",
+            "value": visits / (ms * 1e-3), "unit": "dense pixel-pair visits/s (job equivalent)", "ms_per_step": ms,
+            "steps": steps, "bit_exact_vs_oracle": parity,
+            "cpu_baseline": {"value": visits / cpu_s, "unit": "dense pixel-pair visits/s (job equivalent)",
+                             "cores": _cpu_cores(), "kind": "port",
+                             "sample": f"the whole workload (AC_uw of the {nf} hit lists and the flat-field "
+                                       f"C(q)) in {cpu_s:.3f} s; hit lists built outside the clock"}}
 
 
 def rotated_leg(args, iac, W, dev, d_kmap, d_pix, d_cnt, d_off, data, shape, updates):
