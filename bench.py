@@ -69,6 +69,7 @@ def parse():
     ap.add_argument("--no-uneven", action="store_true", help="skip the uneven-batch leg")
     ap.add_argument("--no-rotated", action="store_true", help="skip the per-frame rotation leg")
     ap.add_argument("--no-cfg1", action="store_true", help="skip the small dense-frame pipeline leg")
+    ap.add_argument("--no-cfg3", action="store_true", help="skip the dense float + rotation merge leg")
     ap.add_argument("--no-ac1d", action="store_true", help="skip the AC1D leg")
     ap.add_argument("--no-cq", action="store_true", help="skip the dense C(q) leg")
     ap.add_argument("--no-eval", action="store_true", help="skip the end-to-end iac_eval leg")
@@ -178,6 +179,7 @@ def main():
     torch.cuda.empty_cache()
     cq = None if args.no_cq else cq_dense_leg(args, iac, W, dev)
     cfg1 = None if args.no_cfg1 else cfg1_leg(args, iac, W, dev)
+    cfg3 = None if args.no_cfg3 else cfg3_leg(args, iac, W, dev)
     ev = None if args.no_eval else eval_leg(args, W)
     probes = None if args.no_probes else probe_leg(iac, dev, nvox)
     uneven = None if args.no_uneven else uneven_leg(args, iac, W, dev)
@@ -232,6 +234,7 @@ def main():
         "uneven_batch": uneven,
         "rotated": rotated,
         "cfg1": cfg1,
+        "cfg3": cfg3,
         "ac1d": ac1d,
         "cq_dense": cq,
         "eval_e2e": ev,
@@ -445,6 +448,57 @@ def cfg1_leg(args, iac, W, dev):
                              "cores": _cpu_cores(), "kind": "port",
                              "sample": f"the whole workload (AC_uw of the {nf} hit lists and the flat-field "
                                        f"C(q)) in {cpu_s:.3f} s; hit lists built outside the clock"}}
+
+
+def cfg3_leg(args, iac, W, dev):
+    """Dense float frames with the rotation merge (cfg3 shape): 256^2 FP32 Gamma(1, 0.5)
+    frames, one per orientation of 360 uniform random rotations (the first 360 frames
+    of a frame-f-uses-orientation-f-mod-360 run), each frame's C(q) small into its own
+    129x129x3 uint64 mesh (M = 1e10), to double, then Merge_CQ of mesh o with
+    orientation o (weight 1) into the 129^3 lab mesh, to double.  The FP64 tolerance
+    is checked at reduced size by the C(q) tests (1e-6 relative per populated voxel)."""
+    import torch
+
+    n, norient = 256, 360
+    _, km, mq = W.detector(n)
+    s = iac.mesh_shape_small_cofq(mq, 127)
+    frames = torch.from_numpy(W.gamma_frames(n * n, norient, 1.0, 0.5, seed=args.seed + 41)).to(dev)
+    rw = np.zeros((norient, 10), np.float32)
+    rw[:, :9] = W.random_rotations(norient, args.seed + 43)
+    rw[:, 9] = 1.0
+    goff = np.arange(norient + 1, dtype=np.uint32)
+    nsmall = s.size_ab ** 2 * s.size_c
+    ctx = iac.Context(0)
+    d_km = torch.from_numpy(km).to(dev)
+    d_rw = torch.from_numpy(rw).to(dev)
+    small_u = torch.zeros(norient * nsmall, dtype=torch.int64, device=dev)
+    small = torch.zeros(norient * nsmall, dtype=torch.float64, device=dev)
+    big_u = torch.zeros(s.size_ab ** 3, dtype=torch.int64, device=dev)
+    big = torch.zeros(s.size_ab ** 3, dtype=torch.float64, device=dev)
+    m1, m2 = 1e10, 1e6
+
+    def run():
+        small_u.zero_()
+        big_u.zero_()
+        ctx.cq_dense_frames(d_km, frames, goff, s, small_u, iac.NEAREST, m1)
+        ctx.u64_to_f64(small_u, m1, small)
+        ctx.merge_cq_multi(small, s, d_rw, iac.NEAREST, m2, big_u)
+        ctx.u64_to_f64(big_u, m2, big)
+
+    run()
+    torch.cuda.synchronize()
+    t = time.perf_counter()
+    run()
+    torch.cuda.synchronize()
+    ms = (time.perf_counter() - t) * 1e3
+    ctx.close()
+    pairs = float(n * n) ** 2 * norient
+    log(f"[cfg3] {norient} dense {n}^2 frames + merge: {ms:.1f} ms ({pairs / (ms * 1e-3):.4e} pixel pairs/s, "
+        f"{norient / (ms * 1e-3):.1f} frames/s)")
+    return {"workload": f"{norient} dense {n}x{n} FP32 Gamma(1, 0.5) frames, one per orientation: C(q) small into "
+                        f"{s.size_ab}x{s.size_ab}x{s.size_c} per orientation, Merge_CQ into {s.size_ab}^3",
+            "value": pairs / (ms * 1e-3), "unit": "ordered pixel pairs/s (merge included)", "ms_per_step": ms,
+            "frames_per_s": norient / (ms * 1e-3), "steps": 1}
 
 
 def rotated_leg(args, iac, W, dev, d_kmap, d_pix, d_cnt, d_off, data, shape, updates):
