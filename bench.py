@@ -86,7 +86,8 @@ def main():
     args = parse()
     if args.gpus != 1:
         print(f"bench.py measures one GPU only; --gpus {args.gpus} is not supported "
-              "(frames shard across GPUs by design, see DESIGN.md)", file=sys.stderr)
+              "(frames would shard across GPUs, each accumulating a partial uint64 mesh, then one sum "
+              "reduction)", file=sys.stderr)
         sys.exit(2)
 
     import torch
