@@ -64,7 +64,7 @@ def test_context_without_device():
 
 
 def test_host_library_and_eval_driver():
-    """The C++ host mirror links against the C ABI; the driver starts without a GPU."""
+    """The C++ host mirror links against the C ABI; the iac_eval program starts without a GPU."""
     import ctypes
     import subprocess
     from pathlib import Path
