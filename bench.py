@@ -183,6 +183,13 @@ def main():
     cfg3 = None if args.no_cfg3 else cfg3_leg(args, iac, W, dev)
     ev = None if args.no_eval else eval_leg(args, W)
     probes = None if args.no_probes else probe_leg(iac, dev, nvox)
+    if probes and ac1d:
+        # AC1D bins in LDS: each unordered pair (twice-weighted) is two linear-split LDS adds
+        lds = probes["atomic_u64_per_s"]["lds_box"]
+        for leg in (ac1d, ac1d["cq"]):
+            upairs = leg["value"] / 2 if leg is ac1d else leg["value"]
+            leg["lds_adds_per_s"] = 2 * upairs
+            leg["frac_of_lds_atomic_probe"] = 2 * upairs / lds
     uneven = None if args.no_uneven else uneven_leg(args, iac, W, dev)
     sim = None if args.no_sim else sim_leg(args, iac, W, dev)
 
