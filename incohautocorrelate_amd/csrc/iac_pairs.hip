@@ -589,9 +589,7 @@ __device__ __forceinline__ void fast_walk(const float4 *__restrict__ src, const 
         if (pre[mid] <= w0) lo = mid; else hi = mid;
     }
     uint32_t ub = lo;  // the item holding P (its end is at or after P)
-    for (uint32_t P = w0; P < w1; P += 64) {
-        const bool active = P + lane < w1;
-        const uint32_t p = active ? P + lane : w1 - 1;
+    auto item_at = [&](uint32_t P) -> uint32_t {
         const uint32_t si = ub + 1 + lane;
         const uint32_t t = si < nnz ? pre[si] - P : 64u;
         fl[lane] = 0u;
@@ -599,7 +597,12 @@ __device__ __forceinline__ void fast_walk(const float4 *__restrict__ src, const 
         const uint64_t starts = __ballot(fl[lane] != 0u);
         const uint32_t below = __builtin_amdgcn_mbcnt_hi((uint32_t)(starts >> 32),
                                                          __builtin_amdgcn_mbcnt_lo((uint32_t)starts, 0u));
-        const uint32_t u = ub + below + (uint32_t)((starts >> lane) & 1ull);
+        return ub + below + (uint32_t)((starts >> lane) & 1ull);
+    };
+    uint32_t u = item_at(w0);
+    for (uint32_t P = w0; P < w1; P += 64) {
+        const bool active = P + lane < w1;
+        const uint32_t p = active ? P + lane : w1 - 1;
         const uint4 d = item[u];
         // an inactive lane's item belongs to its unclamped position, not to p: its
         // offset is meaningless, so it decodes pair 0 (and loads nothing)
@@ -614,18 +617,22 @@ __device__ __forceinline__ void fast_walk(const float4 *__restrict__ src, const 
             ib = r - ia * d.w;
         }
         ub = __builtin_amdgcn_readlane(u, 63);
+        float4 ca = make_float4(0.f, 0.f, 0.f, 0.f), cb = ca;  // the pair's records (active lanes)
+        if (active) {
+            ca = src[d.x + ia];
+            cb = src[d.z + ib];
+        }
+        // the next step's item (flag row, ballot) while this step's records load: its LDS
+        // round trips overlap the global-load latency instead of following it
+        if (P + 64 < w1) u = item_at(P + 64);
         if (active && ORD) {
             // ordered runs (linear second level): every corner through the checked box
-            const float4 ca = src[d.x + ia];
-            const float4 cb = src[d.z + ib];
             bin_ordered<KIND>(S, ca, cb, sink);
         } else if (active && KIND == kKindCq && INBOX) {
             // C(q) of a task whose box holds every voxel its pairs reach: the bin of
             // bin_sym (its value and axis order: round(q[ka] vox) + c0 is round_away
             // of the same product plus c0) as one byte offset, component i scaled by
             // the box stride of the mesh axis it maps to
-            const float4 ca = src[d.x + ia];
-            const float4 cb = src[d.z + ib];
             asm volatile("" ::"v"(cb.w), "v"(ca.w));  // 16-byte loads
             const float w = ca.w * cb.w;
             const float q0 = ca.x - cb.x, q1 = ca.y - cb.y, q2 = ca.z - cb.z;
@@ -638,20 +645,14 @@ __device__ __forceinline__ void fast_walk(const float4 *__restrict__ src, const 
             }
         } else if (active && KIND == kKindCq) {
             // C(q): its own value and axis order, through the checked box
-            const float4 ca = src[d.x + ia];
-            const float4 cb = src[d.z + ib];
             int fs, ms, ss;
             uint64_t v;
             if (bin_sym<kKindCq>(S, ca, cb, fs, ms, ss, v)) sink.add(fs, ms, ss, v);
         } else if (active && !FASTBIN) {
-            const float4 ca = src[d.x + ia];
-            const float4 cb = src[d.z + ib];
             int fs, ms, ss;
             uint64_t v;
             if (bin_sym<KIND>(S, ca, cb, fs, ms, ss, v)) sink.add(fs, ms, ss, v);
         } else if (active) {
-            const float4 ca = src[d.x + ia];
-            const float4 cb = src[d.z + ib];
             asm volatile("" ::"v"(cb.w), "v"(ca.w));  // 16-byte loads
             const float q0 = ca.x - cb.x, q1 = ca.y - cb.y, q2 = ca.z - cb.z;
             const float sq = (q0 * q0 + q1 * q1) + q2 * q2;
