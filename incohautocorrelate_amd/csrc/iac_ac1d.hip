@@ -17,6 +17,7 @@
 #include <algorithm>
 #include <cmath>
 #include <numeric>
+#include <type_traits>
 
 #include "iac_fpexact.h"
 #include "iac_internal.h"
@@ -94,7 +95,7 @@ __device__ __forceinline__ void trunc_u64x2(double x1, double x2, unsigned long 
 }
 
 // AutoCorr_CQ_AV over one image: records are {k, I} of every pixel.
-template <bool kFastDiv>
+template <bool kFastDiv, bool kNoDrop>
 __global__ void __launch_bounds__(kBlock)
     ac1d_cq_kernel(const float4 *__restrict__ rec, uint32_t n, uint32_t size, float max_q, float dq, double inv_dq,
                    int interp, double mult, unsigned long long *__restrict__ hist,
@@ -127,32 +128,46 @@ __global__ void __launch_bounds__(kBlock)
         const uint32_t a = ta * kTile + lane_a;
         if (a >= n) continue;
         const float4 ra = rec[a];
+        // pairs j >= i of the image with I_i I_j >= 1e-37 and |q| <= Max_Q.  An edge tile (the
+        // diagonal, or the ragged last column) tests b; an interior one holds 64 x 64 distinct pairs.
+        auto tile = [&](auto edge_c) {
+            constexpr bool kEdge = decltype(edge_c)::value;
 #pragma unroll 4
-        for (int jj = 0; jj < kBPerThread; jj++) {
-            const int bl = bgrp * kBPerThread + jj;
-            const uint32_t b = tb * kTile + bl;
-            const float4 rb = sb[bl];
-            const float w = ra.w * rb.w;
-            float q = qnorm(ra, rb);
-            // pairs j >= i of the image with I_i I_j >= 1e-37 and |q| <= Max_Q
-            if (b >= n || (ta == tb && b < a) || w < 1e-37f || q > max_q) continue;
-            const double val = (double)w;
-            q = div_dq<kFastDiv>(q, dq, inv_dq);
-            const unsigned sh = a != b;  // off-diagonal pairs count twice
-            if (interp == 0) {
-                const uint32_t sc = (uint32_t)floor((double)q + 0.5);
-                const unsigned long long v = trunc_u64(val * mult);
-                if (sc < size) atomicAdd(&h[sc], v << sh); else drop += 1u << sh;
-            } else {
-                const uint32_t sc1 = (uint32_t)floorf(q), sc2 = sc1 + 1;
-                const double sep = (double)(q - floorf(q));
-                const double x1 = (val * (1 - sep)) * mult, x2 = (val * sep) * mult;
-                unsigned long long v1, v2;
-                trunc_u64x2(x1, x2, v1, v2);
-                if (sc1 < size) atomicAdd(&h[sc1], v1 << sh); else drop += 1u << sh;
-                if (sc2 < size) atomicAdd(&h[sc2], v2 << sh); else drop += 1u << sh;
+            for (int jj = 0; jj < kBPerThread; jj++) {
+                const int bl = bgrp * kBPerThread + jj;
+                const uint32_t b = tb * kTile + bl;
+                const float4 rb = sb[bl];
+                const float w = ra.w * rb.w;
+                float q = qnorm(ra, rb);
+                if (kEdge && (b >= n || (ta == tb && b < a))) continue;
+                if (w < 1e-37f || q > max_q) continue;
+                const double val = (double)w;
+                q = div_dq<kFastDiv>(q, dq, inv_dq);
+                const unsigned sh = kEdge ? (a != b) : 1u;  // off-diagonal pairs count twice
+                if (interp == 0) {
+                    const uint32_t sc = (uint32_t)floor((double)q + 0.5);
+                    const unsigned long long v = trunc_u64(val * mult);
+                    if (kNoDrop) atomicAdd(&h[min(sc, size - 1)], v << sh);
+                    else if (sc < size) atomicAdd(&h[sc], v << sh);
+                    else drop += 1u << sh;
+                } else {
+                    const uint32_t sc1 = (uint32_t)floorf(q), sc2 = sc1 + 1;
+                    const double sep = (double)(q - floorf(q));
+                    const double x1 = (val * (1 - sep)) * mult, x2 = (val * sep) * mult;
+                    unsigned long long v1, v2;
+                    trunc_u64x2(x1, x2, v1, v2);
+                    if (kNoDrop) {
+                        const uint32_t s1 = min(sc1, size - 2);
+                        atomicAdd(&h[s1], v1 << sh);
+                        atomicAdd(&h[s1 + 1], v2 << sh);
+                    } else {
+                        if (sc1 < size) atomicAdd(&h[sc1], v1 << sh); else drop += 1u << sh;
+                        if (sc2 < size) atomicAdd(&h[sc2], v2 << sh); else drop += 1u << sh;
+                    }
+                }
             }
-        }
+        };
+        if (ta == tb || tb + 1 == nt) tile(std::true_type{}); else tile(std::false_type{});
     }
     __syncthreads();
     for (uint32_t i = threadIdx.x; i < size; i += kBlock)
@@ -216,7 +231,7 @@ __global__ void gather_image_kernel(const float *__restrict__ kmap, const float 
 
 // AC_uw: photon pairs j <= k of every frame.  kExact: integer-valued weights
 // accumulate in uint64 (nearest neighbour); otherwise double.
-template <bool kExact, bool kFastDiv>
+template <bool kExact, bool kFastDiv, bool kNoDrop>
 __global__ void __launch_bounds__(kBlock)
     ac1d_acuw_kernel(const float4 *__restrict__ rec, const uint64_t *__restrict__ frame_off,
                      const uint64_t *__restrict__ tile_off, uint32_t nframes, uint32_t size, float max_q, float dq,
@@ -266,33 +281,46 @@ __global__ void __launch_bounds__(kBlock)
         const uint32_t a = ta * kTile + lane_a;
         if (a >= n) continue;
         const float4 ra = rec[base + a];
+        // an edge tile (the diagonal, or the frame's ragged last column) tests b
+        auto tile = [&](auto edge_c) {
+            constexpr bool kEdge = decltype(edge_c)::value;
 #pragma unroll 4
-        for (int jj = 0; jj < kBPerThread; jj++) {
-            const int bl = bgrp * kBPerThread + jj;
-            const uint32_t b = tb * kTile + bl;
-            const float4 rb = sb[bl];
-            float q = qnorm(ra, rb);
-            if (b >= n || (ta == tb && b < a) || q > max_q) continue;
-            q = div_dq<kFastDiv>(q, dq, inv_dq);
-            const float f2 = (a == b) ? self_factor : 2.0f;
-            if (interp == 0) {
-                const uint32_t sc = (uint32_t)floor((double)q + 0.5);
-                const float v = f2 * (ra.w * rb.w);
-                if (sc < size) {
-                    if (kExact) atomicAdd(&hu[sc], (unsigned long long)v);
-                    else atomicAdd(&hd[sc], (double)v);
+            for (int jj = 0; jj < kBPerThread; jj++) {
+                const int bl = bgrp * kBPerThread + jj;
+                const uint32_t b = tb * kTile + bl;
+                const float4 rb = sb[bl];
+                float q = qnorm(ra, rb);
+                if (kEdge && (b >= n || (ta == tb && b < a))) continue;
+                if (q > max_q) continue;
+                q = div_dq<kFastDiv>(q, dq, inv_dq);
+                const float f2 = (kEdge && a == b) ? self_factor : 2.0f;
+                if (interp == 0) {
+                    uint32_t sc = (uint32_t)floor((double)q + 0.5);
+                    const float v = f2 * (ra.w * rb.w);
+                    if (kNoDrop || sc < size) {
+                        if (kNoDrop) sc = min(sc, size - 1);
+                        if (kExact) atomicAdd(&hu[sc], (unsigned long long)v);
+                        else atomicAdd(&hd[sc], (double)v);
+                    } else {
+                        drop++;
+                    }
                 } else {
-                    drop++;
+                    const uint32_t sc1 = (uint32_t)floorf(q), sc2 = sc1 + 1;
+                    const float sep = q - (float)sc1;
+                    const float v1 = f2 * ((ra.w * rb.w) * (1 - sep));
+                    const float v2 = f2 * ((ra.w * rb.w) * sep);
+                    if (kNoDrop) {
+                        const uint32_t s1 = min(sc1, size - 2);
+                        atomicAdd(&hd[s1], (double)v1);
+                        atomicAdd(&hd[s1 + 1], (double)v2);
+                    } else {
+                        if (sc1 < size) atomicAdd(&hd[sc1], (double)v1); else drop++;
+                        if (sc2 < size) atomicAdd(&hd[sc2], (double)v2); else drop++;
+                    }
                 }
-            } else {
-                const uint32_t sc1 = (uint32_t)floorf(q), sc2 = sc1 + 1;
-                const float sep = q - (float)sc1;
-                const float v1 = f2 * ((ra.w * rb.w) * (1 - sep));
-                const float v2 = f2 * ((ra.w * rb.w) * sep);
-                if (sc1 < size) atomicAdd(&hd[sc1], (double)v1); else drop++;
-                if (sc2 < size) atomicAdd(&hd[sc2], (double)v2); else drop++;
             }
-        }
+        };
+        if (ta == tb || tb + 1 == nt) tile(std::true_type{}); else tile(std::false_type{});
     }
     __syncthreads();
     for (uint32_t i = threadIdx.x; i < size; i += kBlock) {
@@ -335,6 +363,16 @@ bool fast_div_ok(float max_q, float dq)
 {
     return std::isnormal(dq) && dq > 0.f && dq <= 0x1p51f && (double)max_q <= (double)dq * 0x1p126;
 }
+
+// Every binned |q| <= Max_Q has q / dq <= fl(Max_Q / dq) (rounding is monotone); below
+// size - 1 no bin (nearest, or both linear ones) can leave the histogram, so the
+// kernels skip the range test (AC1D::Initialize's dq = Max_Q / (Size - 2) always
+// passes).  They still clamp, which changes nothing then and keeps a NaN k in range.
+bool no_drop(float max_q, float dq, uint32_t size)
+{
+    const float r = max_q / dq;
+    return size >= 2 && r >= 0.f && r < (float)(size - 1);
+}
 // dynamic LDS of the AC_uw kernel: the histogram, then one staged tile of records
 size_t acuw_lds(uint32_t size) { return (size_t)(size + 1) / 2 * sizeof(float4) + kTile * sizeof(float4); }
 
@@ -358,10 +396,12 @@ int acuw_run(iac_ctx *ctx, const float4 *rec, const uint64_t *d_frame_off, uint3
     hipLaunchKernelGGL(tile_scan_kernel, dim3(1), dim3(1024), 0, ctx->stream, d_frame_off, nframes, toff);
     IAC_HIP(hipGetLastError());
     const float sf = (float)self_pair_factor;
-    const bool fdiv = fast_div_ok(max_q, dq);
+    const bool fdiv = fast_div_ok(max_q, dq), nd = no_drop(max_q, dq, size);
     const double inv_dq = 1.0 / (double)dq;
-    auto kern = exact ? (fdiv ? ac1d_acuw_kernel<true, true> : ac1d_acuw_kernel<true, false>)
-                      : (fdiv ? ac1d_acuw_kernel<false, true> : ac1d_acuw_kernel<false, false>);
+    auto kern = exact ? (fdiv ? (nd ? ac1d_acuw_kernel<true, true, true> : ac1d_acuw_kernel<true, true, false>)
+                              : (nd ? ac1d_acuw_kernel<true, false, true> : ac1d_acuw_kernel<true, false, false>))
+                      : (fdiv ? (nd ? ac1d_acuw_kernel<false, true, true> : ac1d_acuw_kernel<false, true, false>)
+                              : (nd ? ac1d_acuw_kernel<false, false, true> : ac1d_acuw_kernel<false, false, false>));
     IAC_TRY(iac::timer_begin(ctx));
     if (exact) hipLaunchKernelGGL(zero_u64_kernel, dim3(1), dim3(256), 0, ctx->stream, hu, size);
     hipLaunchKernelGGL(kern, dim3(persistent_blocks(ctx)), dim3(kBlock), acuw_lds(size), ctx->stream, rec,
@@ -419,7 +459,9 @@ int iac_ac1d_cq(iac_ctx *ctx, const float *d_kmap, uint32_t npix, const float *d
                        ctx->stream, d_kmap, d_intensity, npix, stride, static_cast<float4 *>(rec));
     IAC_HIP(hipGetLastError());
     IAC_TRY(iac::timer_begin(ctx));
-    auto kern = fast_div_ok(max_q, dq) ? ac1d_cq_kernel<true> : ac1d_cq_kernel<false>;
+    const bool fdiv = fast_div_ok(max_q, dq), nd = no_drop(max_q, dq, size);
+    auto kern = fdiv ? (nd ? ac1d_cq_kernel<true, true> : ac1d_cq_kernel<true, false>)
+                     : (nd ? ac1d_cq_kernel<false, true> : ac1d_cq_kernel<false, false>);
     hipLaunchKernelGGL(kern, dim3(persistent_blocks(ctx)), dim3(kBlock), acuw_lds(size), ctx->stream,
                        static_cast<const float4 *>(rec), npix, size, max_q, dq, 1.0 / (double)dq, interp, mult,
                        reinterpret_cast<unsigned long long *>(d_hist), ctx->d_counters);

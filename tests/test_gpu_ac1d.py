@@ -145,3 +145,49 @@ def test_short_sqrt_exhaustive(ctx):
     """The AC1D kernels' |q| square root against the library sqrtf on every one of
     the 2^32 float bit patterns (zero, subnormals, the small-input branch, inf, NaN)."""
     assert ctx.selfcheck_sqrt(0, 1 << 32) == 0
+
+
+@pytest.mark.parametrize("interp", [0, 1])
+def test_ac1d_cq_and_acuw_with_drops(ctx, dev, interp):
+    """A bin width half of AC1D::Initialize's: pairs past the histogram are dropped and
+    counted (the range-checked kernels), both for C(q) and for AC_uw, against the oracle."""
+    import torch
+
+    import incohautocorrelate_amd as iac
+
+    n = 80
+    _, km, mq = W.detector(n)
+    size = 500
+    max_q, dq0 = iac.ac1d_shape(mq, size)
+    dq = float(np.float32(dq0 * 0.5))
+    rng = np.random.default_rng(40 + interp)
+    I = rng.gamma(1.0, 0.5, n * n).astype(np.float32)
+    mult = iac.ac1d_cq_multiplicator(I)
+    ref, dref = O.ac1d_cq(I, km, size, max_q, dq, interp, mult)
+    assert dref > 0
+    h = torch.zeros(size, dtype=torch.int64, device=dev)
+    before = ctx.dropped()
+    ctx.ac1d_cq(to_dev(km, dev), to_dev(I, dev), size, max_q, dq, interp, mult, h)
+    assert np.array_equal(mesh_np(h), ref)
+    assert ctx.dropped() - before == dref
+
+    d = W.lognormal_photon_frames(n * n, 8, 150, 1.0, 2, 600, seed=9)
+    d["cnt"] = rng.integers(1, 4, d["cnt"].size).astype(np.uint16)
+    off = d["frame_off"]
+    ref2 = np.zeros(size)
+    dref2 = 0
+    for f in range(off.size - 1):
+        hl = O.hit_list_from_photons(km, d["pix"][off[f]:off[f + 1]], d["cnt"][off[f]:off[f + 1]])
+        ref2, dd = O.ac1d_acuw(hl, size, max_q, dq, interp, 2, ac=ref2)
+        dref2 += dd
+    assert dref2 > 0
+    h2 = torch.zeros(size, dtype=torch.float64, device=dev)
+    before = ctx.dropped()
+    ctx.ac1d_acuw(to_dev(km, dev), to_dev(d["pix"], dev), to_dev(d["cnt"], dev), to_dev(off, dev), int(off[-1]),
+                  size, max_q, dq, interp, h2)
+    got2 = h2.cpu().numpy()
+    if interp == 0:
+        assert np.array_equal(got2, ref2)
+    else:
+        assert np.allclose(got2, ref2, rtol=1e-12, atol=0)
+    assert ctx.dropped() - before == dref2
