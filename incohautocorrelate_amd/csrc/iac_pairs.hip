@@ -30,6 +30,7 @@
 #include <algorithm>
 #include <cmath>
 #include <cstring>
+#include <type_traits>
 #include <vector>
 
 #include "iac_binning.h"
@@ -141,6 +142,15 @@ __device__ __forceinline__ uint32_t pair_val_small(float p)
 {
     if (p >= 0.0f && p < 21054.0f && p == truncf(p)) return __umul24((uint32_t)p, 1000u);
     return pair_val(p);
+}
+
+// AC_uw with a double map, a linear first level and the identity rotation: a pair's
+// eight first-level corners are its voxels.  A linear second level over such an
+// integral corner keeps one corner of weight 1 (Sep = 1), valued FloatToInt(double)
+// of the corner's float value; a nearest one takes FloatToInt(float).
+__host__ __device__ __forceinline__ bool lin_first_identity(const BinParams &P)
+{
+    return P.double_map && P.interp1 == 1 && P.rot_identity;
 }
 
 // the value of a symmetric AC_uw pair: FloatToInt(float), or FloatToInt(double)
@@ -625,7 +635,36 @@ __device__ __forceinline__ void fast_walk(const float4 *__restrict__ src, const 
         // the next step's item (flag row, ballot) while this step's records load: its LDS
         // round trips overlap the global-load latency instead of following it
         if (P + 64 < w1) u = item_at(P + 64);
-        if (active && ORD) {
+        if (active && ORD && KIND == kKindAc3d && INBOX) {
+            // a linear first level (identity) on a task whose box holds every corner: add_q_entry's
+            // eight corners at one base byte offset plus constant strides, their values
+            // ((v wx) wy) wz from a product tree in the same order
+            const float q0 = ca.x - cb.x, q1 = ca.y - cb.y, q2 = ca.z - cb.z;
+            const float sq = (q0 * q0 + q1 * q1) + q2 * q2;
+            if (sq < cut_sq) {
+                const float x0 = q0 * vox, x1 = q1 * vox, x2 = q2 * vox;
+                const float f0 = floorf(x0), f1 = floorf(x1), f2 = floorf(x2);
+                const double s0 = 1.0 - (double)(x0 - f0), s1 = 1.0 - (double)(x1 - f1);
+                const double s2 = 1.0 - (double)(x2 - f2);
+                const double v = (double)(ca.w * cb.w);
+                const double vx[2] = {v * s0, v * (1 - s0)};
+                const int o = __mul24((int)f2, bxby8) + (__mul24((int)f1, bx8) + ((int)f0 * 8 + c8));
+                auto corners = [&](auto dval_c) {
+                    constexpr bool kDouble = decltype(dval_c)::value;  // linear second level
+#pragma unroll
+                    for (int c = 0; c < 8; c++) {
+                        const int ex = c & 1, ey = (c >> 1) & 1, ez = (c >> 2) & 1;
+                        const double w = (vx[ex] * (ey ? 1 - s1 : s1)) * (ez ? 1 - s2 : s2);
+                        const uint32_t u = kDouble ? pair_val_d((double)(float)w) : pair_val((float)w);
+                        if (u)
+                            atomicAdd(
+                                reinterpret_cast<unsigned long long *>(boxb + o + ex * 8 + ey * bx8 + ez * bxby8),
+                                (unsigned long long)u);
+                    }
+                };
+                if (S.P.interp2) corners(std::true_type{}); else corners(std::false_type{});
+            }
+        } else if (active && ORD) {
             // ordered runs (linear second level): every corner through the checked box
             bin_ordered<KIND>(S, ca, cb, sink);
         } else if (active && KIND == kKindCq && INBOX) {
@@ -684,7 +723,8 @@ __device__ __forceinline__ void fast_walk_w(bool small, bool inbox, const float4
                                             BoxSink &sink, lds_flag_t *fl)
 {
     if (!FAST) {
-        if (KIND == kKindCq && !ORD && S.interp == 0 && inbox)
+        if (((KIND == kKindCq && !ORD && S.interp == 0) || (KIND == kKindAc3d && ORD && lin_first_identity(S.P))) &&
+            inbox)
             fast_walk<KIND, ORD, false, false, true>(src, item, pre, nnz, w0, w1, tri, big, S, box, T, sink, fl);
         else
             fast_walk<KIND, ORD, false, false, false>(src, item, pre, nnz, w0, w1, tri, big, S, box, T, sink, fl);
@@ -730,7 +770,9 @@ __global__ void __launch_bounds__(kBlock, (Occupancy<FAST, BOX>::blocks))
     const int fox = S.P.c0 - T.bx0, foy = S.P.c1 - T.by0, foz = S.P.c2 - T.bz0;  // fast path offsets
     const uint32_t fbxby = (uint32_t)(T.bx * T.by);
     // in-box tasks: the AC_uw specialised path, or symmetric nearest-neighbour C(q) (tight boxes)
-    const bool inbox = (FAST || (KIND == kKindCq && !ORD && S.interp == 0)) && T.inbox != 0;
+    const bool inbox = (FAST || (KIND == kKindCq && !ORD && S.interp == 0) ||
+                        (KIND == kKindAc3d && ORD && lin_first_identity(S.P))) &&
+                       T.inbox != 0;
     const bool small = FAST && wflag[0] != 0u;
     // in-box C(q) on a pair range per thread: byte strides of the box axis each record
     // component bins along, and the centre's byte offset in the box

@@ -176,8 +176,9 @@ def test_uneven_lognormal_full_size(ctx, dev):
 
 @pytest.mark.parametrize("mode,cell,box", [(1, 5, 4096), (1, 4, 1920), (2, 4, 4096), (1, 3, 1024)])
 def test_linear_modes_pair_walks(ctx, dev, mode, cell, box):
-    """Linear AC_uw (ordered pairs, 8 or 64 corner adds each) through the engine with
-    lanes on consecutive pairs and with a pair range per thread: the oracle's bits."""
+    """Linear AC_uw (ordered pairs, 8 or 64 corner adds each; a linear first level with a
+    nearest second level bins in-box tasks inline) through the engine with lanes on
+    consecutive pairs and with a pair range per thread: the oracle's bits."""
     import torch
 
     import incohautocorrelate_amd as iac
@@ -188,7 +189,7 @@ def test_linear_modes_pair_walks(ctx, dev, mode, cell, box):
     data = W.lognormal_photon_frames(n * n, 12, 50, 1.0, 1, 400, seed=31)
     n_total = int(data["frame_off"][-1])
     ref = {}
-    for i1, i2 in ((1, 1), (0, 1)):
+    for i1, i2 in ((1, 1), (0, 1), (1, 0)):
         ref[i1, i2], _ = O.ac3d_frames(km, data["pix"], data["cnt"], data["frame_off"], shape, None, i1, i2, True)
     ctx.set_option(iac.OPT_AC3D_KERNEL, 2)
     ctx.set_option(iac.OPT_PAIR_MODE, mode)
