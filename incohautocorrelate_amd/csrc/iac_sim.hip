@@ -216,6 +216,10 @@ __global__ void __launch_bounds__(256)
 // over 1024-pixel chunks in order (one thread per chunk), then over the chunks in
 // order; x (float)(n_expected / sum); Poisson draw from stream 0; white noise
 // (2u - 1) * adu_noise in double from stream 1; x value_per_photon in float.
+// kVec4: every frame starts 16-byte aligned (npix % 4 == 0, aligned base), so a chunk
+// is read as float4 and its four values added in order: the same sum, a quarter of the
+// loads (each lane's loads walk its own 4 KB chunk, so every wave load touches 64 lines)
+template <bool kVec4>
 __global__ void __launch_bounds__(256)
     chunk_sums_kernel(const float *__restrict__ frames, uint32_t npix, uint32_t nch, double *__restrict__ part)
 {
@@ -224,7 +228,18 @@ __global__ void __launch_bounds__(256)
     const float *I = frames + (size_t)f * npix;
     double s = 0.0;
     const uint32_t e = min(npix, (c + 1) * 1024);
-    for (uint32_t l = c * 1024; l < e; l++) s += I[l];
+    uint32_t l = c * 1024;
+    if (kVec4) {
+        const float4 *I4 = reinterpret_cast<const float4 *>(I);
+        for (; l + 4 <= e; l += 4) {
+            const float4 v = I4[l / 4];
+            s += v.x;
+            s += v.y;
+            s += v.z;
+            s += v.w;
+        }
+    }
+    for (; l < e; l++) s += I[l];
     part[(size_t)f * nch + c] = s;
 }
 
@@ -473,8 +488,9 @@ int iac_sim_postprocess(iac_ctx *ctx, float *d_frames, uint32_t npix, uint32_t n
     IAC_TRY(iac::scratch(ctx, 2, (size_t)nframes * nch * sizeof(double) + (size_t)nframes * sizeof(float), &sp));
     double *part = static_cast<double *>(sp);
     float *fac = reinterpret_cast<float *>(part + (size_t)nframes * nch);
-    hipLaunchKernelGGL(chunk_sums_kernel, dim3((nch + 255) / 256, nframes), dim3(256), 0, ctx->stream, d_frames,
-                       npix, nch, part);
+    const bool vec4 = npix % 4 == 0 && reinterpret_cast<uintptr_t>(d_frames) % 16 == 0;
+    hipLaunchKernelGGL(vec4 ? chunk_sums_kernel<true> : chunk_sums_kernel<false>, dim3((nch + 255) / 256, nframes),
+                       dim3(256), 0, ctx->stream, d_frames, npix, nch, part);
     hipLaunchKernelGGL(frame_factor_kernel, dim3((nframes + 255) / 256), dim3(256), 0, ctx->stream, part, nch,
                        nframes, n_expected, fac);
     hipLaunchKernelGGL(postprocess_kernel, dim3((npix + kPostSlice - 1) / kPostSlice, nframes), dim3(256), 0,
