@@ -236,6 +236,9 @@ def main():
             "bytes_per_update": BYTES_PER_UPDATE,
             "frac_of_measured_copy": achieved / probes["hbm_copy_GBps"] if probes else None,
             "updates_vs_uniform_u64_atomics": value / probes["atomic_u64_per_s"]["uniform"] if probes else None,
+            # the kernel's own update rate against random uint64 adds into a 4096-voxel LDS box
+            "frac_of_lds_atomic_probe": (updates / (k_avg_ms * 1e-3) / probes["atomic_u64_per_s"]["lds_box"]
+                                         if probes and k_avg_ms > 0 else None),
         },
         "cpu_baseline": cpu,
         "probes": probes,
