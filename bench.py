@@ -72,6 +72,7 @@ def parse():
     ap.add_argument("--no-cfg3", action="store_true", help="skip the dense float + rotation merge leg")
     ap.add_argument("--no-ac1d", action="store_true", help="skip the AC1D leg")
     ap.add_argument("--no-cq", action="store_true", help="skip the dense C(q) leg")
+    ap.add_argument("--no-ppp", action="store_true", help="skip the LAP photon finder leg")
     ap.add_argument("--no-eval", action="store_true", help="skip the end-to-end iac_eval leg")
     ap.add_argument("--no-probes", action="store_true", help="skip the copy / atomic ceiling probes")
     ap.add_argument("--uneven-only", action="store_true", help="run only the uneven-batch leg")
@@ -190,6 +191,7 @@ def main():
             upairs = leg["value"] / 2 if leg is ac1d else leg["value"]
             leg["lds_adds_per_s"] = 2 * upairs
             leg["frac_of_lds_atomic_probe"] = 2 * upairs / lds
+    ppp = None if args.no_ppp else ppp_leg(args, iac, dev, probes)
     uneven = None if args.no_uneven else uneven_leg(args, iac, W, dev)
     sim = None if args.no_sim else sim_leg(args, iac, W, dev)
 
@@ -248,6 +250,7 @@ def main():
         "cfg3": cfg3,
         "ac1d": ac1d,
         "cq_dense": cq,
+        "ppp_lap": ppp,
         "eval_e2e": ev,
         "simulator": sim,
     }
@@ -733,6 +736,79 @@ def cq_dense_leg(args, iac, W, dev):
                         f"{s.size_ab}x{s.size_ab}x{s.size_c} uint64 mesh, nearest neighbour",
             "value": pairs / (ms * 1e-3), "unit": "ordered pixel pairs/s", "ms_per_step": ms, "steps": steps,
             "pairs_per_step": pairs, "tuned": tuned, "cpu_baseline": cpu}
+
+
+def ppp_leg(args, iac, dev, probes):
+    """LAP photon finder chain (PPP::ProcessData_PF_LAP per frame: pixel mask, offset and
+    gain correction, largest-adjacent-pixel photon finder, photon counts) over 500 frames
+    of a 1024x1024 detector laid out as eight 1024x128 modules; 37 ADU per photon."""
+    import torch
+
+    side, nf, adu = 1024, 500, 37.0
+    npix = side * side
+    panels = [(k * side * 128, side, 128) for k in range(8)]
+    g = torch.Generator(device=dev)
+    g.manual_seed(args.seed + 13)
+    frames = torch.poisson(torch.full((nf, npix), 0.4, device=dev), generator=g)
+    frames.add_(torch.rand((nf, npix), device=dev, generator=g) * 1.3 - 0.3).mul_(adu)
+    mask = (torch.rand(npix, device=dev, generator=g) > 0.05).to(torch.int32)
+    offset = torch.rand(npix, device=dev, generator=g) * 3.0
+    gain = 0.8 + 0.4 * torch.rand(npix, device=dev, generator=g)
+    out = torch.empty_like(frames)
+    pc = torch.empty(nf, dtype=torch.int32, device=dev)
+    mi = torch.empty(nf, dtype=torch.float32, device=dev)
+    ctx = iac.Context(0)
+
+    def run():
+        ctx.ppp_lap(frames, out, panels, adu, mask=mask, offset=offset, gain=gain, photon_count=pc,
+                    mean_intensity=mi)
+
+    run()
+    torch.cuda.synchronize()
+    steps = max(1, min(args.steps, 5))
+    ctx.enable_timing(True)
+    ctx.kernel_time(reset=True)
+    t = time.perf_counter()
+    for _ in range(steps):
+        run()
+    torch.cuda.synchronize()
+    ms = (time.perf_counter() - t) * 1e3 / steps
+    kms, kn = ctx.kernel_time(reset=True)
+    ctx.enable_timing(False)
+    k_ms = kms / max(kn, 1)
+    # bit-exact check of the first two frames against the oracle
+    O = _oracle()
+    h_in = frames[:2].cpu().numpy()
+    ref, rpc, _ = O.ppp_lap(h_in, panels, adu=adu, mask=mask.cpu().numpy(), offset=offset.cpu().numpy(),
+                            gain=gain.cpu().numpy())
+    exact = bool(np.array_equal(out[:2].cpu().numpy().view(np.int32), ref.view(np.int32))
+                 and np.array_equal(pc[:2].cpu().numpy(), rpc))
+    ctx.close()
+    nbytes = nf * npix * 8  # each frame pixel read once and written once
+    gbs = nbytes / (k_ms * 1e-3) / 1e9
+    log(f"[ppp] LAP {side}^2 x {nf} frames: {ms:.3f} ms/step (finder kernel {k_ms:.3f} ms, {gbs:.0f} GB/s), "
+        f"oracle match {exact}")
+    cpu = None
+    if not args.no_cpu:
+        h = frames[:1].cpu().numpy()
+        hm, ho, hg = mask.cpu().numpy(), offset.cpu().numpy(), gain.cpu().numpy()
+        n, t0 = 0, time.perf_counter()
+        while n == 0 or (time.perf_counter() - t0 < min(5.0, args.cpu_seconds / 5) and n < 200):
+            O.ppp_lap(h, panels, adu=adu, mask=hm, offset=ho, gain=hg)
+            n += 1
+        cpu = {"value": n / (time.perf_counter() - t0), "unit": "frames/s", "cores": 1, "kind": "port",
+               "sample": f"{n} frames of {side}x{side}, one thread (the reference loops frames serially)"}
+    del frames, out
+    torch.cuda.empty_cache()
+    return {"workload": f"LAP photon finder chain: {side}x{side} detector as eight {side}x128 modules, {nf} FP32 "
+                        f"frames, pixel mask, offset and gain maps, {adu} ADU per photon, per-event photon counts",
+            "value": nf / (ms * 1e-3), "unit": "frames/s", "ms_per_step": ms, "steps": steps,
+            "bit_exact_first_frames": exact,
+            "roofline": {"bound": "hbm", "kernel": "lap_kernel", "kernel_avg_ms": k_ms, "achieved": gbs,
+                         "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": gbs / HBM_PEAK_GBS,
+                         "bytes_per_pixel": 8,
+                         "frac_of_measured_copy": gbs / probes["hbm_copy_GBps"] if probes else None},
+            "cpu_baseline": cpu}
 
 
 def eval_leg(args, W):

@@ -202,6 +202,34 @@ int iac_intensity_accumulate(iac_ctx *ctx, const float *d_frames, uint32_t nfram
 int iac_intensity_finish(iac_ctx *ctx, uint32_t npix, uint32_t ntotal_frames, float step, const float *d_sum,
                          const int64_t *d_tally, float *d_avg);
 
+/* ---- pattern pre-processing: largest-adjacent-pixel photon finder --------------- */
+/* PPP::DetectorPanel, IAC/PPP.h:14-18 (Scans = {fs, ss}): pixel first_ind + row*fs + col. */
+typedef struct {
+    uint32_t first_ind;
+    uint32_t fs;
+    uint32_t ss;
+} iac_panel;
+/* The per-frame chain of PPP::ProcessData_PF_LAP, IAC/PPP.cpp:469-495, over nframes dense
+ * frames ([nframes][npix] floats; d_out may equal d_in, then the call stages the input in
+ * its scratch memory; other overlaps are not allowed):
+ *  - Detector::ApplyPixelMask (IAC/Detector.cpp:662-665) when d_mask ([npix] ints) is given;
+ *  - PPP::GainCorrection (IAC/PPP.cpp:781-803) when d_offset and d_gain ([npix] floats, the
+ *    gain file's maps) are given, with AllowNegativeValues = gain_only as the reference
+ *    passes it;
+ *  - unless gain_only, PPP::PhotonFinder_LargestAdjacentPixel (IAC/PPP.cpp:18-95) over the
+ *    npanels panels (host array, at most 128, disjoint, inside the first sum(fs*ss) pixels
+ *    of the frame, as the reference's arrays are sized); pixels outside every panel keep
+ *    their corrected value;
+ *  - per event (optional, device arrays of nframes): PhotonCount = (int) serial float sum
+ *    of the frame and MeanIntensity = PhotonCount / npix (IAC/PPP.cpp:490-491).
+ * Replaces the in-place PPP::PhotonFinder_LargestAdjacentPixel(float *Intensity, ...)
+ * (IAC/PPP.h:90) and the frame loop of PPP::ProcessData_PF_LAP (IAC/PPP.h:95); file reads
+ * and writes stay with the caller. */
+int iac_ppp_lap(iac_ctx *ctx, const float *d_in, float *d_out, uint32_t nframes, uint32_t npix, const int *d_mask,
+                const float *d_offset, const float *d_gain, const iac_panel *panels, uint32_t npanels,
+                float adu_per_photon, float seed_threshold, float combined_threshold, int gain_only,
+                int *d_photon_count, float *d_mean_intensity);
+
 /* ---- AC1D: angular-averaged correlation ---------------------------------------- */
 /* AC1D::Initialize(Det, ArraySize, QZoom), IAC/AC1D.cpp:53-58 (host). */
 int iac_ac1d_shape(const float max_q[3], uint32_t size, float q_zoom, float *max_q_out, float *dq_out);

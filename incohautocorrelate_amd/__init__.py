@@ -12,10 +12,10 @@ import weakref
 
 import numpy as np
 
-from ._abi import IACError, MeshShape, check, fptr, lib
+from ._abi import IACError, MeshShape, Panel, check, fptr, lib
 
 __all__ = [
-    "IACError", "MeshShape", "lib", "generate_pixel_map", "calc_kmap", "mesh_shape_big",
+    "IACError", "MeshShape", "Panel", "lib", "generate_pixel_map", "calc_kmap", "mesh_shape_big",
     "mesh_shape_small_cofq", "Context", "SimPlan", "NEAREST", "LINEAR",
 ]
 
@@ -277,6 +277,18 @@ class Context:
         check(lib().iac_average_intensity(self._h, _ptr(frames), nframes, npix, _ptr(mask), threshold, step,
                                           _ptr(avg), _ptr(mean_intensity), _ptr(photon_count)),
               "iac_average_intensity")
+
+    def ppp_lap(self, frames, out, panels, adu_per_photon: float = 1.0, seed_threshold: float = 0.5,
+                combined_threshold: float = 0.9, mask=None, offset=None, gain=None, gain_only: bool = False,
+                photon_count=None, mean_intensity=None) -> None:
+        """PPP::ProcessData_PF_LAP's per-frame chain (mask, gain correction, LAP photon finder)
+        over [nframes][npix] device frames into `out` (which may be `frames`); panels is a
+        list of (first_ind, fs, ss), PPP::DetectorPanel (IAC/PPP.h:14-18)."""
+        nframes, npix = frames.shape
+        tab = (Panel * max(1, len(panels)))(*[Panel(*map(int, q)) for q in panels])
+        check(lib().iac_ppp_lap(self._h, _ptr(frames), _ptr(out), nframes, npix, _ptr(mask), _ptr(offset),
+                                _ptr(gain), tab, len(panels), adu_per_photon, seed_threshold, combined_threshold,
+                                int(gain_only), _ptr(photon_count), _ptr(mean_intensity)), "iac_ppp_lap")
 
     def ac3d_batch(self, kmap, pix, cnt, frame_off, n_total: int, shape: MeshShape, interp1: int = NEAREST,
                    interp2: int = NEAREST, double_map: bool = True) -> "Ac3dBatch":

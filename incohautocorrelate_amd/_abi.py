@@ -34,6 +34,12 @@ class MeshShape(C.Structure):
         }
 
 
+class Panel(C.Structure):
+    """iac_panel: PPP::DetectorPanel (IncohAutoCorrelate/PPP.h:14-18)."""
+
+    _fields_ = [("first_ind", C.c_uint32), ("fs", C.c_uint32), ("ss", C.c_uint32)]
+
+
 P = C.c_void_p
 F3 = C.POINTER(C.c_float)
 U32 = C.c_uint32
@@ -70,6 +76,8 @@ SIGNATURES: dict[str, tuple] = {
                                     I32, I32, I32, P]),
     "iac_ac3d_records_batch": (I32, [P, P, U32, P, P, U32, U64, P, C.POINTER(MeshShape), I32, I32, I32, P]),
     "iac_sparse_hit_list": (I32, [P, P, U32, P, U32, P, C.c_float, C.c_float, P, P, C.POINTER(U64)]),
+    "iac_ppp_lap": (I32, [P, P, P, U32, U32, P, P, P, C.POINTER(Panel), U32, C.c_float, C.c_float, C.c_float, I32,
+                          P, P]),
     "iac_ac1d_shape": (I32, [F3, U32, C.c_float, F3, F3]),
     "iac_ac1d_cq_multiplicator": (I32, [F3, U32, C.POINTER(C.c_double)]),
     "iac_ac1d_cq": (I32, [P, P, U32, P, U32, C.c_float, C.c_float, I32, C.c_double, P]),

@@ -1005,3 +1005,65 @@ void ora_average_intensity(const float *frames, uint32_t nframes, uint32_t npix,
     free(tmp);
     free(discr);
 }
+
+/* ---------------- pattern pre-processing: PPP.cpp ---------------- */
+/* PPP::PhotonFinder_LargestAdjacentPixel, IAC/PPP.cpp:18-95: whole photons and the
+ * remaining fraction of every pixel of the panels' pixel range first, then per panel (in
+ * list order) one more photon where the fraction reaches the seed threshold and, with the
+ * largest fraction among the left / right / upper / lower neighbours inside the panel, the
+ * combined threshold.  The upper neighbour is looked at only when i - fs > 0. */
+static void ora_lap(float *I, const ora_panel *panels, uint32_t npanels, float adu, float seed, float comb)
+{
+    size_t full = 0;
+    for (uint32_t p = 0; p < npanels; p++) full += (size_t)panels[p].fs * panels[p].ss;
+    float *whole = (float *)calloc(full ? full : 1, sizeof(float));
+    float *frag = (float *)calloc(full ? full : 1, sizeof(float));
+    for (size_t j = 0; j < full; j++) {
+        if (I[j] <= 0) continue;
+        whole[j] = floorf(I[j] / adu);
+        frag[j] = (I[j] / adu) - whole[j];
+    }
+    for (uint32_t p = 0; p < npanels; p++) {
+        const int fs = (int)panels[p].fs, n = (int)(panels[p].fs * panels[p].ss);
+        const size_t base = panels[p].first_ind;
+        for (int i = 0; i < n; i++) {
+            const size_t j = base + (size_t)i;
+            if (!(frag[j] >= seed)) {
+                I[j] = whole[j];
+                continue;
+            }
+            float lap = 0.0f;
+            if (i % fs != 0 && frag[j - 1] > lap) lap = frag[j - 1];
+            if ((i + 1) % fs != 0 && frag[j + 1] > lap) lap = frag[j + 1];
+            if (i - fs > 0 && frag[j - fs] > lap) lap = frag[j - fs];
+            if (i + fs < n && frag[j + fs] > lap) lap = frag[j + fs];
+            I[j] = (frag[j] + lap >= comb) ? whole[j] + 1 : whole[j];
+        }
+    }
+    free(whole);
+    free(frag);
+}
+
+float ora_ppp_lap_frame(float *I, uint32_t npix, const int *mask, const float *offset, const float *gain,
+                        const ora_panel *panels, uint32_t npanels, float adu_per_photon, float seed_threshold,
+                        float combined_threshold, int gain_only)
+{
+    if (mask)
+        for (uint32_t j = 0; j < npix; j++) I[j] = I[j] * (float)mask[j];
+    if (offset && gain) {
+        /* IAC/PPP.cpp:781-803: the offset map is negated and added, then the division by
+         * the gain where allowed, 0 elsewhere */
+        for (uint32_t j = 0; j < npix; j++) I[j] = I[j] + (offset[j] * -1.0f);
+        for (uint32_t j = 0; j < npix; j++) {
+            const int m = mask ? mask[j] : 1;
+            if ((I[j] >= 0 || gain_only) && gain[j] > 0 && m == 1)
+                I[j] = I[j] / gain[j];
+            else
+                I[j] = 0;
+        }
+    }
+    if (!gain_only) ora_lap(I, panels, npanels, adu_per_photon, seed_threshold, combined_threshold);
+    float s = 0.0f;
+    for (uint32_t j = 0; j < npix; j++) s += I[j];
+    return s;
+}

@@ -162,6 +162,20 @@ void ora_sim_postprocess(float *intensity, uint32_t npix, double n_expected, int
 void ora_average_intensity(const float *frames, uint32_t nframes, uint32_t npix, const int *mask,
                            float thr, float step, float *avg, float *mean_intensity, int *photon_count);
 
+/* ---------------- pattern pre-processing: PPP.cpp ---------------- */
+/* PPP::DetectorPanel, IAC/PPP.h:14-18 (Scans = {fs, ss}) */
+typedef struct {
+    uint32_t first_ind, fs, ss;
+} ora_panel;
+/* One frame of PPP::ProcessData_PF_LAP (IAC/PPP.cpp:469-495) in place, in the reference's
+ * order: Detector::ApplyPixelMask when mask is given, PPP::GainCorrection (IAC/PPP.cpp:
+ * 672-808, AllowNegativeValues = gain_only) when offset and gain are given, then unless
+ * gain_only PPP::PhotonFinder_LargestAdjacentPixel (IAC/PPP.cpp:18-95) over the panels in
+ * list order.  Returns the serial float sum of the frame (ArrayOperators::Sum). */
+float ora_ppp_lap_frame(float *intensity, uint32_t npix, const int *mask, const float *offset, const float *gain,
+                        const ora_panel *panels, uint32_t npanels, float adu_per_photon, float seed_threshold,
+                        float combined_threshold, int gain_only);
+
 #ifdef __cplusplus
 }
 #endif

@@ -94,6 +94,8 @@ _SIG = {
     "ora_average_intensity": (None, [F, U32, U32, C.POINTER(C.c_int), C.c_float, C.c_float, F, F,
                                      C.POINTER(C.c_int)]),
     "ora_sim_postprocess": (None, [F, U32, C.c_double, I32, C.c_float, C.c_float, C.c_uint64, U32]),
+    "ora_ppp_lap_frame": (C.c_float, [F, U32, C.POINTER(C.c_int), F, F, P, U32, C.c_float, C.c_float, C.c_float,
+                                      I32]),
 }
 
 _lib = None
@@ -397,3 +399,33 @@ def average_intensity(frames, threshold, step, mask=None):
     lib().ora_average_intensity(fp(fr), nframes, npix, None if m is None else m.ctypes.data_as(C.POINTER(C.c_int)),
                                 threshold, step, fp(avg), fp(mi), pc.ctypes.data_as(C.POINTER(C.c_int)))
     return avg, mi, pc
+
+
+# ---- pattern pre-processing: LAP photon finder -----------------------------------
+class Panel(C.Structure):
+    """PPP::DetectorPanel (IAC/PPP.h:14-18): first pixel, fs (row width), ss (rows)."""
+
+    _fields_ = [("first_ind", C.c_uint32), ("fs", C.c_uint32), ("ss", C.c_uint32)]
+
+
+def ppp_lap(frames, panels, adu=1.0, seed=0.5, comb=0.9, mask=None, offset=None, gain=None, gain_only=False):
+    """PPP::ProcessData_PF_LAP's per-frame chain on [nframes][npix] float frames (a copy).
+    panels: list of (first_ind, fs, ss).  Returns (frames, photon_count, mean_intensity)."""
+    out = np.array(frames, np.float32, copy=True, order="C")
+    out2 = out.reshape(-1, out.shape[-1])
+    nframes, npix = out2.shape
+    tab = (Panel * max(1, len(panels)))(*[Panel(*map(int, q)) for q in panels])
+    m = None if mask is None else np.ascontiguousarray(mask, np.int32)
+    o = None if offset is None else _f(offset)
+    g = None if gain is None else _f(gain)
+    pc = np.empty(nframes, np.int32)
+    mi = np.empty(nframes, np.float32)
+    for f in range(nframes):
+        row = out2[f]
+        s = lib().ora_ppp_lap_frame(fp(row), npix, None if m is None else m.ctypes.data_as(C.POINTER(C.c_int)),
+                                    None if o is None else fp(o), None if g is None else fp(g),
+                                    C.cast(tab, P), len(panels), adu, seed, comb, int(gain_only))
+        s32 = np.float32(s)
+        pc[f] = int(s32) if np.isfinite(s32) else 0
+        mi[f] = np.float32(pc[f]) / np.float32(npix)
+    return out, pc, mi
