@@ -804,7 +804,7 @@ def ppp_leg(args, iac, dev, probes):
                         f"frames, pixel mask, offset and gain maps, {adu} ADU per photon, per-event photon counts",
             "value": nf / (ms * 1e-3), "unit": "frames/s", "ms_per_step": ms, "steps": steps,
             "bit_exact_first_frames": exact,
-            "roofline": {"bound": "hbm", "kernel": "lap_kernel", "kernel_avg_ms": k_ms, "achieved": gbs,
+            "roofline": {"bound": "hbm", "kernel": "lap_rows_kernel", "kernel_avg_ms": k_ms, "achieved": gbs,
                          "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": gbs / HBM_PEAK_GBS,
                          "bytes_per_pixel": 8,
                          "frac_of_measured_copy": gbs / probes["hbm_copy_GBps"] if probes else None},

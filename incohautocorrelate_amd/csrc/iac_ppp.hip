@@ -4,10 +4,11 @@
 // finder per detector panel, and each event's PhotonCount / MeanIntensity, with the
 // reference's float arithmetic bit for bit.
 //
-// One thread owns one pixel across a group of frames: its mask, offset and gain (and
-// those of its four panel neighbours) are read once per group, and each frame costs one
-// 4-byte read and one 4-byte write of HBM; the neighbour reads of a frame hit the lines
-// the adjacent threads and row blocks fetch at the same time.
+// A panel is walked row by row: one wave owns a strip of 62 columns (its two outer lanes
+// hold the neighbouring columns) over 32 rows, keeping the rows above and below in
+// registers and taking the left and right neighbours from the adjacent lanes, so each pixel
+// of a frame is read about 1.07 times and written once; the mask, offset and gain of a row
+// are read once for four frames.
 #include <algorithm>
 #include <cstring>
 #include <vector>
@@ -19,17 +20,23 @@
 namespace {
 
 constexpr uint32_t kMaxPanels = 128;
-constexpr uint32_t kFrameGroup = 16;
 constexpr int kBlock = 256;
-constexpr int kBatch = 2;
+constexpr int kWave = 64;
+constexpr uint32_t kCols = kWave - 2;  // columns a wave writes (lanes 1..62)
+constexpr uint32_t kRows = 64;         // rows a wave walks
+constexpr uint32_t kF = 2;             // frames a wave carries
+constexpr uint32_t kFrameGroup = 16;   // frames per thread of the elementwise kernel
+static_assert(kRows <= 64, "per-lane whole-photon sums of a tile must fit an int");
 
 // PPP::DetectorPanel (IAC/PPP.h:14-18), sorted by first pixel: a panel is `fs` pixels wide
-// (the fast-scan direction) and size / fs rows tall.
+// (the fast-scan direction) and `ss` rows tall; tile0[p] is the first wave tile of panel p
+// (strips of kCols columns times segments of kRows rows), tile0[n] the tiles per frame group.
 struct PanelTable {
     uint32_t n;
     uint32_t first[kMaxPanels];
     uint32_t fs[kMaxPanels];
-    uint32_t size[kMaxPanels];
+    uint32_t ss[kMaxPanels];
+    uint32_t tile0[kMaxPanels + 1];
 };
 
 struct Corr {
@@ -76,92 +83,13 @@ __device__ __forceinline__ Corr load_corr(const int *mask, const float *offset, 
     return c;
 }
 
-// grid: x over pixels, y over groups of kFrameGroup frames
-template <bool kMask, bool kGain>
-__global__ void __launch_bounds__(kBlock)
-    lap_kernel(const float *__restrict__ in, float *__restrict__ out, uint32_t nframes, uint32_t npix,
-               const int *__restrict__ mask, const float *__restrict__ offset, const float *__restrict__ gain,
-               int allow_neg, int lap, PanelTable T, float adu, float seed, float comb)
-{
-    const uint32_t ind = blockIdx.x * kBlock + threadIdx.x;
-    if (ind >= npix) return;
-    const uint32_t f0 = blockIdx.y * kFrameGroup, f1 = min(nframes, f0 + kFrameGroup);
-    const bool an = allow_neg != 0;
-    const Corr c = load_corr<kMask, kGain>(mask, offset, gain, ind);
-
-    // the panel holding this pixel: the last one starting at or before it
-    int p = -1;
-    if (lap && T.n) {
-        uint32_t lo = 0, hi = T.n;
-        while (hi - lo > 1) {
-            const uint32_t mid = (lo + hi) / 2;
-            if (T.first[mid] <= ind) lo = mid; else hi = mid;
-        }
-        if (T.first[lo] <= ind && ind - T.first[lo] < T.size[lo]) p = (int)lo;
-    }
-    if (p < 0) {
-        for (uint32_t f = f0; f < f1; f++) {
-            const size_t o = (size_t)f * npix + ind;
-            out[o] = corrected<kMask, kGain>(in[o], c, an);
-        }
-        return;
-    }
-    // neighbours inside the panel, IAC/PPP.cpp:57-76; the upper one only for i > fs
-    const uint32_t i = ind - T.first[p], fs = T.fs[p];
-    const bool hl = i % fs != 0, hr = (i + 1) % fs != 0, hu = i > fs, hd = (uint64_t)i + fs < T.size[p];
-    Corr cl{}, cr{}, cu{}, cd{};
-    if (hl) cl = load_corr<kMask, kGain>(mask, offset, gain, ind - 1);
-    if (hr) cr = load_corr<kMask, kGain>(mask, offset, gain, ind + 1);
-    if (hu) cu = load_corr<kMask, kGain>(mask, offset, gain, ind - fs);
-    if (hd) cd = load_corr<kMask, kGain>(mask, offset, gain, ind + fs);
-    // kBatch frames at a time: their pixel loads first, then the neighbour loads of the
-    // frames whose fraction reaches the seed threshold, so each batch waits on memory twice
-    for (uint32_t fb = f0; fb < f1; fb += kBatch) {
-        float whole[kBatch], frag[kBatch], x[kBatch][4];
-#pragma unroll
-        for (int k = 0; k < kBatch; k++) {
-            frag[k] = 0.0f;
-            whole[k] = 0.0f;
-            if (fb + k < f1) frag[k] = fraction(corrected<kMask, kGain>(in[(size_t)(fb + k) * npix + ind], c, an),
-                                                adu, &whole[k]);
-        }
-#pragma unroll
-        for (int k = 0; k < kBatch; k++) {
-            x[k][0] = x[k][1] = x[k][2] = x[k][3] = 0.0f;
-            if (fb + k < f1 && frag[k] >= seed) {
-                const float *fr = in + (size_t)(fb + k) * npix;
-                if (hl) x[k][0] = fr[ind - 1];
-                if (hr) x[k][1] = fr[ind + 1];
-                if (hu) x[k][2] = fr[ind - fs];
-                if (hd) x[k][3] = fr[ind + fs];
-            }
-        }
-#pragma unroll
-        for (int k = 0; k < kBatch; k++) {
-            if (fb + k >= f1) break;
-            float r = whole[k];
-            if (frag[k] >= seed) {
-                // IAC/PPP.cpp:53-87: the largest neighbouring fraction, then one more photon
-                // when fraction + LAP reaches the combined threshold
-                float best = 0.0f, w, y;
-                if (hl) { y = fraction(corrected<kMask, kGain>(x[k][0], cl, an), adu, &w); if (y > best) best = y; }
-                if (hr) { y = fraction(corrected<kMask, kGain>(x[k][1], cr, an), adu, &w); if (y > best) best = y; }
-                if (hu) { y = fraction(corrected<kMask, kGain>(x[k][2], cu, an), adu, &w); if (y > best) best = y; }
-                if (hd) { y = fraction(corrected<kMask, kGain>(x[k][3], cd, an), adu, &w); if (y > best) best = y; }
-                if (frag[k] + best >= comb) r = whole[k] + 1.0f;
-            }
-            out[(size_t)(fb + k) * npix + ind] = r;
-        }
-    }
-}
-
 // Per-event photon counts.  ArrayOperators::Sum<float> (IAC/ArrayOperators.h:260-268) is a
 // serial float sum.  When every value of a frame is a whole number and the sum of their
 // magnitudes is at most 2^24, every partial sum of that chain is an integer of at most 2^24
-// and therefore exact, so the serial result is the exact integer sum, whatever the order:
-// pass 1 sums each frame in parallel (int64 sum, magnitude sum, a flag for values that are
-// not whole or exceed 2^24); pass 2 keeps those sums where they are exact and sums the
-// remaining frames serially.
+// and therefore exact, so the serial result is the exact integer sum, whatever the order.
+// The kernels that write a frame add its values to an int64 sum and a magnitude sum, and
+// flag values that are not whole or exceed 2^24; photon_count_kernel keeps those sums where
+// they are exact and sums the remaining frames serially.
 struct FrameAcc {
     long long sum;
     unsigned long long mag;
@@ -169,51 +97,194 @@ struct FrameAcc {
     unsigned int pad;
 };
 
-__global__ void __launch_bounds__(kBlock)
-    frame_int_sums_kernel(const float *__restrict__ frames, uint32_t nframes, uint32_t npix,
-                          FrameAcc *__restrict__ acc)
+// adds one value to a lane's partial sums (at most 32 values of at most 2^24 per lane
+// between flushes, so 32-bit partials cannot overflow)
+__device__ __forceinline__ void tally(float v, int *s, unsigned int *m, bool *bad)
 {
-    __shared__ long long ss[kBlock / 64];
-    __shared__ unsigned long long sm[kBlock / 64];
-    __shared__ unsigned int sb[kBlock / 64];
-    const int w = threadIdx.x / 64;
-    for (uint32_t f = blockIdx.y; f < nframes; f += gridDim.y) {
-        const float *fr = frames + (size_t)f * npix;
-        long long s = 0;
-        unsigned long long m = 0;
-        unsigned int bad = 0;
-        for (uint32_t j = blockIdx.x * kBlock + threadIdx.x; j < npix; j += gridDim.x * kBlock) {
-            const float v = fr[j];
-            const float a = fabsf(v);
-            if (!(a <= 0x1p24f) || v != rintf(v)) {
-                bad = 1;
+    const float a = fabsf(v);
+    if (!(a <= 0x1p24f) || v != rintf(v)) {
+        *bad = true;
+    } else {
+        *s += (int)v;
+        *m += (unsigned int)a;
+    }
+}
+
+// The finder's output inside a panel is floorf(I / ADU) or one more, a whole number; for
+// values in [0, 2^24] its magnitude is itself, and anything else (negative with a negative
+// ADU, above 2^24, NaN) is flagged.  kRows values of at most 2^24 fit an int.
+__device__ __forceinline__ void tally_whole(float v, int *s, bool *bad)
+{
+    if (v >= 0.0f && v <= 0x1p24f) *s += (int)v;
+    else *bad = true;
+}
+
+// The finder's per-lane sums of frame f (at most kRows whole values each) into acc[f]; a
+// lane above 2^24 already puts the frame past 2^24 (serial path), so it is flagged and
+// dropped and the 64-lane sum stays below 2^31.  Every lane of the wave calls it.
+__device__ __forceinline__ void flush_whole(FrameAcc *acc, uint32_t f, int s, bool bad)
+{
+    if (s > (1 << 24)) {
+        bad = true;
+        s = 0;
+    }
+    for (int o = 32; o > 0; o >>= 1) s += __shfl_xor(s, o);
+    const bool anybad = __any(bad);
+    if ((threadIdx.x & (kWave - 1)) == 0) {
+        atomicAdd(reinterpret_cast<unsigned long long *>(&acc[f].sum), (unsigned long long)(long long)s);
+        atomicAdd(&acc[f].mag, (unsigned long long)s);
+        if (anybad) atomicOr(&acc[f].bad, 1u);
+    }
+}
+
+// the wave's partial sums of frame f into acc[f]; every lane of the wave calls it
+__device__ __forceinline__ void flush_tally(FrameAcc *acc, uint32_t f, int s32, unsigned int m32, bool bad)
+{
+    long long s = s32;
+    unsigned long long m = m32;
+    for (int o = 32; o > 0; o >>= 1) {
+        s += __shfl_xor(s, o);
+        m += __shfl_xor(m, o);
+    }
+    const bool anybad = __any(bad);
+    if ((threadIdx.x & (kWave - 1)) == 0) {
+        atomicAdd(reinterpret_cast<unsigned long long *>(&acc[f].sum), (unsigned long long)s);
+        atomicAdd(&acc[f].mag, m);
+        if (anybad) atomicOr(&acc[f].bad, 1u);
+    }
+}
+
+// Pixels outside every panel (and every pixel in gain-only mode): the corrected value.
+// grid: x over pixels [j0, npix), y over groups of kFrameGroup frames.
+template <bool kMask, bool kGain>
+__global__ void __launch_bounds__(kBlock)
+    correct_kernel(const float *__restrict__ in, float *__restrict__ out, uint32_t nframes, uint32_t npix, uint32_t j0,
+                   const int *__restrict__ mask, const float *__restrict__ offset, const float *__restrict__ gain,
+                   int allow_neg, FrameAcc *__restrict__ acc)
+{
+    const uint32_t j = j0 + blockIdx.x * kBlock + threadIdx.x;
+    const bool act = j < npix;  // every lane stays for the wave sums
+    const Corr c = act ? load_corr<kMask, kGain>(mask, offset, gain, j) : Corr{1, 0.0f, 0.0f};
+    for (uint32_t f = blockIdx.y * kFrameGroup; f < nframes; f += gridDim.y * kFrameGroup) {
+        const uint32_t f1 = min(nframes, f + kFrameGroup);
+        for (uint32_t g = f; g < f1; g++) {
+            const size_t o = (size_t)g * npix + j;
+            float v = 0.0f;
+            if (act) {
+                v = corrected<kMask, kGain>(in[o], c, allow_neg != 0);
+                out[o] = v;
+            }
+            if (acc) {
+                int s = 0;
+                unsigned int m = 0;
+                bool bad = false;
+                tally(v, &s, &m, &bad);
+                flush_tally(acc, g, s, m, bad);
+            }
+        }
+    }
+}
+
+// The LAP finder over the panels, one wave per tile (a strip of kCols columns and kRows rows
+// of one panel) and group of kF frames.  Lane l holds column strip * kCols + l - 1.
+// IAC/PPP.cpp:53-87: where a pixel's fraction reaches the seed threshold, the largest
+// fraction among its left / right / upper / lower neighbours inside the panel (the upper one
+// only for i > fs, i = row * fs + column) is added and one more photon counted when the sum
+// reaches the combined threshold.
+template <bool kMask, bool kGain>
+__global__ void __launch_bounds__(kWave)
+    lap_rows_kernel(const float *__restrict__ in, float *__restrict__ out, uint32_t nframes, uint32_t npix,
+                    const int *__restrict__ mask, const float *__restrict__ offset, const float *__restrict__ gain,
+                    PanelTable T, float adu, float seed, float comb, FrameAcc *__restrict__ acc)
+{
+    const uint32_t tile = blockIdx.x;
+    uint32_t lo = 0, hi = T.n;
+    while (hi - lo > 1) {
+        const uint32_t mid = (lo + hi) / 2;
+        if (T.tile0[mid] <= tile) lo = mid; else hi = mid;
+    }
+    const uint32_t fs = T.fs[lo], ss = T.ss[lo], base = T.first[lo];
+    const uint32_t nstrip = (fs + kCols - 1) / kCols, t = tile - T.tile0[lo];
+    const uint32_t strip = t % nstrip, seg = t / nstrip;
+    const int lane = threadIdx.x;
+    const int64_t c = (int64_t)strip * kCols + lane - 1;
+    const bool valid = c >= 0 && c < (int64_t)fs;
+    const bool owner = valid && lane >= 1 && lane <= (int)kCols;
+    const uint32_t cc = valid ? (uint32_t)c : 0u;
+    const bool hl = c > 0, hr = c + 1 < (int64_t)fs;
+    const uint32_t r0 = seg * kRows, r1 = min(ss, r0 + kRows);
+
+    for (uint32_t fg = blockIdx.y * kF; fg < nframes; fg += gridDim.y * kF) {
+        const uint32_t nf = min(kF, nframes - fg);
+        // fractions (and whole photons) of row r for the kF frames; 0 off the panel
+        auto load_row = [&](uint32_t r, float *frag, float *whole) {
+            const size_t j = (size_t)base + (size_t)r * fs + cc;
+            Corr k{1, 0.0f, 0.0f};
+            if (valid) k = load_corr<kMask, kGain>(mask, offset, gain, j);
+            float v[kF];
+#pragma unroll
+            for (uint32_t q = 0; q < kF; q++) v[q] = (valid && q < nf) ? in[(size_t)(fg + q) * npix + j] : 0.0f;
+#pragma unroll
+            for (uint32_t q = 0; q < kF; q++) {
+                frag[q] = 0.0f;
+                whole[q] = 0.0f;
+                if (valid && q < nf) frag[q] = fraction(corrected<kMask, kGain>(v[q], k, false), adu, &whole[q]);
+            }
+        };
+        float fu[kF], fc[kF], wc[kF], fd[kF], wd[kF];
+        int ts[kF];
+        bool tb[kF];
+#pragma unroll
+        for (uint32_t q = 0; q < kF; q++) {
+            ts[q] = 0;
+            tb[q] = false;
+        }
+        if (r0 > 0) {
+            load_row(r0 - 1, fu, wd);
+        } else {
+#pragma unroll
+            for (uint32_t q = 0; q < kF; q++) fu[q] = 0.0f;
+        }
+        load_row(r0, fc, wc);
+        for (uint32_t r = r0; r < r1; r++) {
+            const bool hd = r + 1 < ss;
+            if (hd) {
+                load_row(r + 1, fd, wd);
             } else {
-                s += (long long)v;
-                m += (unsigned long long)a;
+#pragma unroll
+                for (uint32_t q = 0; q < kF; q++) fd[q] = wd[q] = 0.0f;
+            }
+            const bool hu = r >= 2 || (r == 1 && c >= 1);
+            const size_t orow = (size_t)base + (size_t)r * fs + cc;
+#pragma unroll
+            for (uint32_t q = 0; q < kF; q++) {
+                const float left = __shfl_up(fc[q], 1), right = __shfl_down(fc[q], 1);
+                if (owner && q < nf) {
+                    float res = wc[q];
+                    if (fc[q] >= seed) {
+                        float best = 0.0f;
+                        if (hl && left > best) best = left;
+                        if (hr && right > best) best = right;
+                        if (hu && fu[q] > best) best = fu[q];
+                        if (hd && fd[q] > best) best = fd[q];
+                        if (fc[q] + best >= comb) res = wc[q] + 1.0f;
+                    }
+                    out[(size_t)(fg + q) * npix + orow] = res;
+                    tally_whole(res, &ts[q], &tb[q]);
+                }
+            }
+#pragma unroll
+            for (uint32_t q = 0; q < kF; q++) {
+                fu[q] = fc[q];
+                fc[q] = fd[q];
+                wc[q] = wd[q];
             }
         }
-        for (int o = 32; o > 0; o >>= 1) {
-            s += __shfl_xor(s, o);
-            m += __shfl_xor(m, o);
-            bad |= __shfl_xor(bad, o);
+        if (acc) {
+#pragma unroll
+            for (uint32_t q = 0; q < kF; q++)
+                if (q < nf) flush_whole(acc, fg + q, ts[q], tb[q]);
         }
-        if ((threadIdx.x & 63) == 0) {
-            ss[w] = s;
-            sm[w] = m;
-            sb[w] = bad;
-        }
-        __syncthreads();
-        if (threadIdx.x == 0) {
-            for (int k = 1; k < kBlock / 64; k++) {
-                s += ss[k];
-                m += sm[k];
-                bad |= sb[k];
-            }
-            atomicAdd(reinterpret_cast<unsigned long long *>(&acc[f].sum), (unsigned long long)s);
-            atomicAdd(&acc[f].mag, m);
-            if (bad) atomicOr(&acc[f].bad, 1u);
-        }
-        __syncthreads();  // the per-wave slots are reused by the next frame
     }
 }
 
@@ -270,10 +341,10 @@ int iac_ppp_lap(iac_ctx *ctx, const float *d_in, float *d_out, uint32_t nframes,
     if (npanels > kMaxPanels) return iac::fail(IAC_ERR_LIMIT, "at most 128 detector panels");
     PanelTable T;
     std::memset(&T, 0, sizeof(T));
+    uint64_t full = 0;  // pixels [0, full) lie in the panels
     if (!gain_only) {
         // the reference sizes its per-pixel arrays by the panels' total pixel count
         // (IAC/PPP.cpp:420-426) and indexes them with each panel's pixels
-        uint64_t full = 0;
         std::vector<iac_panel> sorted(panels, panels + npanels);
         for (const iac_panel &q : sorted) {
             if (q.fs == 0 || q.ss == 0) return iac::fail(IAC_ERR_INVALID_ARG, "empty detector panel");
@@ -282,7 +353,7 @@ int iac_ppp_lap(iac_ctx *ctx, const float *d_in, float *d_out, uint32_t nframes,
         if (full > npix) return iac::fail(IAC_ERR_SHAPE, "detector panels hold more pixels than a frame");
         std::sort(sorted.begin(), sorted.end(),
                   [](const iac_panel &a, const iac_panel &b) { return a.first_ind < b.first_ind; });
-        uint64_t end_prev = 0;
+        uint64_t end_prev = 0, tiles = 0;
         for (uint32_t k = 0; k < npanels; k++) {
             const iac_panel &q = sorted[k];
             const uint64_t end = (uint64_t)q.first_ind + (uint64_t)q.fs * q.ss;
@@ -291,8 +362,12 @@ int iac_ppp_lap(iac_ctx *ctx, const float *d_in, float *d_out, uint32_t nframes,
             end_prev = end;
             T.first[k] = q.first_ind;
             T.fs[k] = q.fs;
-            T.size[k] = (uint32_t)((uint64_t)q.fs * q.ss);
+            T.ss[k] = q.ss;
+            T.tile0[k] = (uint32_t)tiles;
+            tiles += (uint64_t)((q.fs + kCols - 1) / kCols) * ((q.ss + kRows - 1) / kRows);
+            if (tiles > 0x7fffffffu) return iac::fail(IAC_ERR_LIMIT, "detector panels too large");
         }
+        T.tile0[npanels] = (uint32_t)tiles;
         T.n = npanels;
     }
     if (nframes == 0) return IAC_OK;
@@ -308,26 +383,34 @@ int iac_ppp_lap(iac_ctx *ctx, const float *d_in, float *d_out, uint32_t nframes,
         src = static_cast<const float *>(s);
     }
     const bool hm = d_mask != nullptr, hg = d_gain != nullptr;
-    auto kern = hm ? (hg ? lap_kernel<true, true> : lap_kernel<true, false>)
-                   : (hg ? lap_kernel<false, true> : lap_kernel<false, false>);
-    const dim3 grid((npix + kBlock - 1) / kBlock, (nframes + kFrameGroup - 1) / kFrameGroup);
-    if (grid.y > 65535) return iac::fail(IAC_ERR_LIMIT, "at most 1,048,560 frames per call");
-    IAC_TRY(iac::timer_begin(ctx));
-    hipLaunchKernelGGL(kern, grid, dim3(kBlock), 0, ctx->stream, src, d_out, nframes, npix, d_mask, d_offset, d_gain,
-                       gain_only ? 1 : 0, gain_only ? 0 : 1, T, adu_per_photon, seed_threshold, combined_threshold);
-    IAC_HIP(hipGetLastError());
-    IAC_TRY(iac::timer_end(ctx));
+    FrameAcc *acc = nullptr;
     if (d_photon_count || d_mean_intensity) {
         void *s = nullptr;
         IAC_TRY(iac::scratch(ctx, 1, (size_t)nframes * sizeof(FrameAcc), &s));
-        auto *acc = static_cast<FrameAcc *>(s);
+        acc = static_cast<FrameAcc *>(s);
         hipLaunchKernelGGL(zero_acc_kernel, dim3((nframes + 255) / 256), dim3(256), 0, ctx->stream, acc, nframes);
         IAC_HIP(hipGetLastError());
-        // about 16 values per thread
-        const uint32_t bx = std::max<uint32_t>(1, std::min<uint32_t>(1024, (npix + kBlock * 16 - 1) / (kBlock * 16)));
-        hipLaunchKernelGGL(frame_int_sums_kernel, dim3(bx, std::min<uint32_t>(nframes, 65535)), dim3(kBlock), 0,
-                           ctx->stream, d_out, nframes, npix, acc);
+    }
+    IAC_TRY(iac::timer_begin(ctx));
+    if (T.n) {
+        auto kern = hm ? (hg ? lap_rows_kernel<true, true> : lap_rows_kernel<true, false>)
+                       : (hg ? lap_rows_kernel<false, true> : lap_rows_kernel<false, false>);
+        const dim3 grid(T.tile0[T.n], std::min<uint32_t>(65535, (nframes + kF - 1) / kF));
+        hipLaunchKernelGGL(kern, grid, dim3(kWave), 0, ctx->stream, src, d_out, nframes, npix, d_mask, d_offset,
+                           d_gain, T, adu_per_photon, seed_threshold, combined_threshold, acc);
         IAC_HIP(hipGetLastError());
+    }
+    if (full < npix) {
+        auto kern = hm ? (hg ? correct_kernel<true, true> : correct_kernel<true, false>)
+                       : (hg ? correct_kernel<false, true> : correct_kernel<false, false>);
+        const uint32_t rest = npix - (uint32_t)full;
+        const dim3 grid((rest + kBlock - 1) / kBlock, std::min<uint32_t>(65535, (nframes + kFrameGroup - 1) / kFrameGroup));
+        hipLaunchKernelGGL(kern, grid, dim3(kBlock), 0, ctx->stream, src, d_out, nframes, npix, (uint32_t)full, d_mask,
+                           d_offset, d_gain, gain_only ? 1 : 0, acc);
+        IAC_HIP(hipGetLastError());
+    }
+    IAC_TRY(iac::timer_end(ctx));
+    if (acc) {
         const bool v4 = npix % 4 == 0 && reinterpret_cast<uintptr_t>(d_out) % 16 == 0;
         hipLaunchKernelGGL(v4 ? photon_count_kernel<true> : photon_count_kernel<false>, dim3((nframes + 63) / 64),
                            dim3(64), 0, ctx->stream, d_out, nframes, npix, acc, d_photon_count, d_mean_intensity);

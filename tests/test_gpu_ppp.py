@@ -19,6 +19,8 @@ def layout(name):
         pan = [(k * 384, 32, 12) for k in range(8)]
         order = [3, 0, 7, 1, 5, 2, 6, 4]
         return 8 * 384 + 101, [pan[k] for k in order]
+    if name == "odd":  # one-column, narrow, strip-sized and one-row panels, 5 pixels past them
+        return 2491 + 63 + 5, [(0, 1, 70), (70, 125, 3), (445, 62, 33), (2491, 63, 1)]
     if name == "modules":  # a 1024x1024 detector as eight 1024x128 modules
         return 1024 * 1024, [(k * 1024 * 128, 1024, 128) for k in range(8)]
     raise ValueError(name)
@@ -62,6 +64,8 @@ def run_gpu(ctx, dev, I, panels, mask, offset, gain, gain_only, inplace, **kw):
     ("one", True, True, True, False, 3),
     ("one", False, True, False, True, 16),
     ("eight", False, False, False, True, 1),
+    ("odd", True, True, False, False, 9),
+    ("odd", False, False, False, False, 4),
 ])
 def test_lap_vs_oracle(ctx, dev, name, use_mask, use_gain, gain_only, inplace, nframes):
     npix, panels = layout(name)
@@ -124,3 +128,34 @@ def test_lap_rejects_bad_panels(ctx, dev):
         ctx.ppp_lap(d, o, [], offset=torch.zeros(100, device=dev))
     assert e.value.code == -1
     ctx.ppp_lap(d[:0], o[:0], [(0, 10, 10)])  # no frames: nothing to do
+
+
+@pytest.mark.parametrize("kind", ["whole", "huge_value", "huge_sum", "negative_whole"])
+def test_photon_count_paths(ctx, dev, kind):
+    """The per-event photon counts: exact integer sums where every value is whole and the
+    magnitudes sum to at most 2^24, the serial float chain otherwise (values above 2^24, or
+    partial sums that round), with and without the LAP finder."""
+    rng = np.random.default_rng(21)
+    npix, nf = 64 * 40 + 3, 6
+    I = rng.poisson(2.0, (nf, npix)).astype(np.float32)
+    if kind == "huge_value":
+        I[1, 17] = 2.0 ** 25 + 2
+    elif kind == "huge_sum":  # each value whole and below 2^24, the running sum rounds
+        I[2, ::50] = 2.0 ** 20 + 1
+    elif kind == "negative_whole":
+        I[3, ::7] = -5.0
+    for panels in ([], [(0, 64, 40)]):
+        ref, rpc, rmi = O.ppp_lap(I, panels, adu=1.0)
+        import torch
+
+        d_in = torch.from_numpy(I).to(dev)
+        d_out = torch.empty_like(d_in)
+        pc = torch.empty(nf, dtype=torch.int32, device=dev)
+        mi = torch.empty(nf, dtype=torch.float32, device=dev)
+        ctx.ppp_lap(d_in, d_out, panels, 1.0, photon_count=pc, mean_intensity=mi)
+        torch.cuda.synchronize()
+        assert np.array_equal(d_out.cpu().numpy().view(np.int32), ref.view(np.int32))
+        assert np.array_equal(pc.cpu().numpy(), rpc), (kind, panels)
+        assert np.array_equal(mi.cpu().numpy().view(np.int32), rmi.view(np.int32))
+    if kind == "huge_sum":  # the serial chain really differs from the exact sum here
+        assert int(rpc[2]) != int(I[2].astype(np.int64).sum())
