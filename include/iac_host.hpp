@@ -312,6 +312,34 @@ IAC_Report Run_AutoCorr_DataEval(Settings &PrgSettings, CreateDataEval_Settings 
 IAC_Report Run_AutoCorr_DataEval(Settings &PrgSettings, CreateDataEval_Settings EvalSettings, Detector &Det);
 }  // namespace RunIAC
 
+// IAC/PPP.h:11-101: the per-frame operations of pattern pre-processing (the HDF5 / XML handling
+// of ProcessData_PF_LAP stays with the caller; iac_ppp_lap runs a whole stack with mask, gain
+// maps and photon counts)
+namespace PPP {
+// IAC/PPP.h:14-18
+struct DetectorPanel {
+    unsigned int FirstInd = 0;
+    unsigned int Scans[2] = {2, 2};  // fs, ss
+};
+// IAC/PPP.h:20-37 (the finder's part)
+struct Create_LAPSettings {
+    unsigned int DetPanels_Num = 1;
+    std::vector<DetectorPanel> DetectorPanels;
+    float ADU_perPhoton = 1.0f;
+    float SeedThershold = 0.5f;
+    float CombinedThershold = 0.9f;
+};
+// IAC/PPP.cpp:18-95, in place on one host frame of FullDetSize pixels (the panels' total), on a
+// device of a process-wide pool
+void PhotonFinder_LargestAdjacentPixel(float *Intensity, std::vector<DetectorPanel> DetectorPanels, int FullDetSize,
+                                       float ADU_perPhoton = 1.0f, float SeedThershold = 0.5f,
+                                       float CombinedThershold = 0.9f);
+// the same on a device of Options
+void PhotonFinder_LargestAdjacentPixel(float *Intensity, const std::vector<DetectorPanel> &DetectorPanels,
+                                       int FullDetSize, float ADU_perPhoton, float SeedThershold,
+                                       float CombinedThershold, Settings &Options);
+}  // namespace PPP
+
 // IAC/Simulator.h:26-56 (+ the device simulator); emitter positions are supplied by the
 // caller (the reference's Crystal generator is not part of this library)
 class Simulator {

@@ -23,11 +23,50 @@ static void usage()
                  "  [--photon-offset T] [--photon-step S] [--interp1 0|1] [--interp2 0|1] [--no-double-map]\n"
                  "  [--bounds LO HI] [--invert-rot] [--ignore-orientation] [--mask-flatfield] [--batch B]\n"
                  "  [--out-avint F] [--out-acuw F] [--out-cq F] [--out-cq-small F] [--out-final F] [--out-q F]\n"
-                 "  [--echo L] [--version]\n";
+                 "  [--echo L] [--version]\n"
+                 "       iac_eval --lap STACK N NPIX PANELS ADU SEED COMB OUT\n"
+                 "  (PPP::PhotonFinder_LargestAdjacentPixel frame by frame; PANELS = first:fs:ss,...)\n";
+}
+
+// the LAP photon finder over a raw stack, one frame at a time as IAC/PPP.cpp:469-479 loops
+static int lap_main(int argc, char **argv)
+{
+    if (argc != 10) {
+        usage();
+        return 2;
+    }
+    const unsigned long n = std::strtoul(argv[3], nullptr, 10), npix = std::strtoul(argv[4], nullptr, 10);
+    std::vector<PPP::DetectorPanel> panels;
+    unsigned long full = 0;
+    for (const char *p = argv[5]; *p;) {
+        PPP::DetectorPanel d;
+        char *e = nullptr;
+        d.FirstInd = (unsigned)std::strtoul(p, &e, 10);
+        if (*e != ':') break;
+        d.Scans[0] = (unsigned)std::strtoul(e + 1, &e, 10);
+        if (*e != ':') break;
+        d.Scans[1] = (unsigned)std::strtoul(e + 1, &e, 10);
+        panels.push_back(d);
+        full += (unsigned long)d.Scans[0] * d.Scans[1];
+        p = *e == ',' ? e + 1 : e;
+    }
+    if (n == 0 || npix == 0 || panels.empty() || full > npix) {
+        usage();
+        return 2;
+    }
+    std::vector<float> I(n * npix);
+    iac_host::read_raw(argv[2], I.data(), I.size() * sizeof(float));
+    for (unsigned long f = 0; f < n; f++)
+        PPP::PhotonFinder_LargestAdjacentPixel(I.data() + f * npix, panels, (int)full, std::strtof(argv[6], nullptr),
+                                               std::strtof(argv[7], nullptr), std::strtof(argv[8], nullptr));
+    iac_host::write_raw(argv[9], I.data(), I.size() * sizeof(float));
+    std::cout << "lap ok " << n << std::endl;
+    return 0;
 }
 
 int main(int argc, char **argv)
 {
+    if (argc >= 2 && std::string(argv[1]) == "--lap") return lap_main(argc, argv);
     RunIAC::CreateDataEval_Settings E;
     E.EchoLevel = 0;
     std::string frames, rotations;

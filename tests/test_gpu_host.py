@@ -157,3 +157,24 @@ def test_average_intensity_entry_point(ctx, dev, thr, step, masked):
     assert np.array_equal(d_avg.cpu().numpy(), avg)
     assert np.array_equal(d_mi.cpu().numpy(), mi)
     assert np.array_equal(d_pc.cpu().numpy(), pc)
+
+
+def test_host_mirror_lap_photon_finder(tmp_path):
+    """PPP::PhotonFinder_LargestAdjacentPixel of the C++ mirror (reference signature, one host
+    frame at a time through iac_eval --lap) against the oracle, bit for bit; pixels past the
+    panels' FullDetSize stay as they were."""
+    rng = np.random.default_rng(8)
+    panels = [(0, 33, 7), (231, 20, 12), (471, 1, 9)]
+    full = 33 * 7 + 20 * 12 + 9
+    npix, nf, adu = full + 11, 5, 37.0
+    I = (adu * (rng.poisson(0.4, (nf, npix)) + rng.uniform(-0.3, 1.0, (nf, npix)))).astype(np.float32)
+    src, dst = tmp_path / "lap_in.bin", tmp_path / "lap_out.bin"
+    I.tofile(src)
+    spec = ",".join(f"{a}:{b}:{c}" for a, b, c in panels)
+    r = subprocess.run([str(EVAL), "--lap", str(src), str(nf), str(npix), spec, str(adu), "0.5", "0.9", str(dst)],
+                       capture_output=True, text=True, timeout=120)
+    assert r.returncode == 0 and "lap ok 5" in r.stdout, r.stderr
+    out = np.fromfile(dst, np.float32).reshape(nf, npix)
+    ref, _, _ = O.ppp_lap(I, panels, adu=adu)
+    assert np.array_equal(out.view(np.int32), ref.view(np.int32))
+    assert np.array_equal(out[:, full:], I[:, full:])
