@@ -1,6 +1,7 @@
 // iac_fpexact.h -- correctly rounded float square root and division by a bin
 // width, in fewer instructions than the general library forms, for the AC1D
-// pair kernels (each pair takes one |q| = sqrtf(...) and one q / dq).
+// pair kernels (each pair takes one |q| = sqrtf(...) and one q / dq); and the
+// reference's x86 float -> int conversion of per-event sums.
 #pragma once
 
 #include <hip/hip_runtime.h>
@@ -32,6 +33,14 @@ __device__ __forceinline__ float sqrt_rn(float x)
         return sqrtf(x);
     }
     return sqrt_rn_normal(x);
+}
+
+// (int)x as the reference's x86-64 build converts it (cvttss2si): truncation toward zero, and
+// 0x80000000 ("integer indefinite") for NaN and for values outside the int range, where the
+// GPU's conversion would saturate.
+__device__ __forceinline__ int cvtt_i32(float x)
+{
+    return (x >= -0x1p31f && x < 0x1p31f) ? (int)x : (-2147483647 - 1);
 }
 
 }  // namespace iac

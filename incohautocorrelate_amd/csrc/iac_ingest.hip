@@ -6,6 +6,7 @@
 #include <algorithm>
 #include <cmath>
 
+#include "iac_fpexact.h"
 #include "iac_internal.h"
 
 #pragma clang fp contract(off)
@@ -77,7 +78,7 @@ __global__ void __launch_bounds__(64)
     float s = 0.0f;
     for (uint32_t j = 0; j < npix; j++) s += frame_value(fr[j], mask ? mask[j] : 1, mask != nullptr, T, S);
     if (mean_intensity) mean_intensity[f] = s / (float)npix;
-    if (photon_count && S > 0) photon_count[f] = (int)s;
+    if (photon_count && S > 0) photon_count[f] = iac::cvtt_i32(s);
 }
 
 }  // namespace

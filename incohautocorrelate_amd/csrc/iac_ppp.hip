@@ -13,6 +13,7 @@
 #include <cstring>
 #include <vector>
 
+#include "iac_fpexact.h"
 #include "iac_internal.h"
 
 #pragma clang fp contract(off)
@@ -319,7 +320,7 @@ __global__ void __launch_bounds__(64)
     } else {
         for (uint32_t j = 0; j < npix; j++) s += fr[j];
     }
-    const int pc = (int)s;
+    const int pc = iac::cvtt_i32(s);
     if (count) count[f] = pc;
     if (mean) mean[f] = (float)pc / (float)npix;
 }

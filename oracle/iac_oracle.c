@@ -8,6 +8,14 @@
  * plus an explicit ban on FMA contraction.
  */
 #include "iac_oracle.h"
+
+/* (int)x on the reference's x86-64 build: cvttss2si truncates toward zero and returns
+ * 0x80000000 for NaN and out-of-range values (C leaves those undefined, so the rule is
+ * written out; tests compare it with the host's own conversion). */
+int ora_cvtt_i32(float x)
+{
+    return (x >= -0x1p31f && x < 0x1p31f) ? (int)x : INT32_MIN;
+}
 #include <math.h>
 #include <stdlib.h>
 #include <string.h>
@@ -995,7 +1003,7 @@ void ora_average_intensity(const float *frames, uint32_t nframes, uint32_t npix,
         }
         float s = 0.0f;
         for (uint32_t j = 0; j < npix; j++) s += tmp[j];
-        if (step > 0 && photon_count) photon_count[f] = (int)s;
+        if (step > 0 && photon_count) photon_count[f] = ora_cvtt_i32(s);
         if (mean_intensity) mean_intensity[f] = s / ((float)npix);
     }
     if (step > 0)

@@ -162,6 +162,9 @@ void ora_sim_postprocess(float *intensity, uint32_t npix, double n_expected, int
 void ora_average_intensity(const float *frames, uint32_t nframes, uint32_t npix, const int *mask,
                            float thr, float step, float *avg, float *mean_intensity, int *photon_count);
 
+/* (int)x as x86-64 cvttss2si converts it (0x80000000 for NaN and out-of-range values) */
+int ora_cvtt_i32(float x);
+
 /* ---------------- pattern pre-processing: PPP.cpp ---------------- */
 /* PPP::DetectorPanel, IAC/PPP.h:14-18 (Scans = {fs, ss}) */
 typedef struct {

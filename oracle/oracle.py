@@ -94,6 +94,7 @@ _SIG = {
     "ora_average_intensity": (None, [F, U32, U32, C.POINTER(C.c_int), C.c_float, C.c_float, F, F,
                                      C.POINTER(C.c_int)]),
     "ora_sim_postprocess": (None, [F, U32, C.c_double, I32, C.c_float, C.c_float, C.c_uint64, U32]),
+    "ora_cvtt_i32": (C.c_int, [C.c_float]),
     "ora_ppp_lap_frame": (C.c_float, [F, U32, C.POINTER(C.c_int), F, F, P, U32, C.c_float, C.c_float, C.c_float,
                                       I32]),
 }
@@ -425,7 +426,11 @@ def ppp_lap(frames, panels, adu=1.0, seed=0.5, comb=0.9, mask=None, offset=None,
         s = lib().ora_ppp_lap_frame(fp(row), npix, None if m is None else m.ctypes.data_as(C.POINTER(C.c_int)),
                                     None if o is None else fp(o), None if g is None else fp(g),
                                     C.cast(tab, P), len(panels), adu, seed, comb, int(gain_only))
-        s32 = np.float32(s)
-        pc[f] = int(s32) if np.isfinite(s32) else 0
+        pc[f] = lib().ora_cvtt_i32(s)
         mi[f] = np.float32(pc[f]) / np.float32(npix)
     return out, pc, mi
+
+
+def cvtt_i32(x):
+    """(int)x of the reference's x86-64 build (cvttss2si)."""
+    return int(lib().ora_cvtt_i32(float(np.float32(x))))

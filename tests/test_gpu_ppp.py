@@ -159,3 +159,29 @@ def test_photon_count_paths(ctx, dev, kind):
         assert np.array_equal(mi.cpu().numpy().view(np.int32), rmi.view(np.int32))
     if kind == "huge_sum":  # the serial chain really differs from the exact sum here
         assert int(rpc[2]) != int(I[2].astype(np.int64).sum())
+
+
+def test_photon_count_past_int_range(ctx, dev):
+    """Sums past 2^31 convert as the reference's x86 build does: 0x80000000, on both the LAP
+    chain (gain-only frames of large ADU values) and the averaged-intensity pass."""
+    import torch
+
+    I = np.full((3, 1000), 3.0e6, np.float32)
+    I[1] = 1.0  # a frame that stays in range
+    ref, rpc, rmi = O.ppp_lap(I, [], gain_only=True)
+    assert rpc[0] == -2 ** 31 and rpc[1] == 1000
+    d = torch.from_numpy(I).to(dev)
+    pc = torch.empty(3, dtype=torch.int32, device=dev)
+    mi = torch.empty(3, dtype=torch.float32, device=dev)
+    ctx.ppp_lap(d, torch.empty_like(d), [], gain_only=True, photon_count=pc, mean_intensity=mi)
+    torch.cuda.synchronize()
+    assert np.array_equal(pc.cpu().numpy(), rpc) and np.array_equal(mi.cpu().numpy().view(np.int32), rmi.view(np.int32))
+    # averaged intensity with photon discretisation (step 1): 1000 pixels of 3e6 photons each
+    avg, omi, opc = O.average_intensity(I, 0.0, 1.0)
+    assert opc[0] == -2 ** 31
+    g_avg = torch.empty(1000, dtype=torch.float32, device=dev)
+    g_pc = torch.empty(3, dtype=torch.int32, device=dev)
+    g_mi = torch.empty(3, dtype=torch.float32, device=dev)
+    ctx.average_intensity(d, None, 0.0, 1.0, g_avg, g_mi, g_pc)
+    torch.cuda.synchronize()
+    assert np.array_equal(g_pc.cpu().numpy(), opc)

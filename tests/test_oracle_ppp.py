@@ -136,3 +136,21 @@ def test_random_frames_stay_within_one_photon(adu):
     frac = np.where(inside > 0, inside / np.float32(adu) - whole, 0)
     assert (d[frac >= 0.9] == 1).all()
     assert (d[frac < 0.5] == 0).all()
+
+
+def test_photon_count_cast_follows_x86():
+    """PhotonCount = (int)Sum: the x86-64 conversion (cvttss2si) truncates and gives
+    0x80000000 for NaN and out-of-range sums; numpy's float32 -> int32 cast on this x86 host
+    executes that instruction, so it pins the oracle's rule."""
+    import warnings
+
+    vals = np.array([3e9, -3e9, np.nan, np.inf, -np.inf, 2147483520.0, -2147483648.0, 2147483648.0, -2.5, 2.9,
+                     0.0, -0.0, 1e-30], np.float32)
+    with warnings.catch_warnings():
+        warnings.simplefilter("ignore")
+        host = vals.astype(np.int32)
+    assert [O.cvtt_i32(v) for v in vals] == host.tolist()
+    # a gain-only frame whose float sum passes 2^31
+    I = np.full((1, 4), 1.0e9, np.float32)
+    _, pc, mi = O.ppp_lap(I, [], gain_only=True)
+    assert pc[0] == -2 ** 31 and mi[0] == np.float32(-2 ** 31) / np.float32(4)
