@@ -11,6 +11,7 @@
 // are read once for four frames.
 #include <algorithm>
 #include <cstring>
+#include <type_traits>
 #include <vector>
 
 #include "iac_fpexact.h"
@@ -27,7 +28,7 @@ constexpr uint32_t kCols = kWave - 2;  // columns a wave writes (lanes 1..62)
 constexpr uint32_t kRows = 64;         // rows a wave walks
 constexpr uint32_t kF = 2;             // frames a wave carries
 constexpr uint32_t kFrameGroup = 16;   // frames per thread of the elementwise kernel
-static_assert(kRows <= 64, "per-lane whole-photon sums of a tile must fit an int");
+static_assert(kRows <= 64, "per-lane whole-photon sums of a tile must fit an unsigned int");
 
 // PPP::DetectorPanel (IAC/PPP.h:14-18), sorted by first pixel: a panel is `fs` pixels wide
 // (the fast-scan direction) and `ss` rows tall; tile0[p] is the first wave tile of panel p
@@ -113,19 +114,20 @@ __device__ __forceinline__ void tally(float v, int *s, unsigned int *m, bool *ba
 
 // The finder's output inside a panel is floorf(I / ADU) or one more, a whole number; for
 // values in [0, 2^24] its magnitude is itself, and anything else (negative with a negative
-// ADU, above 2^24, NaN) is flagged.  kRows values of at most 2^24 fit an int.
-__device__ __forceinline__ void tally_whole(float v, int *s, bool *bad)
+// ADU, above 2^24, NaN) is flagged.  2 * kRows values of at most 2^24 fit an unsigned int.
+__device__ __forceinline__ void tally_whole(float v, unsigned int *s, bool *bad)
 {
-    if (v >= 0.0f && v <= 0x1p24f) *s += (int)v;
+    if (v >= 0.0f && v <= 0x1p24f) *s += (unsigned int)v;
     else *bad = true;
 }
 
 // The finder's per-lane sums of frame f (at most kRows whole values each) into acc[f]; a
 // lane above 2^24 already puts the frame past 2^24 (serial path), so it is flagged and
 // dropped and the 64-lane sum stays below 2^31.  Every lane of the wave calls it.
-__device__ __forceinline__ void flush_whole(FrameAcc *acc, uint32_t f, int s, bool bad)
+__device__ __forceinline__ void flush_whole(FrameAcc *acc, uint32_t f, unsigned int su, bool bad)
 {
-    if (s > (1 << 24)) {
+    int s = (int)su;
+    if (su > (1u << 24)) {
         bad = true;
         s = 0;
     }
@@ -186,13 +188,47 @@ __global__ void __launch_bounds__(kBlock)
     }
 }
 
-// The LAP finder over the panels, one wave per tile (a strip of kCols columns and kRows rows
-// of one panel) and group of kF frames.  Lane l holds column strip * kCols + l - 1.
+// kW consecutive values from p: one 8-byte access for kW = 2 (p 8-byte aligned)
+template <int kW, typename T>
+__device__ __forceinline__ void load_w(const T *p, T *v)
+{
+    if constexpr (kW == 2) {
+        using V = typename std::conditional<std::is_same<T, int>::value, int2, float2>::type;
+        const V x = *reinterpret_cast<const V *>(p);
+        v[0] = x.x;
+        v[1] = x.y;
+    } else {
+        v[0] = *p;
+    }
+}
+
+template <bool kMask, bool kGain, int kW>
+__device__ __forceinline__ void load_corr_w(const int *mask, const float *offset, const float *gain, size_t j,
+                                            Corr *c)
+{
+    int m[kW];
+    float o[kW], g[kW];
+    for (int w = 0; w < kW; w++) {
+        m[w] = 1;
+        o[w] = g[w] = 0.0f;
+    }
+    if (kMask) load_w<kW>(mask + j, m);
+    if (kGain) {
+        load_w<kW>(offset + j, o);
+        load_w<kW>(gain + j, g);
+    }
+    for (int w = 0; w < kW; w++) c[w] = Corr{m[w], o[w], g[w]};
+}
+
+// The LAP finder over the panels, one wave per tile (a strip of kCols * kW columns and kRows
+// rows of one panel) and group of kF frames.  Lane l holds the kW columns from
+// strip * kCols * kW + (l - 1) * kW (kW = 2 only where every panel starts and ends on even
+// pixels, so the pairs are 8-byte aligned).
 // IAC/PPP.cpp:53-87: where a pixel's fraction reaches the seed threshold, the largest
 // fraction among its left / right / upper / lower neighbours inside the panel (the upper one
 // only for i > fs, i = row * fs + column) is added and one more photon counted when the sum
 // reaches the combined threshold.
-template <bool kMask, bool kGain>
+template <bool kMask, bool kGain, int kW>
 __global__ void __launch_bounds__(kWave)
     lap_rows_kernel(const float *__restrict__ in, float *__restrict__ out, uint32_t nframes, uint32_t npix,
                     const int *__restrict__ mask, const float *__restrict__ offset, const float *__restrict__ gain,
@@ -205,35 +241,42 @@ __global__ void __launch_bounds__(kWave)
         if (T.tile0[mid] <= tile) lo = mid; else hi = mid;
     }
     const uint32_t fs = T.fs[lo], ss = T.ss[lo], base = T.first[lo];
-    const uint32_t nstrip = (fs + kCols - 1) / kCols, t = tile - T.tile0[lo];
+    const uint32_t cols = kCols * kW;
+    const uint32_t nstrip = (fs + cols - 1) / cols, t = tile - T.tile0[lo];
     const uint32_t strip = t % nstrip, seg = t / nstrip;
     const int lane = threadIdx.x;
-    const int64_t c = (int64_t)strip * kCols + lane - 1;
-    const bool valid = c >= 0 && c < (int64_t)fs;
+    const int64_t c = (int64_t)strip * cols + (int64_t)(lane - 1) * kW;  // first column of the lane
+    const bool valid = c >= 0 && c < (int64_t)fs;  // with kW = 2, fs is even: both columns
     const bool owner = valid && lane >= 1 && lane <= (int)kCols;
     const uint32_t cc = valid ? (uint32_t)c : 0u;
-    const bool hl = c > 0, hr = c + 1 < (int64_t)fs;
     const uint32_t r0 = seg * kRows, r1 = min(ss, r0 + kRows);
 
     for (uint32_t fg = blockIdx.y * kF; fg < nframes; fg += gridDim.y * kF) {
         const uint32_t nf = min(kF, nframes - fg);
         // fractions (and whole photons) of row r for the kF frames; 0 off the panel
-        auto load_row = [&](uint32_t r, float *frag, float *whole) {
+        auto load_row = [&](uint32_t r, float (*frag)[kW], float (*whole)[kW]) {
             const size_t j = (size_t)base + (size_t)r * fs + cc;
-            Corr k{1, 0.0f, 0.0f};
-            if (valid) k = load_corr<kMask, kGain>(mask, offset, gain, j);
-            float v[kF];
-#pragma unroll
-            for (uint32_t q = 0; q < kF; q++) v[q] = (valid && q < nf) ? in[(size_t)(fg + q) * npix + j] : 0.0f;
+            Corr k[kW];
+            for (int w = 0; w < kW; w++) k[w] = Corr{1, 0.0f, 0.0f};
+            if (valid) load_corr_w<kMask, kGain, kW>(mask, offset, gain, j, k);
+            float v[kF][kW];
 #pragma unroll
             for (uint32_t q = 0; q < kF; q++) {
-                frag[q] = 0.0f;
-                whole[q] = 0.0f;
-                if (valid && q < nf) frag[q] = fraction(corrected<kMask, kGain>(v[q], k, false), adu, &whole[q]);
+                for (int w = 0; w < kW; w++) v[q][w] = 0.0f;
+                if (valid && q < nf) load_w<kW>(in + (size_t)(fg + q) * npix + j, v[q]);
             }
+#pragma unroll
+            for (uint32_t q = 0; q < kF; q++)
+#pragma unroll
+                for (int w = 0; w < kW; w++) {
+                    frag[q][w] = 0.0f;
+                    whole[q][w] = 0.0f;
+                    if (valid && q < nf)
+                        frag[q][w] = fraction(corrected<kMask, kGain>(v[q][w], k[w], false), adu, &whole[q][w]);
+                }
         };
-        float fu[kF], fc[kF], wc[kF], fd[kF], wd[kF];
-        int ts[kF];
+        float fu[kF][kW], fc[kF][kW], wc[kF][kW], fd[kF][kW], wd[kF][kW];
+        unsigned int ts[kF];
         bool tb[kF];
 #pragma unroll
         for (uint32_t q = 0; q < kF; q++) {
@@ -244,7 +287,8 @@ __global__ void __launch_bounds__(kWave)
             load_row(r0 - 1, fu, wd);
         } else {
 #pragma unroll
-            for (uint32_t q = 0; q < kF; q++) fu[q] = 0.0f;
+            for (uint32_t q = 0; q < kF; q++)
+                for (int w = 0; w < kW; w++) fu[q][w] = 0.0f;
         }
         load_row(r0, fc, wc);
         for (uint32_t r = r0; r < r1; r++) {
@@ -253,33 +297,47 @@ __global__ void __launch_bounds__(kWave)
                 load_row(r + 1, fd, wd);
             } else {
 #pragma unroll
-                for (uint32_t q = 0; q < kF; q++) fd[q] = wd[q] = 0.0f;
+                for (uint32_t q = 0; q < kF; q++)
+                    for (int w = 0; w < kW; w++) fd[q][w] = wd[q][w] = 0.0f;
             }
-            const bool hu = r >= 2 || (r == 1 && c >= 1);
             const size_t orow = (size_t)base + (size_t)r * fs + cc;
 #pragma unroll
             for (uint32_t q = 0; q < kF; q++) {
-                const float left = __shfl_up(fc[q], 1), right = __shfl_down(fc[q], 1);
-                if (owner && q < nf) {
-                    float res = wc[q];
-                    if (fc[q] >= seed) {
+                // the column left of the lane's first and right of its last, from the next lanes
+                const float lft = __shfl_up(fc[q][kW - 1], 1), rgt = __shfl_down(fc[q][0], 1);
+                float res[kW];
+#pragma unroll
+                for (int w = 0; w < kW; w++) {
+                    const int64_t cw = c + w;
+                    const float left = w == 0 ? lft : fc[q][w - 1];
+                    const float right = w == kW - 1 ? rgt : fc[q][w + 1];
+                    const bool hl = cw > 0, hr = cw + 1 < (int64_t)fs, hu = r >= 2 || (r == 1 && cw >= 1);
+                    res[w] = wc[q][w];
+                    if (fc[q][w] >= seed) {
                         float best = 0.0f;
                         if (hl && left > best) best = left;
                         if (hr && right > best) best = right;
-                        if (hu && fu[q] > best) best = fu[q];
-                        if (hd && fd[q] > best) best = fd[q];
-                        if (fc[q] + best >= comb) res = wc[q] + 1.0f;
+                        if (hu && fu[q][w] > best) best = fu[q][w];
+                        if (hd && fd[q][w] > best) best = fd[q][w];
+                        if (fc[q][w] + best >= comb) res[w] = wc[q][w] + 1.0f;
                     }
-                    out[(size_t)(fg + q) * npix + orow] = res;
-                    tally_whole(res, &ts[q], &tb[q]);
+                }
+                if (owner && q < nf) {
+                    float *o = out + (size_t)(fg + q) * npix + orow;
+                    if constexpr (kW == 2) *reinterpret_cast<float2 *>(o) = make_float2(res[0], res[1]);
+                    else o[0] = res[0];
+#pragma unroll
+                    for (int w = 0; w < kW; w++) tally_whole(res[w], &ts[q], &tb[q]);
                 }
             }
 #pragma unroll
-            for (uint32_t q = 0; q < kF; q++) {
-                fu[q] = fc[q];
-                fc[q] = fd[q];
-                wc[q] = wd[q];
-            }
+            for (uint32_t q = 0; q < kF; q++)
+#pragma unroll
+                for (int w = 0; w < kW; w++) {
+                    fu[q][w] = fc[q][w];
+                    fc[q][w] = fd[q][w];
+                    wc[q][w] = wd[q][w];
+                }
         }
         if (acc) {
 #pragma unroll
@@ -354,7 +412,7 @@ int iac_ppp_lap(iac_ctx *ctx, const float *d_in, float *d_out, uint32_t nframes,
         if (full > npix) return iac::fail(IAC_ERR_SHAPE, "detector panels hold more pixels than a frame");
         std::sort(sorted.begin(), sorted.end(),
                   [](const iac_panel &a, const iac_panel &b) { return a.first_ind < b.first_ind; });
-        uint64_t end_prev = 0, tiles = 0;
+        uint64_t end_prev = 0;
         for (uint32_t k = 0; k < npanels; k++) {
             const iac_panel &q = sorted[k];
             const uint64_t end = (uint64_t)q.first_ind + (uint64_t)q.fs * q.ss;
@@ -364,11 +422,7 @@ int iac_ppp_lap(iac_ctx *ctx, const float *d_in, float *d_out, uint32_t nframes,
             T.first[k] = q.first_ind;
             T.fs[k] = q.fs;
             T.ss[k] = q.ss;
-            T.tile0[k] = (uint32_t)tiles;
-            tiles += (uint64_t)((q.fs + kCols - 1) / kCols) * ((q.ss + kRows - 1) / kRows);
-            if (tiles > 0x7fffffffu) return iac::fail(IAC_ERR_LIMIT, "detector panels too large");
         }
-        T.tile0[npanels] = (uint32_t)tiles;
         T.n = npanels;
     }
     if (nframes == 0) return IAC_OK;
@@ -394,8 +448,23 @@ int iac_ppp_lap(iac_ctx *ctx, const float *d_in, float *d_out, uint32_t nframes,
     }
     IAC_TRY(iac::timer_begin(ctx));
     if (T.n) {
-        auto kern = hm ? (hg ? lap_rows_kernel<true, true> : lap_rows_kernel<true, false>)
-                       : (hg ? lap_rows_kernel<false, true> : lap_rows_kernel<false, false>);
+        // two columns per lane (8-byte accesses) where every panel starts and ends on an even
+        // pixel of an even-sized frame and every buffer is 8-byte aligned
+        auto a8 = [](const void *p) { return reinterpret_cast<uintptr_t>(p) % 8 == 0; };
+        bool pairs = npix % 2 == 0 && a8(src) && a8(d_out) && a8(d_mask) && a8(d_offset) && a8(d_gain);
+        for (uint32_t k = 0; k < T.n; k++) pairs = pairs && T.first[k] % 2 == 0 && T.fs[k] % 2 == 0;
+        const uint32_t cols = kCols * (pairs ? 2 : 1);
+        uint64_t tiles = 0;
+        for (uint32_t k = 0; k < T.n; k++) {
+            T.tile0[k] = (uint32_t)tiles;
+            tiles += (uint64_t)((T.fs[k] + cols - 1) / cols) * ((T.ss[k] + kRows - 1) / kRows);
+            if (tiles > 0x7fffffffu) return iac::fail(IAC_ERR_LIMIT, "detector panels too large");
+        }
+        T.tile0[T.n] = (uint32_t)tiles;
+        auto kern = pairs ? (hm ? (hg ? lap_rows_kernel<true, true, 2> : lap_rows_kernel<true, false, 2>)
+                                : (hg ? lap_rows_kernel<false, true, 2> : lap_rows_kernel<false, false, 2>))
+                          : (hm ? (hg ? lap_rows_kernel<true, true, 1> : lap_rows_kernel<true, false, 1>)
+                                : (hg ? lap_rows_kernel<false, true, 1> : lap_rows_kernel<false, false, 1>));
         const dim3 grid(T.tile0[T.n], std::min<uint32_t>(65535, (nframes + kF - 1) / kF));
         hipLaunchKernelGGL(kern, grid, dim3(kWave), 0, ctx->stream, src, d_out, nframes, npix, d_mask, d_offset,
                            d_gain, T, adu_per_photon, seed_threshold, combined_threshold, acc);

@@ -21,6 +21,8 @@ def layout(name):
         return 8 * 384 + 101, [pan[k] for k in order]
     if name == "odd":  # one-column, narrow, strip-sized and one-row panels, 5 pixels past them
         return 2491 + 63 + 5, [(0, 1, 70), (70, 125, 3), (445, 62, 33), (2491, 63, 1)]
+    if name == "even":  # even starts and widths (two columns per lane): 126, 2 and 248 wide
+        return 9700, [(8950, 248, 3), (0, 126, 70), (8820, 2, 65)]
     if name == "modules":  # a 1024x1024 detector as eight 1024x128 modules
         return 1024 * 1024, [(k * 1024 * 128, 1024, 128) for k in range(8)]
     raise ValueError(name)
@@ -66,6 +68,9 @@ def run_gpu(ctx, dev, I, panels, mask, offset, gain, gain_only, inplace, **kw):
     ("eight", False, False, False, True, 1),
     ("odd", True, True, False, False, 9),
     ("odd", False, False, False, False, 4),
+    ("even", True, True, False, False, 7),
+    ("even", False, False, False, True, 3),
+    ("even", True, False, False, False, 1),
 ])
 def test_lap_vs_oracle(ctx, dev, name, use_mask, use_gain, gain_only, inplace, nframes):
     npix, panels = layout(name)
