@@ -49,6 +49,7 @@ def test_null_and_bad_arguments_return_codes():
     assert L.iac_calc_kmap(None, 0, None, None, None, None) == -1
     assert L.iac_ctx_sync(None) == -1
     assert L.iac_ac3d_sparse(None, None, 0, None, None, 0, 0, 1, None) == -1
+    assert L.iac_ppp_lap(None, None, None, 0, 0, None, None, None, None, 0, 1.0, 0.5, 0.9, 0, None, None) == -1
     n = C.c_int(-1)
     assert L.iac_device_count(C.byref(n)) == 0 and n.value >= 0
 
