@@ -8,6 +8,7 @@
 
 #include "iac_fpexact.h"
 #include "iac_internal.h"
+#include "iac_serial_sum.h"
 
 #pragma clang fp contract(off)
 
@@ -64,19 +65,18 @@ __global__ void finish_kernel(uint32_t npix, uint32_t ntotal, float S, const flo
     avg[j] = v * (1.0f / (float)ntotal);
 }
 
-// One thread per frame: ArrayOperators::Sum<float> (IAC/ArrayOperators.h:260-268) is
-// a serial float sum, kept serial so the bits match; MeanIntensity = sum / P and,
-// with photon discretisation, PhotonCount = (int)sum.
-__global__ void __launch_bounds__(64)
-    frame_sums_kernel(const float *__restrict__ frames, uint32_t nframes, uint32_t npix,
-                      const int *__restrict__ mask, float T, float S, float *__restrict__ mean_intensity,
-                      int *__restrict__ photon_count)
+// One wave per frame: ArrayOperators::Sum<float> (IAC/ArrayOperators.h:260-268) of the frame's
+// values, a serial float sum in pixel order; MeanIntensity = sum / P and, with photon
+// discretisation, PhotonCount = (int) sum.
+__global__ void __launch_bounds__(iac::kSumWave)
+    frame_sums_kernel(const float *__restrict__ frames, uint32_t npix, const int *__restrict__ mask, float T, float S,
+                      float *__restrict__ mean_intensity, int *__restrict__ photon_count)
 {
-    const uint32_t f = blockIdx.x * blockDim.x + threadIdx.x;
-    if (f >= nframes) return;
-    const float *fr = frames + (size_t)f * npix;
-    float s = 0.0f;
-    for (uint32_t j = 0; j < npix; j++) s += frame_value(fr[j], mask ? mask[j] : 1, mask != nullptr, T, S);
+    const uint32_t f = blockIdx.x;
+    const float s = iac::serial_sum_wave(frames + (size_t)f * npix, npix, [=](uint32_t j, float v) {
+        return frame_value(v, mask ? mask[j] : 1, mask != nullptr, T, S);
+    });
+    if (threadIdx.x != 0) return;
     if (mean_intensity) mean_intensity[f] = s / (float)npix;
     if (photon_count && S > 0) photon_count[f] = iac::cvtt_i32(s);
 }
@@ -97,8 +97,9 @@ int iac_intensity_accumulate(iac_ctx *ctx, const float *d_frames, uint32_t nfram
                        nframes, npix, d_mask, threshold, step, d_sum, reinterpret_cast<long long *>(d_tally));
     IAC_HIP(hipGetLastError());
     if (d_mean_intensity || d_photon_count) {
-        hipLaunchKernelGGL(frame_sums_kernel, dim3((nframes + 63) / 64), dim3(64), 0, ctx->stream, d_frames, nframes,
-                           npix, d_mask, threshold, step, d_mean_intensity, d_photon_count);
+        if (nframes > 0x7fffffffu) return iac::fail(IAC_ERR_LIMIT, "too many frames in one stack");
+        hipLaunchKernelGGL(frame_sums_kernel, dim3(nframes), dim3(iac::kSumWave), 0, ctx->stream, d_frames, npix,
+                           d_mask, threshold, step, d_mean_intensity, d_photon_count);
         IAC_HIP(hipGetLastError());
     }
     return IAC_OK;

@@ -16,6 +16,7 @@
 
 #include "iac_fpexact.h"
 #include "iac_internal.h"
+#include "iac_serial_sum.h"
 
 #pragma clang fp contract(off)
 
@@ -353,31 +354,18 @@ __global__ void zero_acc_kernel(FrameAcc *acc, uint32_t n)
     if (f < n) acc[f] = FrameAcc{0, 0, 0, 0};
 }
 
-// Pass 2, one thread per frame: PhotonCount = (int) of the frame sum and MeanIntensity =
-// (float)PhotonCount / (float)npix (IAC/PPP.cpp:490-491).
-template <bool kVec4>
-__global__ void __launch_bounds__(64)
-    photon_count_kernel(const float *__restrict__ frames, uint32_t nframes, uint32_t npix,
-                        const FrameAcc *__restrict__ acc, int *__restrict__ count, float *__restrict__ mean)
+// Pass 2: PhotonCount = (int) of the frame sum and MeanIntensity = (float)PhotonCount /
+// (float)npix (IAC/PPP.cpp:490-491); the exact integer sum where it holds, else the serial float
+// sum.  One wave per frame.
+__global__ void __launch_bounds__(iac::kSumWave)
+    photon_count_kernel(const float *__restrict__ frames, uint32_t npix, const FrameAcc *__restrict__ acc,
+                        int *__restrict__ count, float *__restrict__ mean)
 {
-    const uint32_t f = blockIdx.x * blockDim.x + threadIdx.x;
-    if (f >= nframes) return;
-    const float *fr = frames + (size_t)f * npix;
-    float s = 0.0f;
-    if (!acc[f].bad && acc[f].mag <= (1ull << 24)) {
-        s = (float)acc[f].sum;
-    } else if (kVec4) {
-        const float4 *v = reinterpret_cast<const float4 *>(fr);
-        for (uint32_t j = 0; j < npix / 4; j++) {
-            const float4 x = v[j];
-            s += x.x;
-            s += x.y;
-            s += x.z;
-            s += x.w;
-        }
-    } else {
-        for (uint32_t j = 0; j < npix; j++) s += fr[j];
-    }
+    const uint32_t f = blockIdx.x;
+    const bool exact = !acc[f].bad && acc[f].mag <= (1ull << 24);
+    const float s = exact ? (float)acc[f].sum
+                          : iac::serial_sum_wave(frames + (size_t)f * npix, npix, [](uint32_t, float v) { return v; });
+    if (threadIdx.x != 0) return;
     const int pc = iac::cvtt_i32(s);
     if (count) count[f] = pc;
     if (mean) mean[f] = (float)pc / (float)npix;
@@ -426,7 +414,8 @@ int iac_ppp_lap(iac_ctx *ctx, const float *d_in, float *d_out, uint32_t nframes,
         T.n = npanels;
     }
     if (nframes == 0) return IAC_OK;
-    if ((uint64_t)nframes * npix > ((uint64_t)1 << 40)) return iac::fail(IAC_ERR_LIMIT, "frame stack too large");
+    if ((uint64_t)nframes * npix > ((uint64_t)1 << 40) || nframes > 0x7fffffffu)
+        return iac::fail(IAC_ERR_LIMIT, "frame stack too large");
     const size_t bytes = (size_t)nframes * npix * sizeof(float);
     const float *src = d_in;
     if (d_in == d_out) {
@@ -481,9 +470,8 @@ int iac_ppp_lap(iac_ctx *ctx, const float *d_in, float *d_out, uint32_t nframes,
     }
     IAC_TRY(iac::timer_end(ctx));
     if (acc) {
-        const bool v4 = npix % 4 == 0 && reinterpret_cast<uintptr_t>(d_out) % 16 == 0;
-        hipLaunchKernelGGL(v4 ? photon_count_kernel<true> : photon_count_kernel<false>, dim3((nframes + 63) / 64),
-                           dim3(64), 0, ctx->stream, d_out, nframes, npix, acc, d_photon_count, d_mean_intensity);
+        hipLaunchKernelGGL(photon_count_kernel, dim3(nframes), dim3(iac::kSumWave), 0, ctx->stream, d_out, npix, acc,
+                           d_photon_count, d_mean_intensity);
         IAC_HIP(hipGetLastError());
     }
     return IAC_OK;
