@@ -76,3 +76,9 @@ def test_host_library_and_eval_driver():
     assert r.returncode == 0 and "abi" in r.stdout
     r = subprocess.run([str(pkg / "iac_eval")], capture_output=True, text=True, timeout=60)
     assert r.returncode == 2 and "usage" in r.stderr
+    # the LAP mode checks its arguments before touching a device: wrong count, no panels,
+    # panels larger than the frame
+    for args in (["--lap"], ["--lap", "x", "1", "10", "", "1", "0.5", "0.9", "y"],
+                 ["--lap", "x", "1", "10", "0:4:4", "1", "0.5", "0.9", "y"]):
+        r = subprocess.run([str(pkg / "iac_eval"), *args], capture_output=True, text=True, timeout=60)
+        assert r.returncode == 2 and "--lap" in r.stderr, args
