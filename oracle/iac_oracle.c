@@ -23,6 +23,24 @@ int ora_cvtt_i32(float x)
 #include <omp.h>
 #endif
 
+/* Planted mistakes for the pin tests (tests/test_oracle_slips.py).  The shipped
+ * oracle is built with ORA_SLIP = 0, where every SLIP(n) is a constant false and
+ * the compiler drops the branch.  A test build with -DORA_SLIP=n plants mistake n:
+ *   1  AC1D AC_uw: self pairs weighted 1 instead of 2
+ *   2  AC1D AC_uw nearest neighbour: floor(q) instead of floor(q + 0.5)
+ *   3  AC1D AC_uw linear: Sep and the split values in double, no float rounding
+ *   4  AC3D linear second level: the x weights of the two corners swapped
+ *   5  AC3D linear first level: the x weights of the two corners swapped
+ *   6  C(q) small linear: the x weights of the two corners swapped
+ *   7  Merge_CQ: the transposed rotation matrix
+ *   8  Merge_CQ linear: the x weights of the two corners swapped
+ *   9  AC1D C(q) linear: Sep given to the lower bin
+ *  10  AC1D C(q) nearest neighbour: floor(q) instead of floor(q + 0.5) */
+#ifndef ORA_SLIP
+#define ORA_SLIP 0
+#endif
+#define SLIP(n) (ORA_SLIP == (n))
+
 /* ------------------------------------------------------------------------- */
 /* Fixed point: Settings::FloatToInt / IntToFloat, IAC/Settings.h:162-187.
  * F_I_Conversion is {Offset = 0.0f, Step = 0.001f} and never configured. */
@@ -244,6 +262,7 @@ static void add_rotated(uint64_t *mesh, const ora_mesh_shape *s, const float q[3
         double sfs = (double)(1.0f - (q[0] - floorf(q[0])));
         double sms = (double)(1.0f - (q[1] - floorf(q[1])));
         double sss = (double)(1.0f - (q[2] - floorf(q[2])));
+        if (SLIP(4)) sfs = 1 - sfs;
         int fs = (int)floorf(q[0]) + c0;
         int ms = (int)floorf(q[1]) + c1;
         int ss = (int)floorf(q[2]) + c2;
@@ -308,6 +327,7 @@ static void add_q_entry(uint64_t *mesh, const ora_mesh_shape *s, float q[3], con
                 qq[2] = (float)((ss_l + dz[c]) - c2);
                 rotate(qq, R);
                 double wx = dx[c] ? (1 - sfs) : sfs;
+                if (SLIP(5)) wx = 1 - wx;
                 double wy = dy[c] ? (1 - sms) : sms;
                 double wz = dz[c] ? (1 - sss) : sss;
                 float w = (float)(((v * wx) * wy) * wz);
@@ -448,6 +468,7 @@ void ora_cq_small(const float *intensity, const float *kmap, uint32_t npix, cons
                 double sfs = (double)(1.0f - (q[ai] - floorf(q[ai])));
                 double sms = (double)(1.0f - (q[bi] - floorf(q[bi])));
                 double sss = (double)(1.0f - (q[ci] - floorf(q[ci])));
+                if (SLIP(6)) sfs = 1 - sfs;
                 int fs = (int)(floorf(q[ai]) + (float)cab);
                 int ms = (int)(floorf(q[bi]) + (float)cab);
                 int ss = (int)(floorf(q[ci]) + (float)cc);
@@ -537,9 +558,15 @@ void ora_merge_cq(const double *small_cq, const ora_mesh_shape *ss_, const float
         for (uint32_t i = 0; i < nev; i++) {
             const float *R = rw10 + 10 * (size_t)i;
             float q[3];
-            q[0] = R[0] * q_in[0] + R[1] * q_in[1] + R[2] * q_in[2];
-            q[1] = R[3] * q_in[0] + R[4] * q_in[1] + R[5] * q_in[2];
-            q[2] = R[6] * q_in[0] + R[7] * q_in[1] + R[8] * q_in[2];
+            if (SLIP(7)) {
+                q[0] = R[0] * q_in[0] + R[3] * q_in[1] + R[6] * q_in[2];
+                q[1] = R[1] * q_in[0] + R[4] * q_in[1] + R[7] * q_in[2];
+                q[2] = R[2] * q_in[0] + R[5] * q_in[1] + R[8] * q_in[2];
+            } else {
+                q[0] = R[0] * q_in[0] + R[1] * q_in[1] + R[2] * q_in[2];
+                q[1] = R[3] * q_in[0] + R[4] * q_in[1] + R[5] * q_in[2];
+                q[2] = R[6] * q_in[0] + R[7] * q_in[1] + R[8] * q_in[2];
+            }
             double w = (double)R[9];
             if (interp == 0) {
                 uint64_t vout = (uint64_t)((val_in * w) * mult);
@@ -552,6 +579,7 @@ void ora_merge_cq(const double *small_cq, const ora_mesh_shape *ss_, const float
                 double sfs = (double)(1.0f - (q[0] - floorf(q[0])));
                 double sms = (double)(1.0f - (q[1] - floorf(q[1])));
                 double sss = (double)(1.0f - (q[2] - floorf(q[2])));
+                if (SLIP(8)) sfs = 1 - sfs;
                 int fs = (int)(floorf(q[0]) + (float)cab);
                 int ms = (int)(floorf(q[1]) + (float)cab);
                 int ss = (int)(floorf(q[2]) + (float)cab);
@@ -679,12 +707,14 @@ void ora_ac1d_cq(const float *intensity, const float *kmap, uint32_t npix, uint3
             if (interp == 0) {
                 int64_t vc = (int64_t)(val * mult);
                 uint32_t sc = (uint32_t)(floor((double)q + 0.5));
+                if (SLIP(10)) sc = (uint32_t)floorf(q);
                 hist_add(hist, size, sc, (uint64_t)vc, dropped);
             }
             if (interp == 1) {
                 uint32_t sc1 = (uint32_t)(floorf(q));
                 uint32_t sc2 = sc1 + 1;
                 double sep = (double)(q - floorf(q));
+                if (SLIP(9)) sep = 1 - sep;
                 int64_t v1 = (int64_t)((val * (1 - sep)) * mult);
                 int64_t v2 = (int64_t)((val * sep) * mult);
                 hist_add(hist, size, sc1, (uint64_t)v1, dropped);
@@ -707,8 +737,10 @@ void ora_ac1d_acuw(const float *hl4, uint32_t n, uint32_t size, float max_q, flo
             if (q > max_q) continue;
             q = q / dqdx;
             float f2 = (j == k && self_pair_factor == 1) ? 1.0f : 2.0f;
+            if (SLIP(1) && j == k) f2 = 1.0f;
             if (interp == 0) {
                 uint32_t sc = (uint32_t)(floor((double)q + 0.5));
+                if (SLIP(2)) sc = (uint32_t)floorf(q);
                 if (sc >= size) { (*dropped)++; continue; }
                 ac[sc] += (double)(f2 * (a[3] * b[3]));
             } else if (interp == 1) {
@@ -717,6 +749,14 @@ void ora_ac1d_acuw(const float *hl4, uint32_t n, uint32_t size, float max_q, flo
                 float sep = q - (float)sc1;
                 float v1 = f2 * ((a[3] * b[3]) * (1 - sep));
                 float v2 = f2 * ((a[3] * b[3]) * (sep));
+                if (SLIP(3)) {
+                    double dsep = (double)q - floor((double)q);
+                    double d1 = (double)f2 * ((double)a[3] * (double)b[3] * (1 - dsep));
+                    double d2 = (double)f2 * ((double)a[3] * (double)b[3] * dsep);
+                    if (sc1 < size) ac[sc1] += d1; else (*dropped)++;
+                    if (sc2 < size) ac[sc2] += d2; else (*dropped)++;
+                    continue;
+                }
                 if (sc1 < size) ac[sc1] += (double)v1; else (*dropped)++;
                 if (sc2 < size) ac[sc2] += (double)v2; else (*dropped)++;
             }

@@ -102,18 +102,41 @@ _SIG = {
 _lib = None
 
 
+def _open(path) -> C.CDLL:
+    L = C.CDLL(str(path))
+    for k, (r, a) in _SIG.items():
+        fn = getattr(L, k)
+        fn.restype = r
+        fn.argtypes = a
+    return L
+
+
 def lib() -> C.CDLL:
     global _lib
     if _lib is None:
         if not LIB_PATH.exists():
             raise ImportError(f"{LIB_PATH} missing: run `make -C oracle` or __graft_entry__.build()")
-        L = C.CDLL(str(LIB_PATH))
-        for k, (r, a) in _SIG.items():
-            fn = getattr(L, k)
-            fn.restype = r
-            fn.argtypes = a
-        _lib = L
+        _lib = _open(LIB_PATH)
     return _lib
+
+
+class use_library:
+    """Context manager: every wrapper below calls the oracle built at `path`
+    (a build with a planted mistake, -DORA_SLIP=n) until the block ends."""
+
+    def __init__(self, path):
+        self.path = path
+
+    def __enter__(self):
+        global _lib
+        self.saved = lib()
+        _lib = _open(self.path)
+        return _lib
+
+    def __exit__(self, *exc):
+        global _lib
+        _lib = self.saved
+        return False
 
 
 def _f(a):
