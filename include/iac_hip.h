@@ -98,7 +98,9 @@ int iac_ctx_timing(iac_ctx *ctx, double *ms, uint64_t *launches, int reset);
                                    neighbour, 4 for C(q) linear, else the whole grid) */
 #define IAC_OPT_PAIR_MODE 9 /* LDS kernel: 0 default (1 for AC_uw, 2 for C(q)), 1 lanes on consecutive pairs,
                                 2 a contiguous pair range per thread */
-#define IAC_OPT_SIM_CHUNK_ROWS 4 /* simulator: detector rows (pixel x subpixel) per GEMM, 0 = auto (1 GiB of FP32 field) */
+#define IAC_OPT_SIM_CHUNK_ROWS 4 /* simulator with subpixels or modes: detector rows (pixel x subpixel) per GEMM
+                                    launch, rounded to whole groups of 256 pixels, 0 = auto (1 GiB of FP32
+                                    magnitudes); one subpixel and one mode need no chunks */
 int iac_ctx_set_option(iac_ctx *ctx, int option, int64_t value);
 /* Updates whose voxel fell outside the mesh since the context was created.
  * The reference writes such updates out of bounds; this path drops them. */
@@ -171,6 +173,31 @@ int iac_ac3d_batch_create(iac_ctx *ctx, const float *d_kmap, uint32_t npix, cons
 int iac_ac3d_batch_launch(iac_ctx *ctx, iac_ac3d_batch *batch, uint64_t *d_mesh, int zero_mesh);
 int iac_ac3d_batch_capture(iac_ctx *ctx, iac_ac3d_batch *batch, uint64_t *d_mesh, int zero_mesh);
 int iac_ac3d_batch_destroy(iac_ac3d_batch *batch);
+
+/* A frame queue: the per-event call of the reference's frame loop,
+ * Detector::AutoCorrelateSparseList once per event (IAC/RunIAC.cpp:524-550,
+ * IAC/Detector.cpp:1010-1181), at the speed of the batched pass.  Each push appends one
+ * frame's hit list {kx,ky,kz,value} x n and its row-major rotation to the queue (host
+ * records are copied into pinned staging; device records by one device-to-device copy on
+ * the context's stream); nothing is binned until the queue fills (max_records records or
+ * max_frames frames; 0 selects 2^22 and 4096) or is flushed.  A flush uploads the pending
+ * frames and enqueues one pass of the LDS cell-box engine (per-frame rotations included
+ * when the second level is nearest-neighbour and every rotated voxel stays in the mesh,
+ * else the tile kernel), with no allocation; the host waits only for the previous flush's
+ * uploads before it writes the staging again.  The mesh equals calling iac_ac3d_sparse once
+ * per frame.  d_kmap ([npix][3]) sets the engine's cell grid (records may hold any k);
+ * d_kmap and d_mesh must outlive the queue, and the queue must be destroyed before its
+ * context.  After iac_ac3d_queue_flush the mesh is complete in stream order;
+ * iac_ac3d_queue_destroy flushes and synchronises. */
+typedef struct iac_ac3d_queue iac_ac3d_queue;
+int iac_ac3d_queue_create(iac_ctx *ctx, const float *d_kmap, uint32_t npix, const iac_mesh_shape *shape,
+                          int interp1, int interp2, int double_map, uint64_t *d_mesh, uint64_t max_records,
+                          uint32_t max_frames, iac_ac3d_queue **out);
+int iac_ac3d_queue_push(iac_ac3d_queue *queue, const float *h_hitlist4, uint32_t n, const float rot[9]);
+int iac_ac3d_queue_push_device(iac_ac3d_queue *queue, const float *d_hitlist4, uint32_t n, const float rot[9]);
+int iac_ac3d_queue_flush(iac_ac3d_queue *queue);
+int iac_ac3d_queue_pending(const iac_ac3d_queue *queue, uint32_t *frames, uint64_t *records);
+int iac_ac3d_queue_destroy(iac_ac3d_queue *queue);
 
 /* ---- sparsification ------------------------------------------------------------- */
 /* Detector::CreateSparseHitList(Threshold, PhotonSamplingStep) after the pixel-mask

@@ -74,6 +74,9 @@ public:
 
     // device pool in place of IAC/Settings.cpp:331-470
     void SetUp_GPU(int MaxDevices = -1);
+    // A pool of contexts the caller made and keeps (e.g. ones bound to its own streams):
+    // the Settings reserves and frees them like its own but never destroys them.
+    void UseContexts(const std::vector<iac_ctx *> &Contexts);
     int GPU_ReserveDevice();  // -1 if every device is taken
     void GPU_FreeDevice(int DeviceIndex);
     iac_ctx *Context(int DeviceIndex);
@@ -91,6 +94,7 @@ public:
 private:
     std::vector<iac_ctx *> ctxs_;
     std::vector<bool> available_;
+    bool owns_ctxs_ = true;
     std::mutex pool_mutex_;
 };
 
@@ -217,12 +221,22 @@ public:
     iac_ctx *ctx = nullptr;      // the context the device meshes live on
     uint64_t *DeviceMesh();      // allocates and zeroes d_Mesh on first use
     double *DeviceCQMesh();
-    void Download();             // d_Mesh -> Mesh, d_CQMesh -> CQMesh
+    void Download();             // Flush(), then d_Mesh -> Mesh, d_CQMesh -> CQMesh
     iac_mesh_shape CShape() const;
+    // The frame queue of d_Mesh that Detector::AutoCorrelateSparseList appends events to
+    // (made on first use for the detector's device k-map and the binning flags; other
+    // flags or another k-map flush it and make a new one).  Flush() enqueues the binning
+    // of the queued events (no synchronisation); ~ACMesh and Download() flush as well.
+    iac_ac3d_queue *Queue(const float *d_kmap, uint32_t npix, int interp1, int interp2, bool double_map);
+    void Flush();
 
 private:
     void reset_device();
     void take_shape(const iac_mesh_shape &s);
+    void drop_queue();
+    iac_ac3d_queue *queue_ = nullptr;
+    const float *queue_kmap_ = nullptr;
+    int queue_flags_ = -1;
 };
 
 // IAC/AC1D.h

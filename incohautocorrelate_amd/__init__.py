@@ -302,6 +302,17 @@ class Context:
         self._plans.add(b)
         return b
 
+    def ac3d_queue(self, kmap, shape: MeshShape, mesh, interp1: int = NEAREST, interp2: int = NEAREST,
+                   double_map: bool = True, max_records: int = 0, max_frames: int = 0) -> "Ac3dQueue":
+        """A frame queue: Detector::AutoCorrelateSparseList once per event, binned in batches."""
+        h = C.c_void_p()
+        check(lib().iac_ac3d_queue_create(self._h, _ptr(kmap), kmap.numel() // 3, C.byref(shape), interp1, interp2,
+                                          int(double_map), _ptr(mesh), int(max_records), int(max_frames),
+                                          C.byref(h)), "iac_ac3d_queue_create")
+        q = Ac3dQueue(self, h, (kmap, mesh))
+        self._plans.add(q)
+        return q
+
     # ---- measurement probes ----
     def probe_atomics(self, mesh, n_updates: int, mode: int, hot: int = 0) -> float:
         ms = C.c_float()
@@ -385,6 +396,46 @@ class Ac3dBatch:
         if self.handle:
             lib().iac_ac3d_batch_destroy(self.handle)
             self.handle = None
+
+    def __del__(self):
+        try:
+            self.destroy()
+        except Exception:
+            pass
+
+
+class Ac3dQueue:
+    """Owns an iac_ac3d_queue; destroy() flushes, synchronises and frees it."""
+
+    def __init__(self, ctx: Context, handle: C.c_void_p, keep: tuple):
+        self.ctx, self.handle, self._keep = ctx, handle, keep
+
+    def push(self, hitlist: np.ndarray, rot=None) -> None:
+        """One frame's host hit list {kx,ky,kz,value} x n (float32) and rotation."""
+        hl = np.ascontiguousarray(hitlist, np.float32).reshape(-1, 4)
+        r = _f32(rot if rot is not None else np.eye(3, dtype=np.float32), 9)
+        check(lib().iac_ac3d_queue_push(self.handle, hl.ctypes.data if len(hl) else None, len(hl), fptr(r)),
+              "iac_ac3d_queue_push")
+
+    def push_device(self, hitlist, rot=None) -> None:
+        """One frame's device hit list ([n][4] float32 tensor) and rotation."""
+        n = hitlist.shape[0] if hitlist is not None else 0
+        r = _f32(rot if rot is not None else np.eye(3, dtype=np.float32), 9)
+        check(lib().iac_ac3d_queue_push_device(self.handle, _ptr(hitlist) if n else None, n, fptr(r)),
+              "iac_ac3d_queue_push_device")
+
+    def flush(self) -> None:
+        check(lib().iac_ac3d_queue_flush(self.handle), "iac_ac3d_queue_flush")
+
+    def pending(self) -> tuple[int, int]:
+        f, r = C.c_uint32(), C.c_uint64()
+        check(lib().iac_ac3d_queue_pending(self.handle, C.byref(f), C.byref(r)), "iac_ac3d_queue_pending")
+        return int(f.value), int(r.value)
+
+    def destroy(self) -> None:
+        if self.handle:
+            h, self.handle = self.handle, None
+            check(lib().iac_ac3d_queue_destroy(h), "iac_ac3d_queue_destroy")
 
     def __del__(self):
         try:

@@ -101,6 +101,12 @@ SIGNATURES: dict[str, tuple] = {
     "iac_ac3d_batch_launch": (I32, [P, P, P, I32]),
     "iac_ac3d_batch_capture": (I32, [P, P, P, I32]),
     "iac_ac3d_batch_destroy": (I32, [P]),
+    "iac_ac3d_queue_create": (I32, [P, P, U32, C.POINTER(MeshShape), I32, I32, I32, P, U64, U32, C.POINTER(P)]),
+    "iac_ac3d_queue_push": (I32, [P, P, U32, F3]),
+    "iac_ac3d_queue_push_device": (I32, [P, P, U32, F3]),
+    "iac_ac3d_queue_flush": (I32, [P]),
+    "iac_ac3d_queue_pending": (I32, [P, C.POINTER(U32), C.POINTER(U64)]),
+    "iac_ac3d_queue_destroy": (I32, [P]),
     "iac_average_intensity": (I32, [P, P, U32, U32, P, C.c_float, C.c_float, P, P, P]),
     "iac_intensity_accumulate": (I32, [P, P, U32, U32, P, C.c_float, C.c_float, P, P, P, P]),
     "iac_intensity_finish": (I32, [P, U32, U32, C.c_float, P, P, P]),

@@ -64,7 +64,7 @@ def build_hip(verbose: bool = True) -> Path:
         objs = list(ex.map(_compile, srcs))
     tmp = LIB.with_suffix(".so.tmp")
     _run([HIPCC, "--offload-arch=gfx950", "-shared", "-fPIC", "-o", str(tmp), *map(str, objs),
-          "-L/opt/rocm/lib", "-lrocblas", "-Wl,-rpath,/opt/rocm/lib"])
+          "-Wl,-rpath,/opt/rocm/lib"])
     os.replace(tmp, LIB)
     if verbose:
         print(f"built {LIB}")

@@ -27,7 +27,6 @@ struct iac_ctx {
     std::vector<hipEvent_t> ev_start, ev_stop;
     size_t ev_used = 0;
     std::vector<iac_pair_plan *> pair_plans;  // small cache, oldest first
-    void *rocblas = nullptr;                   // rocblas_handle of the simulator GEMM, made on first use
     // options (IAC_OPT_*)
     int ac3d_kernel = 0;
     int cell_vox = 0;     // pair engine: cell side along u in voxels (0 = per kind and walk)
@@ -35,7 +34,7 @@ struct iac_ctx {
     int task_region = 0;  // pair engine: cells per task region side (0 = per kind)
     int pair_mode = 0;    // pair engine: 0/1 lanes on consecutive pairs, 2 a pair range per thread
     uint32_t frame_chunk = 0;
-    uint64_t sim_chunk_rows = 0;
+    uint64_t sim_chunk_rows = 0;  // simulator: rows (pixel x subpixel) per GEMM launch with subpixels or modes
     int box_cap = 0;        // pair engine: LDS box voxels, 1024, 1920 or 4096 (0 = per kind)
     int pair_prefetch = 0;  // pair engine: pairs per load batch (1, 4 or 8; 0 = 1)
 };

@@ -55,4 +55,15 @@ int pairs_prepare(iac_ctx *ctx, const PairRun &run, PairBatch **out);
 int pairs_enqueue(PairBatch *batch, hipStream_t stream, uint64_t *d_mesh);
 void pairs_batch_free(PairBatch *batch);
 
+// Frame queues: buffers for up to max_records records and max_frames frames made once,
+// the plan of R0 (spec, ordered, k-map) pinned in the context's cache.  pairs_queue_run
+// bins the frames of run (rec, d_frame_off, nframes, n_total, d_rots / h_rots, d_mesh)
+// given their offsets on the host too (h_off, nframes + 1): it builds the chunk table on
+// the host, uploads it from pinned memory and enqueues, with no allocation and no host
+// synchronisation beyond waiting for the previous run's upload of that table.
+struct PairQueue;
+int pairs_queue_create(iac_ctx *ctx, const PairRun &R0, uint64_t max_records, uint32_t max_frames, PairQueue **out);
+int pairs_queue_run(PairQueue *queue, const PairRun &run, const uint64_t *h_off);
+void pairs_queue_free(PairQueue *queue);
+
 }  // namespace iac

@@ -1150,6 +1150,7 @@ struct iac_pair_plan {
     // product
     std::vector<iac::Task> tasks;
     iac::Task *d_tasks = nullptr;
+    int pins = 0;  // frame queues holding this plan (never evicted while > 0)
 };
 
 namespace iac {
@@ -1329,12 +1330,15 @@ int get_plan(iac_ctx *ctx, hipStream_t st, const PairRun &R, int cell_vox, int b
     iac_pair_plan *pl = find_plan(ctx, R.d_kmap, R.npix, cell_vox, box_cap, region, S);
     if (pl && (std::memcmp(&pl->g, &g, sizeof(g)) != 0 || pl->cell_ext != cext)) pl = nullptr;
     if (!pl) {
-        // reuse the oldest slot beyond four cached plans
+        // reuse the oldest unpinned slot beyond four cached plans
         if (ctx->pair_plans.size() >= 4) {
-            iac_pair_plan *old = ctx->pair_plans.front();
-            if (old->d_tasks) IAC_HIP(hipFree(old->d_tasks));
-            delete old;
-            ctx->pair_plans.erase(ctx->pair_plans.begin());
+            for (auto it = ctx->pair_plans.begin(); it != ctx->pair_plans.end(); ++it) {
+                if ((*it)->pins > 0) continue;
+                if ((*it)->d_tasks) IAC_HIP(hipFree((*it)->d_tasks));
+                delete *it;
+                ctx->pair_plans.erase(it);
+                break;
+            }
         }
         pl = new iac_pair_plan();
         ctx->pair_plans.push_back(pl);
@@ -1362,10 +1366,9 @@ int get_plan(iac_ctx *ctx, hipStream_t st, const PairRun &R, int cell_vox, int b
 
 // Frame chunks inside each group, balanced by pair work: a chunk closes once its
 // frames' n^2 reach fchunk times the mean per frame, or at 4 fchunk frames.
-std::vector<Chunk> make_chunks(const std::vector<uint64_t> &off, const uint32_t *group_off, uint32_t ngroups,
+std::vector<Chunk> make_chunks(const uint64_t *off, uint32_t nframes, const uint32_t *group_off, uint32_t ngroups,
                                uint32_t fchunk, const float *h_rots = nullptr)
 {
-    const uint32_t nframes = (uint32_t)off.size() - 1;
     double total = 0;
     for (uint32_t f = 0; f < nframes; f++) {
         const double n = (double)(off[f + 1] - off[f]);
@@ -1579,7 +1582,7 @@ int pairs_run(iac_ctx *ctx, const PairRun &R)
     std::vector<uint64_t> off(R.nframes + 1);
     IAC_HIP(hipMemcpyAsync(off.data(), R.d_frame_off, off.size() * 8, hipMemcpyDeviceToHost, st));
     IAC_HIP(hipStreamSynchronize(st));
-    const std::vector<Chunk> chunks = make_chunks(off, R.h_group_off, R.ngroups, fchunk_of(ctx), R.h_rots);
+    const std::vector<Chunk> chunks = make_chunks(off.data(), R.nframes, R.h_group_off, R.ngroups, fchunk_of(ctx), R.h_rots);
     const int nc = pl->g.ncu * pl->g.ncv;
     const Sizes z = sizes_of(R, nc, chunks.size());
     // scratch: sorted records + cell offsets (slot 1), H + group offsets + self + chunks (slot 2)
@@ -1632,7 +1635,7 @@ int pairs_prepare(iac_ctx *ctx, const PairRun &R0, PairBatch **out)
     B->R = R0;
     B->group_off.assign(R0.h_group_off, R0.h_group_off + R0.ngroups + 1);
     B->R.h_group_off = B->group_off.data();
-    const std::vector<Chunk> chunks = make_chunks(off, B->R.h_group_off, R0.ngroups, fchunk_of(ctx));
+    const std::vector<Chunk> chunks = make_chunks(off.data(), R0.nframes, B->R.h_group_off, R0.ngroups, fchunk_of(ctx));
     const int nc = pl->g.ncu * pl->g.ncv;
     const Sizes z = sizes_of(B->R, nc, chunks.size());
     const size_t tbytes = std::max<size_t>(1, pl->tasks.size()) * sizeof(Task);
@@ -1676,7 +1679,8 @@ int pairs_enqueue(PairBatch *B, hipStream_t st, uint64_t *d_mesh)
 {
     PairRun R = B->R;
     R.d_mesh = d_mesh;
-    return enqueue(B->ctx, st, R, B->b, false);
+    // the pair kernel is timed on the context's own stream (not inside a capture)
+    return enqueue(B->ctx, st, R, B->b, st == B->ctx->stream);
 }
 
 void pairs_batch_free(PairBatch *B)
@@ -1684,6 +1688,113 @@ void pairs_batch_free(PairBatch *B)
     if (!B) return;
     if (B->mem) (void)hipFree(B->mem);
     delete B;
+}
+
+// ---- frame queues --------------------------------------------------------------
+
+struct PairQueue {
+    iac_ctx *ctx = nullptr;
+    PairRun R0{};
+    iac_pair_plan *pl = nullptr;
+    uint64_t max_records = 0;
+    uint32_t max_frames = 0;
+    void *mem = nullptr;        // srec | cs | H | goff | self | chunks (device)
+    Bufs b{};
+    Chunk *h_chunks = nullptr;  // pinned: the chunk table and group offsets of the last run
+    uint32_t *h_goff = nullptr;
+    hipEvent_t staged = nullptr;  // the last run's uploads from h_chunks / h_goff are done
+};
+
+int pairs_queue_create(iac_ctx *ctx, const PairRun &R0, uint64_t max_records, uint32_t max_frames, PairQueue **out)
+{
+    *out = nullptr;
+    if (max_records == 0 || max_records >= (1ull << 32) || max_frames == 0)
+        return fail(IAC_ERR_LIMIT, "pair engine queue: 1 .. 2^32 - 1 records and at least one frame");
+    iac_pair_plan *pl = nullptr;
+    IAC_TRY(get_plan(ctx, ctx->stream, R0, cell_vox_of(ctx, R0), box_cap_of(ctx, R0), region_of(ctx, R0), &pl));
+    auto *Q = new PairQueue();
+    Q->ctx = ctx;
+    Q->R0 = R0;
+    Q->R0.ngroups = 1;
+    Q->R0.n_total = max_records;
+    Q->R0.nframes = max_frames;
+    Q->max_records = max_records;
+    Q->max_frames = max_frames;
+    const int nc = pl->g.ncu * pl->g.ncv;
+    const Sizes z = sizes_of(Q->R0, nc, max_frames);
+    hipError_t e = hipMalloc(&Q->mem, z.srec + z.cs + z.h + z.goff + z.self + z.chunks);
+    if (e == hipSuccess)
+        e = hipHostMalloc((void **)&Q->h_chunks, (size_t)max_frames * sizeof(Chunk) + 64, hipHostMallocDefault);
+    if (e == hipSuccess) e = hipEventCreateWithFlags(&Q->staged, hipEventDisableTiming);
+    if (e == hipSuccess) e = hipEventRecord(Q->staged, ctx->stream);
+    if (e != hipSuccess) {
+        if (Q->mem) (void)hipFree(Q->mem);
+        if (Q->h_chunks) (void)hipHostFree(Q->h_chunks);
+        if (Q->staged) (void)hipEventDestroy(Q->staged);
+        delete Q;
+        return hip_fail(e, "pair engine queue allocation", __FILE__, __LINE__);
+    }
+    Q->h_goff = reinterpret_cast<uint32_t *>(Q->h_chunks + max_frames);
+    char *p = static_cast<char *>(Q->mem);
+    Bufs &b = Q->b;
+    b.srec = reinterpret_cast<float4 *>(p);
+    b.cs = reinterpret_cast<uint32_t *>(p + z.srec);
+    b.H = reinterpret_cast<unsigned long long *>(p + z.srec + z.cs);
+    b.d_goff = reinterpret_cast<uint32_t *>(p + z.srec + z.cs + z.h);
+    b.d_self = reinterpret_cast<unsigned long long *>(p + z.srec + z.cs + z.h + z.goff);
+    b.d_flag = reinterpret_cast<uint32_t *>(b.d_self + 1);
+    b.d_chunks = reinterpret_cast<Chunk *>(p + z.srec + z.cs + z.h + z.goff + z.self);
+    b.d_tasks = pl->d_tasks;
+    b.ntasks = (uint32_t)pl->tasks.size();
+    b.g = pl->g;
+    b.box_cap = pl->box_cap;
+    pl->pins++;
+    Q->pl = pl;
+    *out = Q;
+    return IAC_OK;
+}
+
+int pairs_queue_run(PairQueue *Q, const PairRun &R, const uint64_t *h_off)
+{
+    if (R.nframes == 0 || R.n_total == 0) return IAC_OK;
+    if (R.nframes > Q->max_frames || R.n_total > Q->max_records)
+        return fail(IAC_ERR_LIMIT, "pair engine queue: run larger than the queue");
+    hipStream_t st = Q->ctx->stream;
+    const uint32_t goff[2] = {0, R.nframes};
+    const std::vector<Chunk> chunks = make_chunks(h_off, R.nframes, goff, 1, fchunk_of(Q->ctx), R.h_rots);
+    // the previous run's uploads read the pinned table: wait for them before rewriting it
+    IAC_HIP(hipEventSynchronize(Q->staged));
+    std::copy(chunks.begin(), chunks.end(), Q->h_chunks);
+    Q->h_goff[0] = 0;
+    Q->h_goff[1] = R.nframes;
+    IAC_HIP(hipMemcpyAsync(Q->b.d_goff, Q->h_goff, 2 * sizeof(uint32_t), hipMemcpyHostToDevice, st));
+    if (!chunks.empty())
+        IAC_HIP(hipMemcpyAsync(Q->b.d_chunks, Q->h_chunks, chunks.size() * sizeof(Chunk), hipMemcpyHostToDevice, st));
+    IAC_HIP(hipEventRecord(Q->staged, st));
+    PairRun RR = Q->R0;
+    RR.rec = R.rec;
+    RR.d_frame_off = R.d_frame_off;
+    RR.nframes = R.nframes;
+    RR.n_total = R.n_total;
+    RR.h_group_off = goff;
+    RR.ngroups = 1;
+    RR.d_mesh = R.d_mesh;
+    RR.d_rots = R.d_rots;
+    RR.h_rots = R.h_rots;
+    Bufs b = Q->b;
+    b.nchunks = (uint32_t)chunks.size();
+    return enqueue(Q->ctx, st, RR, b, true);
+}
+
+void pairs_queue_free(PairQueue *Q)
+{
+    if (!Q) return;
+    (void)hipStreamSynchronize(Q->ctx->stream);
+    if (Q->pl) Q->pl->pins--;
+    if (Q->mem) (void)hipFree(Q->mem);
+    if (Q->h_chunks) (void)hipHostFree(Q->h_chunks);
+    if (Q->staged) (void)hipEventDestroy(Q->staged);
+    delete Q;
 }
 
 }  // namespace iac

@@ -6,14 +6,21 @@
 //   direct FP32: the reference kernel's arithmetic, one thread per (pixel, frame),
 //     emitters staged through LDS.  The plain baseline.
 //   complex GEMM: psi = A B with A[p,e] = exp(i k_p . r_e) (phase argument and its
-//     reduction in FP64, once per geometry) and B[e,f] = exp(i phi_{e,f}).  As one
-//     real GEMM Z = X Y: X = [cos | sin] (rows x 2E), Y[f] = [cos phi | -sin phi],
-//     Y[F+f] = [sin phi | cos phi], Z = [Re psi | Im psi].  X and Y are stored split,
-//     x = hi + lo in bf16 or fp16; Z = Xh Yh + Xh Yl + Xl Yh on MFMA with FP32
-//     accumulation recovers ~16 bits per element (the dropped Xl Yl is ~2^-16 of a term).
-// The epilogue forms |psi| (reference, kernel.cl:844) or |psi|^2, averaged over subpixels.
-#include <rocblas/rocblas.h>
-
+//     reduction in FP64, once per geometry: the plan) and B[e,f] = exp(i phi_{e,f}).
+//     With C + iS = A and c + is = B:  Re psi = C c - S s,  Im psi = C s + S c.
+//     Every operand is stored split, x = hi + lo in bf16 or fp16, and each product
+//     is hi*hi + hi*lo + lo*hi on v_mfma_f32_32x32x16_{bf16,f16} with FP32
+//     accumulation, which keeps ~16 bits per element (the dropped lo*lo is ~2^-16 of
+//     a term).  One kernel does all of it and forms |psi| (reference, kernel.cl:844)
+//     or |psi|^2 from the Re / Im accumulators in registers: the FP32 field never
+//     reaches memory.
+//
+// MFMA operand layout (gfx950, 32x32x16, 64 lanes): lane l supplies row l % 32 of A
+// and column l % 32 of B, each with k = 8 (l / 32) .. 8 (l / 32) + 7 as 8 16-bit
+// values (16 bytes); accumulator register r of lane l is D[8 (r / 4) + 4 (l / 32) +
+// r % 4][l % 32].  The plan and the phase operand are written in exactly that order,
+// one 1 KB "fragment" per (32 rows or columns) x (16 emitters) x (cos or sin) x (hi or
+// lo), so every operand load is one contiguous 16-byte-per-lane global load.
 #include <algorithm>
 #include <cmath>
 #include <cstring>
@@ -23,16 +30,34 @@
 
 #pragma clang fp contract(off)
 
+namespace {
+
+constexpr double kPi = 3.14159265358979323846;
+
+// GEMM tiling: a block of 4 waves owns 256 rows x 128 columns (frame x mode); wave w
+// owns rows 128 (w & 1) .. +127 and columns 64 (w >> 1) .. +63, i.e. 4 x 2 tiles of
+// 32 x 32, each with a Re and an Im accumulator (256 accumulator registers per lane).
+constexpr uint32_t kRowsPerBlock = 256, kColsPerBlock = 128;
+constexpr uint32_t kWaveRowTiles = 4, kWaveColTiles = 2;
+constexpr uint32_t kEmittersPerStep = 16;  // the MFMA's k
+constexpr uint32_t kFragU4 = 64;            // uint4 per fragment (1 KB)
+
+}  // namespace
+
 struct iac_sim_plan {
     iac_ctx *ctx = nullptr;
     int mode = 0;
     uint32_t npix = 0, nsub = 0, rows = 0, nem = 0;
-    void *d_xhi = nullptr, *d_xlo = nullptr;  // [rows][2 nem] 16-bit; lo only for split modes
+    uint32_t rows_pad = 0;  // multiple of kRowsPerBlock
+    uint32_t nkc = 0;       // emitter steps of 16, an even count (emitters zero-padded)
+    void *d_x = nullptr;    // fragments [rows_pad / 32][nkc][cos, sin][hi (, lo)]
 };
 
 namespace {
 
-constexpr double kPi = 3.14159265358979323846;
+typedef __bf16 bf16x8_t __attribute__((ext_vector_type(8)));
+typedef _Float16 f16x8_t __attribute__((ext_vector_type(8)));
+typedef float f32x16_t __attribute__((ext_vector_type(16)));
 
 // per row (pixel p, subpixel s): k = 2 pi / lambda * pos / |pos| in double,
 // pos = p + v * v_step + u * u_step (Simulator.cpp:742-760, kernel.cl:823-832)
@@ -56,98 +81,300 @@ __global__ void k_rows_kernel(const float *__restrict__ pixmap, uint32_t npix, i
 }
 
 template <typename T>
-__device__ __forceinline__ void split16(double x, T &hi, T &lo)
+__device__ __forceinline__ uint16_t bits16(T v)
 {
-    hi = (T)(float)x;
-    lo = (T)(float)(x - (double)(float)hi);
+    return __builtin_bit_cast(uint16_t, v);
 }
 
-// X[row][e] = cos(k_row . r_e), X[row][E + e] = sin(k_row . r_e), split hi/lo
+// x = hi + lo, hi = T(x), lo = T(x - hi) (through float, as the values are |x| <= 1)
 template <typename T>
-__global__ void phase_matrix_kernel(const double *__restrict__ krow, uint64_t rows, const double *__restrict__ em,
-                                    uint32_t nem, T *__restrict__ xhi, T *__restrict__ xlo)
+__device__ __forceinline__ void split16(double x, uint16_t &hi, uint16_t &lo)
 {
-    const uint32_t e = blockIdx.x * blockDim.x + threadIdx.x;
-    if (e >= nem) return;
-    const double ex = em[3 * (size_t)e], ey = em[3 * (size_t)e + 1], ez = em[3 * (size_t)e + 2];
-    const size_t K = 2 * (size_t)nem;
-    for (uint64_t r = blockIdx.y; r < rows; r += gridDim.y) {
-        const double arg = krow[3 * r] * ex + krow[3 * r + 1] * ey + krow[3 * r + 2] * ez;
-        double sn, cs;
-        sincos(arg, &sn, &cs);
-        T h, l;
-        split16(cs, h, l);
-        xhi[r * K + e] = h;
-        if (xlo) xlo[r * K + e] = l;
-        split16(sn, h, l);
-        xhi[r * K + nem + e] = h;
-        if (xlo) xlo[r * K + nem + e] = l;
-    }
+    const T h = (T)(float)x;
+    hi = bits16(h);
+    lo = bits16((T)(float)(x - (double)(float)h));
 }
 
-// Y rows: n = c -> [cos phi | -sin phi], n = C + c -> [sin phi | cos phi], for the
-// C = frames x modes phase columns c; lo rows only for split modes (ylo non-null).
-template <typename T>
-__global__ void phase_vector_kernel(const double *__restrict__ phases, uint32_t ncols, uint32_t nem,
-                                    T *__restrict__ yhi, T *__restrict__ ylo)
+__device__ __forceinline__ uint4 pack8(const uint16_t *v)
 {
-    const uint32_t e = blockIdx.x * blockDim.x + threadIdx.x;
-    if (e >= nem) return;
-    const size_t K = 2 * (size_t)nem;
-    for (uint32_t c = blockIdx.y; c < ncols; c += gridDim.y) {
-        double sn, cs;
-        sincos(phases[(size_t)c * nem + e], &sn, &cs);
-        const size_t r0 = (size_t)c * K, r1 = (size_t)(ncols + c) * K;
-        T h, l;
-        split16(cs, h, l);
-        yhi[r0 + e] = h;
-        yhi[r1 + nem + e] = h;
-        if (ylo) {
-            ylo[r0 + e] = l;
-            ylo[r1 + nem + e] = l;
+    return make_uint4(v[0] | ((uint32_t)v[1] << 16), v[2] | ((uint32_t)v[3] << 16), v[4] | ((uint32_t)v[5] << 16),
+                      v[6] | ((uint32_t)v[7] << 16));
+}
+
+// One thread per (row tile t, emitter step kc, lane): the 8 entries of that lane's slot
+// in the four fragments (cos hi, cos lo, sin hi, sin lo; hi only when nh == 1) of
+// X[row][e] = exp(i k_row . r_e), row = 32 t + lane % 32, e = 16 kc + 8 (lane / 32) + j.
+// Padded rows and emitters are zero (they add nothing to any sum).
+template <typename T>
+__global__ void phase_matrix_kernel(const double *__restrict__ krow, uint32_t rows, uint32_t rows_pad,
+                                    const double *__restrict__ em, uint32_t nem, uint32_t nkc, int nh,
+                                    uint4 *__restrict__ x)
+{
+    const uint64_t slots = (uint64_t)(rows_pad / 32) * nkc * 64;
+    for (uint64_t i = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; i < slots;
+         i += (uint64_t)gridDim.x * blockDim.x) {
+        const uint32_t lane = (uint32_t)(i % 64);
+        const uint64_t tk = i / 64;  // t * nkc + kc
+        const uint32_t kc = (uint32_t)(tk % nkc), t = (uint32_t)(tk / nkc);
+        const uint32_t row = 32 * t + lane % 32;
+        uint16_t ch[8], cl[8], sh[8], sl[8];
+        double kx = 0, ky = 0, kz = 0;
+        if (row < rows) {
+            kx = krow[3 * (size_t)row];
+            ky = krow[3 * (size_t)row + 1];
+            kz = krow[3 * (size_t)row + 2];
         }
-        split16(-sn, h, l);
-        yhi[r0 + nem + e] = h;
-        if (ylo) ylo[r0 + nem + e] = l;
-        split16(sn, h, l);
-        yhi[r1 + e] = h;
-        if (ylo) ylo[r1 + e] = l;
+        for (int j = 0; j < 8; j++) {
+            const uint32_t e = kEmittersPerStep * kc + 8 * (lane / 32) + j;
+            double sn = 0.0, cs = 0.0;
+            if (row < rows && e < nem) {
+                const double arg = kx * em[3 * (size_t)e] + ky * em[3 * (size_t)e + 1] + kz * em[3 * (size_t)e + 2];
+                sincos(arg, &sn, &cs);
+            }
+            split16<T>(cs, ch[j], cl[j]);
+            split16<T>(sn, sh[j], sl[j]);
+        }
+        uint4 *f = x + tk * (2 * nh) * kFragU4 + lane;
+        f[0] = pack8(ch);
+        if (nh == 2) {
+            f[kFragU4] = pack8(cl);
+            f[2 * kFragU4] = pack8(sh);
+            f[3 * kFragU4] = pack8(sl);
+        } else {
+            f[kFragU4] = pack8(sh);
+        }
     }
 }
 
-// out[f][p] = sum over the modes m of the subpixel mean of |psi| (kind 0) or
-// |psi|^2 (kind 1) of column f * modes + m; Z is [rows][2C], C = frames x modes
-// (Re in column c, Im in C + c).  32 x 32 tile transposed through LDS.
-__global__ void epilogue_kernel(const float *__restrict__ Z, uint32_t rows0, uint32_t nrows, uint32_t nsub,
-                                uint32_t nframes, uint32_t modes, int kind, float *__restrict__ out, uint32_t npix)
+// The phase operand, one thread per (column tile u, emitter step kc, lane): fragments
+// (cos hi, cos lo, sin hi, sin lo) of B[e][col] = exp(i phi[col][e]) for col = 32 u +
+// lane % 32 (col = frame * modes + mode), e = 16 kc + 8 (lane / 32) + j.
+template <typename T>
+__global__ void phase_vector_kernel(const double *__restrict__ phases, uint32_t ncols, uint32_t ncols_pad,
+                                    uint32_t nem, uint32_t nkc, int nh, uint4 *__restrict__ y)
 {
-    __shared__ float tile[32][33];
-    const uint32_t f0 = blockIdx.x * 32, p0 = blockIdx.y * 32;  // pixel index relative to rows0 / nsub
-    const uint32_t tx = threadIdx.x, ty = threadIdx.y;          // 32 x 8
-    const uint32_t C = nframes * modes, C2 = 2 * C;
-    for (uint32_t j = ty; j < 32; j += 8) {
-        const uint32_t pl = p0 + j, f = f0 + tx;
-        float acc = 0.0f;
-        if (pl * nsub < nrows && f < nframes) {
-            for (uint32_t m = 0; m < modes; m++) {
-                const uint32_t c = f * modes + m;
-                float t = 0.0f;
-                for (uint32_t s = 0; s < nsub; s++) {
-                    const size_t row = (size_t)pl * nsub + s;
-                    const float re = Z[row * C2 + c], im = Z[row * C2 + C + c];
-                    const float a2 = re * re + im * im;
-                    t += kind == 0 ? sqrtf(a2) : a2;
+    const uint64_t slots = (uint64_t)(ncols_pad / 32) * nkc * 64;
+    for (uint64_t i = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; i < slots;
+         i += (uint64_t)gridDim.x * blockDim.x) {
+        const uint32_t lane = (uint32_t)(i % 64);
+        const uint64_t uk = i / 64;
+        const uint32_t kc = (uint32_t)(uk % nkc), u = (uint32_t)(uk / nkc);
+        const uint32_t col = 32 * u + lane % 32;
+        uint16_t ch[8], cl[8], sh[8], sl[8];
+        for (int j = 0; j < 8; j++) {
+            const uint32_t e = kEmittersPerStep * kc + 8 * (lane / 32) + j;
+            double sn = 0.0, cs = 0.0;
+            if (col < ncols && e < nem) sincos(phases[(size_t)col * nem + e], &sn, &cs);
+            split16<T>(cs, ch[j], cl[j]);
+            split16<T>(sn, sh[j], sl[j]);
+        }
+        uint4 *f = y + uk * (2 * nh) * kFragU4 + lane;
+        f[0] = pack8(ch);
+        if (nh == 2) {
+            f[kFragU4] = pack8(cl);
+            f[2 * kFragU4] = pack8(sh);
+            f[3 * kFragU4] = pack8(sl);
+        } else {
+            f[kFragU4] = pack8(sh);
+        }
+    }
+}
+
+template <bool kBf16>
+__device__ __forceinline__ f32x16_t mfma(const uint4 &a, const uint4 &b, f32x16_t c)
+{
+    if constexpr (kBf16)
+        return __builtin_amdgcn_mfma_f32_32x32x16_bf16(__builtin_bit_cast(bf16x8_t, a),
+                                                      __builtin_bit_cast(bf16x8_t, b), c, 0, 0, 0);
+    else
+        return __builtin_amdgcn_mfma_f32_32x32x16_f16(__builtin_bit_cast(f16x8_t, a), __builtin_bit_cast(f16x8_t, b),
+                                                     c, 0, 0, 0);
+}
+
+__device__ __forceinline__ uint4 negate16(const uint4 &v)
+{
+    constexpr uint32_t s = 0x80008000u;  // sign bit of both 16-bit halves (bf16 and fp16 alike)
+    return make_uint4(v.x ^ s, v.y ^ s, v.z ^ s, v.w ^ s);
+}
+
+// One emitter step's operands of a wave: A fragments per row tile [cos, sin][hi, lo],
+// B fragments per column tile [cos, sin][hi, lo].
+template <int NH>
+struct StepFrags {
+    uint4 a[kWaveRowTiles][2][NH];
+    uint4 b[kWaveColTiles][2][NH];
+};
+
+template <int NH>
+__device__ __forceinline__ void load_step(StepFrags<NH> &s, const uint4 *__restrict__ xa, const uint4 *__restrict__ yb,
+                                          size_t row_tile_stride, size_t col_tile_stride, uint32_t kc)
+{
+#pragma unroll
+    for (uint32_t i = 0; i < kWaveRowTiles; i++) {
+        const uint4 *p = xa + i * row_tile_stride + (size_t)kc * (2 * NH * kFragU4);
+#pragma unroll
+        for (int q = 0; q < 2; q++)
+#pragma unroll
+            for (int h = 0; h < NH; h++) s.a[i][q][h] = p[(q * NH + h) * kFragU4];
+    }
+#pragma unroll
+    for (uint32_t j = 0; j < kWaveColTiles; j++) {
+        const uint4 *p = yb + j * col_tile_stride + (size_t)kc * (2 * NH * kFragU4);
+#pragma unroll
+        for (int q = 0; q < 2; q++)
+#pragma unroll
+            for (int h = 0; h < NH; h++) s.b[j][q][h] = p[(q * NH + h) * kFragU4];
+    }
+}
+
+// Re += C c - S s, Im += C s + S c for every (row tile, column tile) of the wave; with
+// NH == 2 each product is hi hi + hi lo + lo hi.  Terms outermost, tiles innermost:
+// consecutive MFMAs write different accumulators.
+template <bool kBf16, int NH>
+__device__ __forceinline__ void step_mfma(const StepFrags<NH> &s, f32x16_t (&re)[kWaveRowTiles][kWaveColTiles],
+                                          f32x16_t (&im)[kWaveRowTiles][kWaveColTiles])
+{
+    constexpr int nterms = NH == 2 ? 3 : 1;
+    constexpr int ah[3] = {0, 0, 1}, bh[3] = {0, 1, 0};
+    uint4 ns[kWaveColTiles][NH];
+#pragma unroll
+    for (uint32_t j = 0; j < kWaveColTiles; j++)
+#pragma unroll
+        for (int h = 0; h < NH; h++) ns[j][h] = negate16(s.b[j][1][h]);
+#pragma unroll
+    for (int t = 0; t < nterms; t++) {
+#pragma unroll
+        for (uint32_t i = 0; i < kWaveRowTiles; i++)
+#pragma unroll
+            for (uint32_t j = 0; j < kWaveColTiles; j++) re[i][j] = mfma<kBf16>(s.a[i][0][ah[t]], s.b[j][0][bh[t]], re[i][j]);
+#pragma unroll
+        for (uint32_t i = 0; i < kWaveRowTiles; i++)
+#pragma unroll
+            for (uint32_t j = 0; j < kWaveColTiles; j++) im[i][j] = mfma<kBf16>(s.a[i][0][ah[t]], s.b[j][1][bh[t]], im[i][j]);
+#pragma unroll
+        for (uint32_t i = 0; i < kWaveRowTiles; i++)
+#pragma unroll
+            for (uint32_t j = 0; j < kWaveColTiles; j++) re[i][j] = mfma<kBf16>(s.a[i][1][ah[t]], ns[j][bh[t]], re[i][j]);
+#pragma unroll
+        for (uint32_t i = 0; i < kWaveRowTiles; i++)
+#pragma unroll
+            for (uint32_t j = 0; j < kWaveColTiles; j++) im[i][j] = mfma<kBf16>(s.a[i][1][ah[t]], s.b[j][0][bh[t]], im[i][j]);
+    }
+}
+
+// Block index -> (row block, column block).  Blocks go to the 8 XCDs round robin; the
+// remap hands each XCD a contiguous range of the (row block major) tile order, so the
+// column blocks of one row block run together on one XCD and read its rows of the plan
+// from that XCD's L2 once.
+__device__ __forceinline__ void block_tile(uint32_t nblocks, uint32_t ncb, uint32_t &rb, uint32_t &cb)
+{
+    const uint32_t b = blockIdx.x, xcd = b % 8, local = b / 8;
+    const uint32_t q = nblocks / 8, r = nblocks % 8;
+    const uint32_t lin = xcd < r ? xcd * (q + 1) + local : r * (q + 1) + (xcd - r) * q + local;
+    rb = lin / ncb;
+    cb = lin % ncb;
+}
+
+// The fused simulator GEMM.  Grid: nrb x ncb blocks of 256 threads (one wave per SIMD).
+// kDirect (one subpixel, one mode): out[f][p] = |psi| or |psi|^2 straight from the
+// accumulators (float4 stores when vec4: npix % 4 == 0 and a 16-byte aligned out);
+// otherwise the magnitude of every (row, column) goes to
+// w[col][row - row0] for the subpixel / mode reduction (w_rows rows per column).
+template <bool kBf16, int NH, bool kDirect>
+__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(1, 1)))
+    sim_gemm_kernel(const uint4 *__restrict__ x, const uint4 *__restrict__ y, uint32_t nkc, uint32_t rb0,
+                    uint32_t nrb, uint32_t ncb, uint32_t rows, uint32_t ncols, int kind, float *__restrict__ out,
+                    uint32_t npix, int vec4, float *__restrict__ w, uint32_t w_rows)
+{
+    uint32_t rb, cb;
+    block_tile(nrb * ncb, ncb, rb, cb);
+    rb += rb0;
+    const uint32_t wave = threadIdx.x / 64, lane = threadIdx.x % 64;
+    const uint32_t t0 = rb * (kRowsPerBlock / 32) + (wave & 1) * kWaveRowTiles;
+    const uint32_t u0 = cb * (kColsPerBlock / 32) + (wave >> 1) * kWaveColTiles;
+    const size_t tile_stride = (size_t)nkc * 2 * NH * kFragU4;
+    const uint4 *xa = x + (size_t)t0 * tile_stride + lane;
+    const uint4 *yb = y + (size_t)u0 * tile_stride + lane;
+
+    f32x16_t re[kWaveRowTiles][kWaveColTiles], im[kWaveRowTiles][kWaveColTiles];
+#pragma unroll
+    for (uint32_t i = 0; i < kWaveRowTiles; i++)
+#pragma unroll
+        for (uint32_t j = 0; j < kWaveColTiles; j++) {
+            re[i][j] = f32x16_t{};
+            im[i][j] = f32x16_t{};
+        }
+
+    // two steps in flight: the next step's fragments load while this one's MFMAs run.
+    // nkc is even; the last iteration reloads its own step instead of reading past the
+    // end, so the loop body has no branch (no copies of the accumulators).
+    StepFrags<NH> s0, s1;
+    load_step<NH>(s0, xa, yb, tile_stride, tile_stride, 0);
+    // The scheduling barriers keep each step's loads ahead of the previous step's MFMAs
+    // (left alone, the scheduler sinks the loads to their first use).
+    for (uint32_t kc = 0; kc < nkc; kc += 2) {
+        load_step<NH>(s1, xa, yb, tile_stride, tile_stride, kc + 1);
+        __builtin_amdgcn_sched_barrier(0);
+        step_mfma<kBf16, NH>(s0, re, im);
+        __builtin_amdgcn_sched_barrier(0);
+        load_step<NH>(s0, xa, yb, tile_stride, tile_stride, kc + 2 < nkc ? kc + 2 : kc);
+        __builtin_amdgcn_sched_barrier(0);
+        step_mfma<kBf16, NH>(s1, re, im);
+        __builtin_amdgcn_sched_barrier(0);
+    }
+
+    // epilogue: lane holds column 32 u + lane % 32 and rows 32 t + 8 g + 4 (lane / 32) + r
+#pragma unroll
+    for (uint32_t i = 0; i < kWaveRowTiles; i++) {
+#pragma unroll
+        for (uint32_t j = 0; j < kWaveColTiles; j++) {
+            const uint32_t col = 32 * (u0 + j) + lane % 32;
+            if (col >= ncols) continue;
+#pragma unroll
+            for (uint32_t g = 0; g < 4; g++) {
+                const uint32_t row = 32 * (t0 + i) + 8 * g + 4 * (lane / 32);
+                float v[4];
+#pragma unroll
+                for (uint32_t r = 0; r < 4; r++) {
+                    const float a = re[i][j][4 * g + r], b = im[i][j][4 * g + r];
+                    const float a2 = a * a + b * b;
+                    v[r] = kind == 0 ? sqrtf(a2) : a2;
                 }
-                acc += t / (float)nsub;
+                if (kDirect) {
+                    float *o = out + (size_t)col * npix + row;
+                    if (vec4 && row + 3 < rows) {
+                        *reinterpret_cast<float4 *>(o) = make_float4(v[0], v[1], v[2], v[3]);
+                    } else {
+#pragma unroll
+                        for (uint32_t r = 0; r < 4; r++)
+                            if (row + r < rows) o[r] = v[r];
+                    }
+                } else {
+                    float *o = w + (size_t)col * w_rows + (row - rb0 * kRowsPerBlock);
+                    *reinterpret_cast<float4 *>(o) = make_float4(v[0], v[1], v[2], v[3]);
+                }
             }
         }
-        tile[j][tx] = acc;
     }
-    __syncthreads();
-    for (uint32_t j = ty; j < 32; j += 8) {
-        const uint32_t f = f0 + j, pl = p0 + tx;
-        const uint32_t p = rows0 / nsub + pl;
-        if (f < nframes && pl * nsub < nrows && p < npix) out[(size_t)f * npix + p] = tile[tx][j];
+}
+
+// out[f][p] = sum over the modes m of (sum over the subpixels s of w[f modes + m][row]) /
+// nsub, rows p nsub + s relative to the chunk's first pixel p0; sums in order.
+__global__ void subpixel_mode_reduce_kernel(const float *__restrict__ w, uint32_t w_rows, uint32_t p0, uint32_t np,
+                                            uint32_t nsub, uint32_t nframes, uint32_t modes,
+                                            float *__restrict__ out, uint32_t npix)
+{
+    const uint32_t pl = blockIdx.x * blockDim.x + threadIdx.x;
+    if (pl >= np) return;
+    for (uint32_t f = blockIdx.y; f < nframes; f += gridDim.y) {
+        float acc = 0.0f;
+        for (uint32_t m = 0; m < modes; m++) {
+            const float *wc = w + (size_t)(f * modes + m) * w_rows + (size_t)pl * nsub;
+            float t = 0.0f;
+            for (uint32_t s = 0; s < nsub; s++) t += wc[s];
+            acc += t / (float)nsub;
+        }
+        out[(size_t)f * npix + p0 + pl] = acc;
     }
 }
 
@@ -282,30 +509,49 @@ __global__ void __launch_bounds__(256)
 bool split_mode(int mode) { return mode == IAC_SIM_BF16X3 || mode == IAC_SIM_FP16X3; }
 bool bf16_mode(int mode) { return mode == IAC_SIM_BF16X3 || mode == IAC_SIM_BF16; }
 
-int rocblas_handle_for(iac_ctx *ctx, rocblas_handle *h)
+unsigned grid_for(uint64_t threads)
 {
-    if (!ctx->rocblas) {
-        rocblas_handle nh;
-        if (rocblas_create_handle(&nh) != rocblas_status_success)
-            return iac::fail(IAC_ERR_HIP, "rocblas_create_handle failed");
-        ctx->rocblas = nh;
+    return (unsigned)std::min<uint64_t>((threads + 255) / 256, 1u << 20);
+}
+
+template <bool kBf16, int NH>
+void launch_gemm(hipStream_t st, bool direct, const uint4 *x, const uint4 *y, uint32_t nkc, uint32_t rb0, uint32_t nrb,
+                 uint32_t ncb, uint32_t rows, uint32_t ncols, int kind, float *out, uint32_t npix, bool vec4, float *w,
+                 uint32_t w_rows)
+{
+    const dim3 grid(nrb * ncb), block(256);
+    if (direct)
+        hipLaunchKernelGGL((sim_gemm_kernel<kBf16, NH, true>), grid, block, 0, st, x, y, nkc, rb0, nrb, ncb, rows,
+                           ncols, kind, out, npix, vec4 ? 1 : 0, w, w_rows);
+    else
+        hipLaunchKernelGGL((sim_gemm_kernel<kBf16, NH, false>), grid, block, 0, st, x, y, nkc, rb0, nrb, ncb, rows,
+                           ncols, kind, out, npix, 0, w, w_rows);
+}
+
+void launch_gemm_mode(int mode, hipStream_t st, bool direct, const uint4 *x, const uint4 *y, uint32_t nkc,
+                      uint32_t rb0, uint32_t nrb, uint32_t ncb, uint32_t rows, uint32_t ncols, int kind, float *out,
+                      uint32_t npix, bool vec4, float *w, uint32_t w_rows)
+{
+    switch (mode) {
+    case IAC_SIM_BF16X3:
+        launch_gemm<true, 2>(st, direct, x, y, nkc, rb0, nrb, ncb, rows, ncols, kind, out, npix, vec4, w, w_rows);
+        break;
+    case IAC_SIM_FP16X3:
+        launch_gemm<false, 2>(st, direct, x, y, nkc, rb0, nrb, ncb, rows, ncols, kind, out, npix, vec4, w, w_rows);
+        break;
+    case IAC_SIM_BF16:
+        launch_gemm<true, 1>(st, direct, x, y, nkc, rb0, nrb, ncb, rows, ncols, kind, out, npix, vec4, w, w_rows);
+        break;
+    default:
+        launch_gemm<false, 1>(st, direct, x, y, nkc, rb0, nrb, ncb, rows, ncols, kind, out, npix, vec4, w, w_rows);
+        break;
     }
-    *h = (rocblas_handle)ctx->rocblas;
-    if (rocblas_set_stream(*h, ctx->stream) != rocblas_status_success)
-        return iac::fail(IAC_ERR_HIP, "rocblas_set_stream failed");
-    return IAC_OK;
 }
 
 }  // namespace
 
 namespace iac {
-void free_sim_resources(iac_ctx *ctx)
-{
-    if (ctx->rocblas) {
-        (void)rocblas_destroy_handle((rocblas_handle)ctx->rocblas);
-        ctx->rocblas = nullptr;
-    }
-}
+void free_sim_resources(iac_ctx *) {}
 }  // namespace iac
 
 extern "C" {
@@ -322,27 +568,26 @@ int iac_sim_plan_create(iac_ctx *ctx, const float *d_pixmap, uint32_t npix, int 
     if (mode < IAC_SIM_BF16X3 || mode > IAC_SIM_FP16)
         return iac::fail(IAC_ERR_INVALID_ARG, "plan modes are the GEMM modes (IAC_SIM_BF16X3..IAC_SIM_FP16)");
     *out = nullptr;
+    const uint32_t nsub = (uint32_t)((2 * subsampling + 1) * (2 * subsampling + 1));
+    const uint64_t rows = (uint64_t)npix * nsub;
+    const uint64_t rows_pad = (rows + kRowsPerBlock - 1) / kRowsPerBlock * kRowsPerBlock;
+    if (rows_pad >= (1ull << 31)) return iac::fail(IAC_ERR_LIMIT, "pixels x subpixels must stay below 2^31");
     auto *pl = new iac_sim_plan();
     pl->ctx = ctx;
     pl->mode = mode;
     pl->npix = npix;
-    pl->nsub = (uint32_t)((2 * subsampling + 1) * (2 * subsampling + 1));
-    const uint64_t rows = (uint64_t)npix * pl->nsub;
-    if (rows >= (1ull << 31)) {
-        delete pl;
-        return iac::fail(IAC_ERR_LIMIT, "pixels x subpixels must stay below 2^31");
-    }
+    pl->nsub = nsub;
     pl->rows = (uint32_t)rows;
+    pl->rows_pad = (uint32_t)rows_pad;
     pl->nem = nem;
-    const size_t K = 2 * (size_t)nem;
-    const size_t bytes = rows * K * 2;
+    pl->nkc = (nem + 2 * kEmittersPerStep - 1) / (2 * kEmittersPerStep) * 2;  // even: the GEMM loop takes steps in pairs
+    const int nh = split_mode(mode) ? 2 : 1;
+    const size_t bytes = rows_pad * (size_t)pl->nkc * kEmittersPerStep * 2 * nh * 2;
     double *krow = nullptr;
-    hipError_t e = hipMalloc(&pl->d_xhi, bytes);
-    if (e == hipSuccess && split_mode(mode)) e = hipMalloc(&pl->d_xlo, bytes);
+    hipError_t e = hipMalloc(&pl->d_x, bytes);
     if (e == hipSuccess) e = hipMalloc(&krow, rows * 3 * sizeof(double));
     if (e != hipSuccess) {
-        if (pl->d_xhi) (void)hipFree(pl->d_xhi);
-        if (pl->d_xlo) (void)hipFree(pl->d_xlo);
+        if (pl->d_x) (void)hipFree(pl->d_x);
         if (krow) (void)hipFree(krow);
         delete pl;
         return iac::hip_fail(e, "iac_sim_plan_create allocation", __FILE__, __LINE__);
@@ -351,19 +596,18 @@ int iac_sim_plan_create(iac_ctx *ctx, const float *d_pixmap, uint32_t npix, int 
     hipLaunchKernelGGL(k_rows_kernel, dim3((unsigned)std::min<uint64_t>(65535, (rows + 255) / 256)), dim3(256), 0,
                        ctx->stream, d_pixmap, npix, subsampling, make_double3(u_step[0], u_step[1], u_step[2]),
                        make_double3(v_step[0], v_step[1], v_step[2]), kk, krow);
-    const dim3 grid((nem + 255) / 256, (unsigned)std::min<uint64_t>(65535, rows));
+    const uint64_t slots = (rows_pad / 32) * (uint64_t)pl->nkc * 64;
     if (bf16_mode(mode))
-        hipLaunchKernelGGL(phase_matrix_kernel<__bf16>, grid, dim3(256), 0, ctx->stream, krow, rows, d_emitters, nem,
-                           (__bf16 *)pl->d_xhi, (__bf16 *)pl->d_xlo);
+        hipLaunchKernelGGL(phase_matrix_kernel<__bf16>, dim3(grid_for(slots)), dim3(256), 0, ctx->stream, krow,
+                           pl->rows, pl->rows_pad, d_emitters, nem, pl->nkc, nh, static_cast<uint4 *>(pl->d_x));
     else
-        hipLaunchKernelGGL(phase_matrix_kernel<_Float16>, grid, dim3(256), 0, ctx->stream, krow, rows, d_emitters,
-                           nem, (_Float16 *)pl->d_xhi, (_Float16 *)pl->d_xlo);
+        hipLaunchKernelGGL(phase_matrix_kernel<_Float16>, dim3(grid_for(slots)), dim3(256), 0, ctx->stream, krow,
+                           pl->rows, pl->rows_pad, d_emitters, nem, pl->nkc, nh, static_cast<uint4 *>(pl->d_x));
     e = hipGetLastError();
     if (e == hipSuccess) e = hipStreamSynchronize(ctx->stream);
     (void)hipFree(krow);
     if (e != hipSuccess) {
-        (void)hipFree(pl->d_xhi);
-        (void)hipFree(pl->d_xlo);
+        (void)hipFree(pl->d_x);
         delete pl;
         return iac::hip_fail(e, "iac_sim_plan_create", __FILE__, __LINE__);
     }
@@ -376,8 +620,7 @@ int iac_sim_plan_destroy(iac_sim_plan *plan)
     if (!plan) return IAC_OK;
     (void)hipSetDevice(plan->ctx->device);
     (void)hipStreamSynchronize(plan->ctx->stream);
-    if (plan->d_xhi) (void)hipFree(plan->d_xhi);
-    if (plan->d_xlo) (void)hipFree(plan->d_xlo);
+    if (plan->d_x) (void)hipFree(plan->d_x);
     delete plan;
     return IAC_OK;
 }
@@ -393,57 +636,63 @@ int iac_sim_run(iac_ctx *ctx, iac_sim_plan *plan, const double *d_phases, uint32
     if (nframes == 0) return IAC_OK;
     const uint64_t cols = (uint64_t)nframes * modes;
     if (cols > (1u << 24)) return iac::fail(IAC_ERR_LIMIT, "at most 2^24 frames x modes per call");
-    const uint32_t E = plan->nem;
-    const uint64_t N = 2 * cols;
-    const size_t K = 2 * (size_t)E;
-    const bool split = split_mode(plan->mode), bf = bf16_mode(plan->mode);
-    // Y hi and lo (scratch slot 0), a Z chunk (slot 1)
-    void *yb = nullptr, *zb = nullptr;
-    const size_t ybytes = (size_t)N * K * 2;
-    IAC_TRY(iac::scratch(ctx, 0, (split ? 2 : 1) * ybytes, &yb));
-    void *yhi = yb, *ylo = split ? static_cast<char *>(yb) + ybytes : nullptr;
-    // rows per GEMM chunk: a Z chunk of at most 1 GiB holding whole pixels
-    uint64_t chunk = ctx->sim_chunk_rows ? ctx->sim_chunk_rows : (1ull << 30) / (4ull * N);
-    chunk = std::max<uint64_t>(plan->nsub, chunk);
-    chunk = std::min<uint64_t>(chunk, plan->rows);
-    chunk = std::min<uint64_t>(chunk, 65535ull * 32 * plan->nsub);  // epilogue grid.y bound
-    chunk -= chunk % plan->nsub;
-    IAC_TRY(iac::scratch(ctx, 1, chunk * N * sizeof(float), &zb));
-    rocblas_handle h;
-    IAC_TRY(rocblas_handle_for(ctx, &h));
-    const dim3 yg((E + 255) / 256, (unsigned)std::min<uint64_t>(cols, 65535));
+    const uint32_t ncols = (uint32_t)cols;
+    const uint32_t ncols_pad = (ncols + kColsPerBlock - 1) / kColsPerBlock * kColsPerBlock;
+    const uint32_t ncb = ncols_pad / kColsPerBlock;
+    const int nh = split_mode(plan->mode) ? 2 : 1;
+    const bool bf = bf16_mode(plan->mode);
+    // the phase operand (scratch slot 0)
+    void *yb = nullptr;
+    const size_t ybytes = (size_t)ncols_pad * plan->nkc * kEmittersPerStep * 2 * nh * 2;
+    IAC_TRY(iac::scratch(ctx, 0, ybytes, &yb));
+    const uint64_t yslots = (uint64_t)(ncols_pad / 32) * plan->nkc * 64;
     if (bf)
-        hipLaunchKernelGGL(phase_vector_kernel<__bf16>, yg, dim3(256), 0, ctx->stream, d_phases, (uint32_t)cols, E,
-                           (__bf16 *)yhi, (__bf16 *)ylo);
+        hipLaunchKernelGGL(phase_vector_kernel<__bf16>, dim3(grid_for(yslots)), dim3(256), 0, ctx->stream, d_phases,
+                           ncols, ncols_pad, plan->nem, plan->nkc, nh, static_cast<uint4 *>(yb));
     else
-        hipLaunchKernelGGL(phase_vector_kernel<_Float16>, yg, dim3(256), 0, ctx->stream, d_phases, (uint32_t)cols, E,
-                           (_Float16 *)yhi, (_Float16 *)ylo);
+        hipLaunchKernelGGL(phase_vector_kernel<_Float16>, dim3(grid_for(yslots)), dim3(256), 0, ctx->stream,
+                           d_phases, ncols, ncols_pad, plan->nem, plan->nkc, nh, static_cast<uint4 *>(yb));
     IAC_HIP(hipGetLastError());
-    const rocblas_datatype dt = bf ? rocblas_datatype_bf16_r : rocblas_datatype_f16_r;
-    const float one = 1.0f, zero = 0.0f;
-    IAC_TRY(iac::timer_begin(ctx));
-    for (uint64_t r0 = 0; r0 < plan->rows; r0 += chunk) {
-        const uint64_t nr = std::min<uint64_t>(chunk, plan->rows - r0);
-        const char *xh = static_cast<const char *>(plan->d_xhi) + r0 * K * 2;
-        const char *xl = split ? static_cast<const char *>(plan->d_xlo) + r0 * K * 2 : nullptr;
-        // column major: Z^T (N x nr) = Y^T (N x K) * X^T (K x nr); the lo x lo term is dropped
-        struct Term {
-            const void *y, *x;
-            const float *beta;
-        } terms[3] = {{yhi, xh, &zero}, {ylo, xh, &one}, {yhi, xl, &one}};
-        const int nterms = split ? 3 : 1;
-        for (int t = 0; t < nterms; t++) {
-            rocblas_status st = rocblas_gemm_ex(
-                h, rocblas_operation_transpose, rocblas_operation_none, (rocblas_int)N, (rocblas_int)nr,
-                (rocblas_int)K, &one, terms[t].y, dt, (rocblas_int)K, terms[t].x, dt, (rocblas_int)K, terms[t].beta,
-                zb, rocblas_datatype_f32_r, (rocblas_int)N, zb, rocblas_datatype_f32_r, (rocblas_int)N,
-                rocblas_datatype_f32_r, rocblas_gemm_algo_standard, 0, 0);
-            if (st != rocblas_status_success)
-                return iac::fail(IAC_ERR_HIP, std::string("rocblas_gemm_ex: ") + rocblas_status_to_string(st));
+    const uint4 *x = static_cast<const uint4 *>(plan->d_x), *y = static_cast<const uint4 *>(yb);
+    const uint32_t nrb_all = plan->rows_pad / kRowsPerBlock;
+    const bool direct = plan->nsub == 1 && modes == 1;
+    if (direct) {
+        // one launch; the magnitudes go straight to d_out (float4 stores need npix % 4 == 0
+        // and a 16-byte aligned output)
+        const bool vec4 = plan->npix % 4 == 0 && reinterpret_cast<uintptr_t>(d_out) % 16 == 0;
+        const uint32_t nrb_step = std::max<uint32_t>(1, (1u << 30) / ncb);  // blocks per launch < 2^31
+        IAC_TRY(iac::timer_begin(ctx));
+        for (uint32_t rb0 = 0; rb0 < nrb_all; rb0 += nrb_step) {
+            launch_gemm_mode(plan->mode, ctx->stream, true, x, y, plan->nkc, rb0, std::min(nrb_step, nrb_all - rb0),
+                             ncb, plan->rows, ncols, out_kind, d_out, plan->npix, vec4, nullptr, 0);
+            IAC_HIP(hipGetLastError());
         }
-        const dim3 eg((nframes + 31) / 32, (unsigned)((nr / plan->nsub + 31) / 32));
-        hipLaunchKernelGGL(epilogue_kernel, eg, dim3(32, 8), 0, ctx->stream, static_cast<const float *>(zb),
-                           (uint32_t)r0, (uint32_t)nr, plan->nsub, nframes, modes, out_kind, d_out, plan->npix);
+        IAC_TRY(iac::timer_end(ctx));
+        return IAC_OK;
+    }
+    // subpixels or modes: magnitudes per (row, column) into w (scratch slot 1), chunks of
+    // whole 256-pixel groups (a multiple of 256 rows), then the ordered reduction
+    uint64_t chunk_pix = ctx->sim_chunk_rows ? ctx->sim_chunk_rows / plan->nsub
+                                             : (1ull << 30) / (4ull * ncols_pad * plan->nsub);
+    chunk_pix = std::max<uint64_t>(kRowsPerBlock, chunk_pix / kRowsPerBlock * kRowsPerBlock);
+    const uint64_t chunk_rows_max = std::min<uint64_t>(chunk_pix * plan->nsub, plan->rows_pad);
+    void *wb = nullptr;
+    IAC_TRY(iac::scratch(ctx, 1, chunk_rows_max * ncols_pad * sizeof(float), &wb));
+    IAC_TRY(iac::timer_begin(ctx));
+    for (uint64_t p0 = 0; p0 < plan->npix; p0 += chunk_pix) {
+        const uint64_t np = std::min<uint64_t>(chunk_pix, plan->npix - p0);
+        const uint64_t r0 = p0 * plan->nsub;  // a multiple of 256
+        const uint64_t r1 = std::min<uint64_t>(plan->rows_pad, (p0 + np) * plan->nsub + kRowsPerBlock - 1) /
+                            kRowsPerBlock * kRowsPerBlock;
+        const uint32_t rb0 = (uint32_t)(r0 / kRowsPerBlock), nrb = (uint32_t)((r1 - r0 + kRowsPerBlock - 1) / kRowsPerBlock);
+        const uint32_t w_rows = nrb * kRowsPerBlock;
+        launch_gemm_mode(plan->mode, ctx->stream, false, x, y, plan->nkc, rb0, nrb, ncb, plan->rows, ncols, out_kind,
+                         nullptr, plan->npix, false, static_cast<float *>(wb), w_rows);
+        IAC_HIP(hipGetLastError());
+        hipLaunchKernelGGL(subpixel_mode_reduce_kernel, dim3((unsigned)((np + 255) / 256), std::min(nframes, 65535u)),
+                           dim3(256), 0,
+                           ctx->stream, static_cast<const float *>(wb), w_rows, (uint32_t)p0, (uint32_t)np, plan->nsub,
+                           nframes, modes, d_out, plan->npix);
         IAC_HIP(hipGetLastError());
     }
     IAC_TRY(iac::timer_end(ctx));

@@ -61,9 +61,8 @@ void AC1D::CalcAC()
 void AC1D::Calculate_CQ(Detector &Det, Settings &Options, Settings::Interpolation IterpolMode)
 {
     if (!Det.Intensity) throw std::runtime_error("AC1D::Calculate_CQ: averaged intensity not loaded");
-    const int dev = Options.GPU_ReserveDevice();
-    if (dev < 0) throw std::runtime_error("AC1D::Calculate_CQ: no free device");
-    iac_ctx *ctx = Options.Context(dev);
+    iac_host::Reserved res(Options, "AC1D::Calculate_CQ");
+    iac_ctx *ctx = res.ctx;
     const size_t P = Det.NumPixels();
     double mult = 0;
     check(iac_ac1d_cq_multiplicator(Det.Intensity, (uint32_t)P, &mult), "iac_ac1d_cq_multiplicator");
@@ -76,7 +75,6 @@ void AC1D::Calculate_CQ(Detector &Det, Settings &Options, Settings::Interpolatio
     check(iac_u64_to_f64(ctx, hist.as<uint64_t>(), Shape.Size, mult, cq.as<double>()), "iac_u64_to_f64");
     fresh(CQ, Shape.Size);
     cq.download(CQ, (size_t)Shape.Size * 8);
-    Options.GPU_FreeDevice(dev);
 }
 
 // IAC/AC1D.cpp:393-469 with the per-event work of Calculate_AC_UW_Mapped (:201-293):
@@ -88,9 +86,8 @@ void AC1D::Calculate_AC_UW_MR(Settings &Options, Detector &RefDet, Settings::Int
 {
     if (Options.HitEvents.empty() || UpperBoundary <= LowerBoundary || UpperBoundary > Options.HitEvents.size())
         throw std::runtime_error("AC1D::Calculate_AC_UW_MR: no entries in HitEvents");
-    const int dev = Options.GPU_ReserveDevice();
-    if (dev < 0) throw std::runtime_error("AC1D::Calculate_AC_UW_MR: no free device");
-    iac_ctx *ctx = Options.Context(dev);
+    iac_host::Reserved res(Options, "AC1D::Calculate_AC_UW_MR");
+    iac_ctx *ctx = res.ctx;
     const size_t P = RefDet.NumPixels();
     const uint32_t N = UpperBoundary - LowerBoundary;
     const uint32_t B = std::max<uint32_t>(1, std::min<uint32_t>(Options.BatchFrames, (uint32_t)((1ull << 31) / (P * 20))));
@@ -115,5 +112,4 @@ void AC1D::Calculate_AC_UW_MR(Settings &Options, Detector &RefDet, Settings::Int
     }
     fresh(AC_UW, Shape.Size);
     hist.download(AC_UW, (size_t)Shape.Size * 8);
-    Options.GPU_FreeDevice(dev);
 }

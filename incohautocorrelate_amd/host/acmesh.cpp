@@ -16,8 +16,40 @@ ACMesh::~ACMesh()
     delete[] CQMesh;
 }
 
+void ACMesh::drop_queue()
+{
+    if (queue_) {
+        iac_ac3d_queue *q = queue_;
+        queue_ = nullptr;
+        queue_kmap_ = nullptr;
+        queue_flags_ = -1;
+        check(iac_ac3d_queue_destroy(q), "iac_ac3d_queue_destroy");
+    }
+}
+
+iac_ac3d_queue *ACMesh::Queue(const float *d_kmap, uint32_t npix, int interp1, int interp2, bool double_map)
+{
+    const int flags = interp1 + 2 * interp2 + 4 * (double_map ? 1 : 0);
+    if (queue_ && (queue_kmap_ != d_kmap || queue_flags_ != flags)) drop_queue();
+    if (!queue_) {
+        iac_mesh_shape s = CShape();
+        check(iac_ac3d_queue_create(ctx, d_kmap, npix, &s, interp1, interp2, double_map ? 1 : 0, DeviceMesh(), 0, 0,
+                                    &queue_),
+              "iac_ac3d_queue_create in ACMesh::Queue()");
+        queue_kmap_ = d_kmap;
+        queue_flags_ = flags;
+    }
+    return queue_;
+}
+
+void ACMesh::Flush()
+{
+    if (queue_) check(iac_ac3d_queue_flush(queue_), "iac_ac3d_queue_flush in ACMesh::Flush()");
+}
+
 void ACMesh::reset_device()
 {
+    drop_queue();
     if (d_Mesh) iac_dev_free(ctx, d_Mesh);
     if (d_CQMesh) iac_dev_free(ctx, d_CQMesh);
     d_Mesh = nullptr;
@@ -112,6 +144,7 @@ double *ACMesh::DeviceCQMesh()
 
 void ACMesh::Download()
 {
+    Flush();
     const size_t n = NumVoxels();
     if (d_Mesh) {
         if (!Mesh) Mesh = new unsigned long[n];

@@ -277,9 +277,8 @@ void Detector::LoadAndAverageIntensity(std::vector<Settings::HitEvent> &Events, 
         throw std::runtime_error("Detector::LoadAndAverageIntensity: bad event bounds");
     if (!Checklist.PixelMask) GenerateFlatOnesPixelMask();
     Settings &O = opts("LoadAndAverageIntensity");
-    const int dev = O.GPU_ReserveDevice();
-    if (dev < 0) throw std::runtime_error("Detector::LoadAndAverageIntensity: no free device");
-    iac_ctx *ctx = O.Context(dev);
+    iac_host::Reserved res(O, "Detector::LoadAndAverageIntensity");
+    iac_ctx *ctx = res.ctx;
     const size_t P = NumPixels();
     const uint32_t N = (uint32_t)(UpperBound - LowerBound);
     const uint32_t B = std::max<uint32_t>(1, std::min<uint32_t>(O.BatchFrames, (uint32_t)((1ull << 30) / (P * 4))));
@@ -311,7 +310,6 @@ void Detector::LoadAndAverageIntensity(std::vector<Settings::HitEvent> &Events, 
     delete[] Intensity;
     Intensity = new float[P];
     avg.download(Intensity, P * 4);
-    O.GPU_FreeDevice(dev);
     Checklist.Intensity = true;
 }
 
@@ -328,9 +326,8 @@ void Detector::CreateSparseHitList(float Threshold, float PhotonSamplingStep, bo
 {
     if (!Intensity) throw std::runtime_error("Detector::CreateSparseHitList: no intensity loaded");
     Settings &O = opts("CreateSparseHitList");
-    const int dev = O.GPU_ReserveDevice();
-    if (dev < 0) throw std::runtime_error("Detector::CreateSparseHitList: no free device");
-    iac_ctx *ctx = O.Context(dev);
+    iac_host::Reserved res(O, "Detector::CreateSparseHitList");
+    iac_ctx *ctx = res.ctx;
     const size_t P = NumPixels();
     iac_host::DevBuf fr(ctx, P * 4), rec(ctx, P * 16), off(ctx, 16);
     fr.upload(Intensity, P * 4);
@@ -340,7 +337,6 @@ void Detector::CreateSparseHitList(float Threshold, float PhotonSamplingStep, bo
           "iac_sparse_hit_list in Detector::CreateSparseHitList()");
     SparseHitList.resize(n);
     rec.download(SparseHitList.data(), n * 16);
-    O.GPU_FreeDevice(dev);
     Checklist.SparseHitList = true;
 }
 
@@ -358,8 +354,10 @@ float Detector::CalculateMeanIntensity(bool FromSparse)
     return (float)IntInt;
 }
 
-// IAC/Detector.cpp:1010-1181: the current SparseHitList binned into the resident
-// device mesh with the event's rotation.
+// IAC/Detector.cpp:1010-1181: the current SparseHitList, with the event's rotation,
+// appended to the mesh's frame queue (pinned staging, no allocation, no device
+// synchronisation); the LDS engine bins the queued events in one pass when the queue
+// fills and on ACMesh::Flush / Download.
 int Detector::AutoCorrelateSparseList(ACMesh &BigMesh, AutoCorrFlags FlagsFirstMap, AutoCorrFlags FlagsSecondMap,
                                       bool DoubleMapping, Settings &Options_, int CpuGpu)
 {
@@ -370,24 +368,18 @@ int Detector::AutoCorrelateSparseList(ACMesh &BigMesh, AutoCorrFlags FlagsFirstM
                      "the device path gives the same mesh (CpuGpu = -1 or 1)\n";
         exit(EXIT_FAILURE);
     }
-    const int dev = Options_.GPU_ReserveDevice();
-    if (dev < 0) throw std::runtime_error("Detector::AutoCorrelateSparseList: no free device");
-    iac_ctx *ctx = Options_.Context(dev);
+    iac_host::Reserved res(Options_, "Detector::AutoCorrelateSparseList");
+    iac_ctx *ctx = res.ctx;
     if (BigMesh.ctx && BigMesh.ctx != ctx) throw std::runtime_error("AutoCorrelateSparseList: mesh lives on another device");
     BigMesh.ctx = ctx;
     const float identity[9] = {1, 0, 0, 0, 1, 0, 0, 0, 1};
     const float *rot = DetectorEvent ? DetectorEvent->RotMatrix : identity;
     const size_t n = SparseHitList.size();
-    if (n) {
-        iac_host::DevBuf hl(ctx, n * 16);
-        hl.upload(SparseHitList.data(), n * 16);
-        iac_mesh_shape s = BigMesh.CShape();
-        check(iac_ac3d_sparse(ctx, hl.as<float>(), (uint32_t)n, rot, &s, FlagsFirstMap.InterpolationMode,
-                              FlagsSecondMap.InterpolationMode, DoubleMapping ? 1 : 0, BigMesh.DeviceMesh()),
-              "iac_ac3d_sparse in Detector::AutoCorrelateSparseList()");
-        check(iac_ctx_sync(ctx), "iac_ctx_sync");
-    }
-    Options_.GPU_FreeDevice(dev);
+    if (n >= (1ull << 32)) throw std::runtime_error("Detector::AutoCorrelateSparseList: hit list too long");
+    iac_ac3d_queue *q = BigMesh.Queue(DeviceKMap(ctx), (uint32_t)NumPixels(), FlagsFirstMap.InterpolationMode,
+                                      FlagsSecondMap.InterpolationMode, DoubleMapping);
+    check(iac_ac3d_queue_push(q, n ? SparseHitList.data()->data() : nullptr, (uint32_t)n, rot),
+          "iac_ac3d_queue_push in Detector::AutoCorrelateSparseList()");
     return 1;
 }
 
@@ -396,9 +388,8 @@ void Detector::AutoCorrelate_CofQ_SmallMesh(ACMesh &SmallMesh, AutoCorrFlags Fla
 {
     if (!Checklist.Intensity || !Intensity) throw std::runtime_error("AutoCorrelate_CofQ_SmallMesh: no intensity");
     if (!SmallMesh.Checklist.CofQSmallMesh) throw std::runtime_error("AutoCorrelate_CofQ_SmallMesh: no small mesh");
-    const int dev = Options_.GPU_ReserveDevice();
-    if (dev < 0) throw std::runtime_error("AutoCorrelate_CofQ_SmallMesh: no free device");
-    iac_ctx *ctx = Options_.Context(dev);
+    iac_host::Reserved res(Options_, "AutoCorrelate_CofQ_SmallMesh");
+    iac_ctx *ctx = res.ctx;
     SmallMesh.ctx = ctx;
     const size_t P = NumPixels();
     double mult = 0;
@@ -411,7 +402,6 @@ void Detector::AutoCorrelate_CofQ_SmallMesh(ACMesh &SmallMesh, AutoCorrFlags Fla
           "iac_cq_small in Detector::AutoCorrelate_CofQ_SmallMesh()");
     check(iac_u64_to_f64(ctx, m, SmallMesh.NumVoxels(), mult, SmallMesh.DeviceCQMesh()), "iac_u64_to_f64");
     check(iac_ctx_sync(ctx), "iac_ctx_sync");
-    Options_.GPU_FreeDevice(dev);
 }
 
 void Detector::Merge_smallCofQ(ACMesh &BigMesh, ACMesh &SmallMesh, std::vector<Settings::HitEvent> &Events,
@@ -433,9 +423,8 @@ void Detector::Merge_smallCofQ(ACMesh &BigMesh, ACMesh &SmallMesh, std::vector<S
         throw std::runtime_error("Detector::Merge_smallCofQ: perpendicular mesh size does not fit");
     if (UpperBound > Events.size() || UpperBound <= LowerBound)
         throw std::runtime_error("Detector::Merge_smallCofQ: bad event bounds");
-    const int dev = Options_.GPU_ReserveDevice();
-    if (dev < 0) throw std::runtime_error("Detector::Merge_smallCofQ: no free device");
-    iac_ctx *ctx = Options_.Context(dev);
+    iac_host::Reserved res(Options_, "Detector::Merge_smallCofQ");
+    iac_ctx *ctx = res.ctx;
     if (SmallMesh.ctx != ctx) throw std::runtime_error("Detector::Merge_smallCofQ: small mesh on another device");
     BigMesh.ctx = ctx;
     const uint32_t nev = UpperBound - LowerBound;
@@ -457,5 +446,4 @@ void Detector::Merge_smallCofQ(ACMesh &BigMesh, ACMesh &SmallMesh, std::vector<S
           "iac_merge_cq in Detector::Merge_smallCofQ()");
     check(iac_u64_to_f64(ctx, big, BigMesh.NumVoxels(), mult, BigMesh.DeviceCQMesh()), "iac_u64_to_f64");
     check(iac_ctx_sync(ctx), "iac_ctx_sync");
-    Options_.GPU_FreeDevice(dev);
 }

@@ -7,7 +7,25 @@
 
 #include "iac_hip.h"
 
+class Settings;
+
 namespace iac_host {
+
+// A device of a Settings pool held for one scope: OCL_ReserveDevice at entry and
+// OCL_FreeDevice on every exit, exceptions included (IAC/Settings.cpp:432-470).
+// Throws std::runtime_error("<who>: no free device") when the pool is exhausted.
+class Reserved {
+public:
+    Reserved(Settings &pool, const char *who);
+    ~Reserved();
+    Reserved(const Reserved &) = delete;
+    Reserved &operator=(const Reserved &) = delete;
+    int dev = -1;
+    iac_ctx *ctx = nullptr;
+
+private:
+    Settings &pool_;
+};
 
 void read_raw(const std::string &path, void *dst, size_t bytes, size_t offset = 0);
 void write_raw(const std::string &path, const void *src, size_t bytes);

@@ -24,16 +24,6 @@ Settings &default_pool()
     return *pool;
 }
 
-struct Reserved {
-    Settings &O;
-    int dev;
-    explicit Reserved(Settings &o) : O(o), dev(o.GPU_ReserveDevice())
-    {
-        if (dev < 0) throw std::runtime_error("PPP: no free device");
-    }
-    ~Reserved() { O.GPU_FreeDevice(dev); }
-};
-
 }  // namespace
 
 void PhotonFinder_LargestAdjacentPixel(float *Intensity, std::vector<DetectorPanel> DetectorPanels, int FullDetSize,
@@ -52,8 +42,8 @@ void PhotonFinder_LargestAdjacentPixel(float *Intensity, const std::vector<Detec
     std::vector<iac_panel> pan;
     pan.reserve(DetectorPanels.size());
     for (const DetectorPanel &p : DetectorPanels) pan.push_back({p.FirstInd, p.Scans[0], p.Scans[1]});
-    Reserved r(Options);
-    iac_ctx *ctx = Options.Context(r.dev);
+    iac_host::Reserved r(Options, "PPP::PhotonFinder_LargestAdjacentPixel");
+    iac_ctx *ctx = r.ctx;
     // the reference's arrays span the panels' FullDetSize pixels; the rest of the frame is untouched
     const size_t bytes = (size_t)FullDetSize * sizeof(float);
     iac_host::DevBuf buf(ctx, bytes);

@@ -23,6 +23,7 @@ void Merge_ACandCQ(double *&Output, ACMesh &AC, ACMesh &CQ, Settings &)
 {
     if (!AC.d_Mesh || !CQ.d_CQMesh) throw std::runtime_error("Merge_ACandCQ: AC or C(q) mesh not calculated");
     if (AC.ctx != CQ.ctx) throw std::runtime_error("Merge_ACandCQ: meshes on different devices");
+    AC.Flush();  // events queued by Detector::AutoCorrelateSparseList
     if (AC.NumVoxels() != CQ.NumVoxels()) {
         std::cerr << "ERROR: AC and C(q) meshes differ in size\n    -> in Merge_ACandCQ()\n";
         throw std::runtime_error("Merge_ACandCQ: mesh size mismatch");
@@ -148,9 +149,10 @@ IAC_Report Run_AutoCorr_DataEval(Settings &PrgSettings, CreateDataEval_Settings 
     // AC_uw: the frame loop of IAC/RunIAC.cpp:524-550 in device stacks
     ACMesh AC_uw(&PrgSettings);
     AC_uw.CreateBigMeshForDetector(Det, EvalSettings.MeshSize, EvalSettings.QZoom);
-    const int dev = PrgSettings.GPU_ReserveDevice();
-    if (dev < 0) throw std::runtime_error("Run_AutoCorr_DataEval: no free device");
-    iac_ctx *ctx = PrgSettings.Context(dev);
+    iac_ctx *ctx = nullptr;
+    {
+    iac_host::Reserved res(PrgSettings, "Run_AutoCorr_DataEval");
+    ctx = res.ctx;
     if (Mesh_CQ.ctx != ctx) throw std::runtime_error("Run_AutoCorr_DataEval: C(q) on another device");
     AC_uw.ctx = ctx;
     const uint32_t N = upperBound - lowerBound;
@@ -188,7 +190,7 @@ IAC_Report Run_AutoCorr_DataEval(Settings &PrgSettings, CreateDataEval_Settings 
         if (echo > 2) std::cout << "AC_uw: " << (b0 + nb) << "/" << N << " events\n";
     }
     check(iac_ctx_sync(ctx), "iac_ctx_sync");
-    PrgSettings.GPU_FreeDevice(dev);
+    }
     // Report.MeanPhotonCount += PhotonCount per event, then / N (IAC/RunIAC.cpp:528, 553)
     {
         float m = 0.0f;

@@ -16,7 +16,8 @@ Settings::Settings() {}
 
 Settings::~Settings()
 {
-    for (iac_ctx *c : ctxs_) iac_ctx_destroy(c);
+    if (owns_ctxs_)
+        for (iac_ctx *c : ctxs_) iac_ctx_destroy(c);
 }
 
 void Settings::Echo(const std::string &output)
@@ -82,6 +83,16 @@ void Settings::SetUp_GPU(int MaxDevices)
     if (echo) std::cout << "GPU pool: " << n << " device(s)\n";
 }
 
+void Settings::UseContexts(const std::vector<iac_ctx *> &Contexts)
+{
+    std::lock_guard<std::mutex> g(pool_mutex_);
+    if (!ctxs_.empty()) throw std::runtime_error("Settings::UseContexts: the device pool is already set up");
+    if (Contexts.empty()) throw std::runtime_error("Settings::UseContexts: no contexts");
+    ctxs_ = Contexts;
+    available_.assign(Contexts.size(), true);
+    owns_ctxs_ = false;
+}
+
 // OCL_ReserveDevice / OCL_FreeDevice, IAC/Settings.cpp:432-470
 int Settings::GPU_ReserveDevice()
 {
@@ -125,6 +136,20 @@ void Settings::LoadFrame(const HitEvent &ev, float *dst, size_t npix) const
 }
 
 namespace iac_host {
+
+Reserved::Reserved(Settings &pool, const char *who) : pool_(pool)
+{
+    dev = pool.GPU_ReserveDevice();
+    if (dev < 0) throw std::runtime_error(std::string(who) + ": no free device");
+    try {
+        ctx = pool.Context(dev);
+    } catch (...) {
+        pool.GPU_FreeDevice(dev);
+        throw;
+    }
+}
+
+Reserved::~Reserved() { pool_.GPU_FreeDevice(dev); }
 
 // ArrayOperators::LoadArrayFromFile / SafeArrayToFile (Binary), IAC/ArrayOperators.h
 void read_raw(const std::string &path, void *dst, size_t bytes, size_t offset)

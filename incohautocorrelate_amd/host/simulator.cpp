@@ -4,6 +4,7 @@
 // Poisson, noise and the event bookkeeping of SimulatePart.
 #include <algorithm>
 #include <cmath>
+#include <memory>
 #include <random>
 #include <stdexcept>
 #include <vector>
@@ -66,9 +67,8 @@ void Simulator::Simulate(Detector &Det, const std::vector<std::array<double, 3>>
     Output.DetectorSize[1] = Det.DetectorSize[1];
     if (N == 0) return;
 
-    const int dev = Options.GPU_ReserveDevice();
-    if (dev < 0) throw std::runtime_error("Simulator::Simulate: no free device");
-    iac_ctx *ctx = Options.Context(dev);
+    iac_host::Reserved res(Options, "Simulator::Simulate");
+    iac_ctx *ctx = res.ctx;
     // positions and phases as the reference hands them to the kernel: float
     // (IAC/Simulator.cpp:727-737), carried in double to the device
     std::vector<double> em(3 * (size_t)nem);
@@ -79,6 +79,8 @@ void Simulator::Simulate(Detector &Det, const std::vector<std::array<double, 3>>
     d_em.upload(em.data(), em.size() * 8);
     const bool direct = SimSettings.Precision == IAC_SIM_FP32_DIRECT;
     iac_sim_plan *plan = nullptr;
+    // the plan is destroyed on every exit, exceptions included
+    std::unique_ptr<iac_sim_plan, int (*)(iac_sim_plan *)> plan_guard(nullptr, iac_sim_plan_destroy);
     iac_host::DevBuf d_em32;
     if (direct) {
         std::vector<float> e32(em.begin(), em.end());
@@ -89,6 +91,7 @@ void Simulator::Simulate(Detector &Det, const std::vector<std::array<double, 3>>
         check(iac_sim_plan_create(ctx, d_pm.as<float>(), (uint32_t)P, (int)SimSettings.SubSampling, us, vs,
                                   SimSettings.Wavelength, d_em.as<double>(), nem, SimSettings.Precision, &plan),
               "iac_sim_plan_create in Simulator::Simulate()");
+        plan_guard.reset(plan);
     }
     std::mt19937_64 rng(SimSettings.Seed);
     std::uniform_real_distribution<double> uni(0.0, 1.0);
@@ -139,6 +142,4 @@ void Simulator::Simulate(Detector &Det, const std::vector<std::array<double, 3>>
             Output.HitEvents.push_back(ev);
         }
     }
-    if (plan) iac_sim_plan_destroy(plan);
-    Options.GPU_FreeDevice(dev);
 }

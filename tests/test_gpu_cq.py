@@ -158,6 +158,89 @@ def test_dense_float_rotation_merge_vs_fp64(ctx, dev):
     assert abs(got.sum() - ref.sum()) <= 1e-6 * ref.sum()
 
 
+def _cfg3_setup(dev, nframes, norient, seed):
+    """The dense-float configuration at its shape: 256^2 detector, Gamma(1, 0.5) frames,
+    frame f in orientation f mod norient, small C(q) 129x129x3, lab mesh 129^3."""
+    import incohautocorrelate_amd as iac
+
+    n = 256
+    _, km, mq = W.detector(n)
+    s = iac.mesh_shape_small_cofq(mq, 127)
+    frames = W.gamma_frames(n * n, nframes, 1.0, 0.5, seed=seed)
+    orient = np.arange(nframes) % norient
+    order = np.argsort(orient, kind="stable")
+    goff = np.searchsorted(orient[order], np.arange(norient + 1)).astype(np.uint32)
+    rw = np.zeros((norient, 10), np.float32)
+    rw[:, :9] = W.random_rotations(norient, seed + 1)
+    rw[:, 9] = 1.0
+    return km, s, frames, orient, order, goff, rw
+
+
+def _cfg3_gpu(ctx, dev, km, s, frames, order, goff, rw, m1, m2):
+    import torch
+
+    norient = len(goff) - 1
+    nsmall = s.size_ab ** 2 * s.size_c
+    small_u = torch.zeros(norient * nsmall, dtype=torch.int64, device=dev)
+    ctx.cq_dense_frames(to_dev(km, dev), to_dev(frames[order], dev), goff, s, small_u, 0, m1)
+    small = torch.zeros(norient * nsmall, dtype=torch.float64, device=dev)
+    ctx.u64_to_f64(small_u, m1, small)
+    big_u = torch.zeros(s.size_ab ** 3, dtype=torch.int64, device=dev)
+    ctx.merge_cq_multi(small, s, to_dev(rw, dev), 0, m2, big_u)
+    return mesh_np(small_u), small.cpu().numpy(), mesh_np(big_u)
+
+
+@pytest.mark.timeout(400)
+def test_cfg3_shape_reference_multiplicator_bit_exact(ctx, dev):
+    """256^2 Gamma(1, 0.5) frames, 2 orientations x 2 frames, with the reference's
+    fixed point: the C(q) small Multiplicator of Detector.cpp:1215-1227 (the smallest
+    10^k >= 1e-10 with <I>^2 10^k >= 1, times 1e5: 1e6 for these frames) and the merge
+    Multiplicator of Detector.cpp:1395-1413 from the merged weights.  The uint64 small
+    meshes and the uint64 lab mesh equal the oracle bit for bit."""
+    km, s, frames, orient, order, goff, rw = _cfg3_setup(dev, 4, 2, seed=31)
+    m1 = O.cq_small_multiplicator(frames.mean(axis=0))
+    assert m1 == 1e6
+    nsmall = s.size_ab ** 2 * s.size_c
+    ref_small = np.zeros((2, nsmall), np.uint64)
+    for f in range(4):
+        O.cq_small(frames[f], km, s, 0, m1, mesh=ref_small[orient[f]])
+    small_d = ref_small.astype(np.float64) / m1
+    # the merge Multiplicator from the first orientation's mesh and unit weights
+    _, m2 = O.merge_prepare(rw[:, :9], np.ones(2, np.float32), small_d[0])
+    got_small, got_small_d, got_big = _cfg3_gpu(ctx, dev, km, s, frames, order, goff, rw, m1, m2)
+    assert np.array_equal(got_small.reshape(2, nsmall), ref_small)
+    assert np.array_equal(got_small_d.reshape(2, nsmall), small_d)
+    ref_big = np.zeros(s.size_ab ** 3, np.uint64)
+    for o in range(2):
+        ref_big += O.merge_cq(small_d[o], s, rw[o:o + 1], 0, m2)[0]
+    assert np.array_equal(got_big, ref_big)
+    assert ref_big.sum() > 0
+
+
+@pytest.mark.timeout(400)
+def test_cfg3_shape_fp64_1e6(ctx, dev):
+    """The same shape with the finer fixed point (small mesh M = 1e10, merge M = 1e6)
+    against the FP64 oracle: every populated lab voxel within 1e-6 relative.  With the
+    reference's 1e6 the truncation of each pair value alone moves a voxel by about
+    2e-6 (a bias of half a unit in 1e6 per pair on values near 0.25), above that bar."""
+    km, s, frames, orient, order, goff, rw = _cfg3_setup(dev, 4, 2, seed=37)
+    _, _, got_big = _cfg3_gpu(ctx, dev, km, s, frames, order, goff, rw, 1e10, 1e6)
+    got = got_big.astype(np.float64) / 1e6
+    nsmall = s.size_ab ** 2 * s.size_c
+    ref = np.zeros(s.size_ab ** 3)
+    for o in range(2):
+        sm = np.zeros(nsmall)
+        for f in range(4):
+            if orient[f] == o:
+                sm = O.cq_small_f64(frames[f], km, s, mesh=sm)
+        ref = O.merge_f64(sm, s, rw[o], big=ref)
+    pop = ref > 1e-3 * ref.max()
+    assert pop.sum() > 1000
+    rel = np.abs(got[pop] - ref[pop]) / ref[pop]
+    print(f"max relative error over {pop.sum()} populated voxels: {rel.max():.3e}")
+    assert rel.max() <= 1e-6, rel.max()
+
+
 @pytest.mark.parametrize("mode,cell,box,region", [(1, 4, 4096, 0), (1, 8, 4096, 0), (1, 6, 1920, 0), (2, 4, 4096, 0),
                                                 (2, 6, 1920, 0), (1, 3, 1920, 2), (2, 3, 1920, 2), (2, 3, 1024, 0),
                                                 (2, 5, 4096, 3)])
