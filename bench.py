@@ -44,6 +44,8 @@ BYTES_PER_UPDATE = 16  # one 8 B bin read + one 8 B bin write per pair-update
 # (pair mode, box, cell_vox, frame_chunk): FETCH_SIZE x2.0 + WRITE_SIZE x1.0 KiB from two
 # rocprofv3 --pmc passes, scaled by a 2 GiB copy kernel (profiles/pairs_lds_sparse1024_10k.md).
 MEASURED_TRAFFIC = {(1, 1920, 6, 32): (2 * 1422613 + 2891933) * 1024}
+# The same for sim_gemm_kernel per launch of 1,000 frames (profiles/sim_gemm_mfma_512_10k_1000.md)
+SIM_TRAFFIC = {"bf16x3": (2 * 70271911 + 1025181) * 1024}
 
 
 def log(*a):
@@ -85,6 +87,7 @@ def parse():
     ap.add_argument("--sim-frames", type=int, default=1000)
     ap.add_argument("--sim-steps", type=int, default=3)
     ap.add_argument("--sim-direct-frames", type=int, default=8)
+    ap.add_argument("--sim-modes", default="bf16x3,fp16x3,fp16", help="GEMM modes of the simulator leg")
     return ap.parse_args()
 
 
@@ -1144,8 +1147,11 @@ def sim_leg(args, iac, W, dev):
         return float(np.linalg.norm(a - b) / np.linalg.norm(b))
 
     res = {}
+    want = set(args.sim_modes.split(","))
     for mode, name, dt, nterms in ((iac.SIM_BF16X3, "bf16x3", "bf16", 3), (iac.SIM_FP16X3, "fp16x3", "fp16", 3),
                                    (iac.SIM_FP16, "fp16", "fp16", 1)):
+        if name not in want:
+            continue
         flops = 8.0 * npix * nem * nf  # algorithmic: one complex multiply-add per (pixel, emitter, frame)
         issued = nterms * flops  # the split products the MFMAs execute
         torch.cuda.synchronize()
@@ -1183,7 +1189,8 @@ def sim_leg(args, iac, W, dev):
             "frames_per_s": nf / (ms * 1e-3), "ms_per_step": ms, "gemm_ms": gemm_ms, "plan_s": plan_s,
             "err_vs_fp64": errs, "photons_per_frame": photons,
             "roofline": {"bound": "mfma", "achieved": ach, "peak": MFMA_PEAK_TFLOPS[dt], "unit": "TFLOP/s",
-                         "frac": ach / MFMA_PEAK_TFLOPS[dt], "traffic": None,
+                         "frac": ach / MFMA_PEAK_TFLOPS[dt], "traffic": SIM_TRAFFIC.get(name),
+                         "traffic_unit": "HBM bytes per launch (rocprofv3 FETCH_SIZE x2 + WRITE_SIZE, copy-scaled)",
                          "kernel": f"sim_gemm_kernel ({dt} v_mfma_f32_32x32x16, {nterms} product(s) per term, "
                                    "fused |psi| epilogue)",
                          "flops_per_step": flops, "flops_basis": "algorithmic 8 P E per frame",
