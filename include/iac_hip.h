@@ -282,6 +282,16 @@ int iac_ac1d_acuw(iac_ctx *ctx, const float *d_kmap, uint32_t npix, const uint32
 int iac_ac1d_acuw_records(iac_ctx *ctx, const float *d_rec4, const uint64_t *d_frame_off, uint32_t nframes,
                           uint64_t n_total, uint32_t size, float max_q, float dq, int interp, int self_pair_factor,
                           double *d_hist);
+/* iac_ac1d_acuw_records without any allocation.  No counterpart in the reference: for
+ * callers that cannot allocate device memory (code under HIP graph capture, tight serving
+ * loops, harnesses that forbid hipMalloc).  Same arguments and result as
+ * iac_ac1d_acuw_records, at most 65,536 frames per call (IAC_ERR_LIMIT beyond).  Its work
+ * memory is a static device array of 560 KiB per GPU, held until the process ends and
+ * shared by every call on that GPU: calls that overlap on one GPU (several contexts or
+ * streams at once) race; calls in one stream do not.  Not timed by iac_ctx_enable_timing. */
+int iac_ac1d_acuw_records_noalloc(iac_ctx *ctx, const float *d_rec4, const uint64_t *d_frame_off, uint32_t nframes,
+                                  uint64_t n_total, uint32_t size, float max_q, float dq, int interp,
+                                  int self_pair_factor, double *d_hist);
 /* AC1D::CalcAC, IAC/AC1D.cpp:482-493: ac = ac_uw / cq, NaN -> 0. */
 int iac_ac1d_calc_ac(iac_ctx *ctx, const double *d_ac_uw, const double *d_cq, uint32_t size,
                      double *d_ac);

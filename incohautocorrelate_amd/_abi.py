@@ -111,6 +111,7 @@ SIGNATURES: dict[str, tuple] = {
     "iac_intensity_accumulate": (I32, [P, P, U32, U32, P, C.c_float, C.c_float, P, P, P, P]),
     "iac_intensity_finish": (I32, [P, U32, U32, C.c_float, P, P, P]),
     "iac_ac1d_acuw_records": (I32, [P, P, P, U32, U64, U32, C.c_float, C.c_float, I32, I32, P]),
+    "iac_ac1d_acuw_records_noalloc": (I32, [P, P, P, U32, U64, U32, C.c_float, C.c_float, I32, I32, P]),
     "iac_sim_run": (I32, [P, P, P, U32, U32, I32, P]),
     "iac_sim_direct_fp32": (I32, [P, P, U32, I32, C.POINTER(C.c_float), C.POINTER(C.c_float), C.c_float, P, U32, P,
                                   U32, U32, I32, P]),

@@ -16,8 +16,10 @@
 // for bit (all partial sums are integers below 2^53).
 #include <algorithm>
 #include <cmath>
+#include <mutex>
 #include <numeric>
 #include <type_traits>
+#include <vector>
 
 #include "iac_fpexact.h"
 #include "iac_internal.h"
@@ -380,17 +382,53 @@ size_t acuw_lds(uint32_t size) { return (size_t)(size + 1) / 2 * sizeof(float4) 
 
 namespace {
 
-// Shared tail of the AC_uw entry points: frame tiles, then the histogram
-// kernel (uint64 bins when every weight is an integer, else double bins).
-int acuw_run(iac_ctx *ctx, const float4 *rec, const uint64_t *d_frame_off, uint32_t nframes, uint32_t size,
-             float max_q, float dq, int interp, int self_pair_factor, bool exact, double *d_hist)
+// Work memory of iac_ac1d_acuw_records_noalloc: a file-scope device array (one copy per
+// GPU, held until the process ends) sized for kNoAllocMaxFrames frames and kMaxBins bins.
+// Host code never passes `scratch` itself to a kernel (its host-side name is not a device
+// address): noalloc_scratch looks up each device's copy once, with that device current.
+constexpr uint32_t kNoAllocMaxFrames = 65536;
+constexpr size_t kNoAllocScratchBytes = (size_t)(kNoAllocMaxFrames + 1) * 8 + (size_t)kMaxBins * 8;
+static __device__ __align__(256) unsigned char scratch[kNoAllocScratchBytes];
+
+std::mutex g_noalloc_mutex;
+std::vector<void *> g_noalloc_ptr;  // per device: the address of its copy of `scratch`
+
+// The device address of `device`'s copy of scratch (the caller has made that device current).
+int noalloc_scratch(int device, void **out)
+{
+    std::lock_guard<std::mutex> g(g_noalloc_mutex);
+    if ((size_t)device >= g_noalloc_ptr.size()) g_noalloc_ptr.resize((size_t)device + 1, nullptr);
+    if (!g_noalloc_ptr[device]) {
+        void *p = nullptr;
+        IAC_HIP(hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)));
+        g_noalloc_ptr[device] = p;
+    }
+    *out = g_noalloc_ptr[device];
+    return IAC_OK;
+}
+
+int acuw_check(uint32_t size, int interp, int self_pair_factor)
 {
     if (size == 0 || size > (uint32_t)kMaxBins) return iac::fail(IAC_ERR_LIMIT, "AC1D size must be 1..6144 bins");
     if (interp != IAC_NEAREST && interp != IAC_LINEAR) return iac::fail(IAC_ERR_INVALID_ARG, "bad interpolation");
     if (self_pair_factor != 1 && self_pair_factor != 2)
         return iac::fail(IAC_ERR_INVALID_ARG, "self_pair_factor is 1 or 2");
-    void *s1 = nullptr;
-    IAC_TRY(iac::scratch(ctx, 1, (size_t)(nframes + 1) * sizeof(uint64_t) + (size_t)size * 8, &s1));
+    return IAC_OK;
+}
+
+// bytes of work memory of acuw_run: the frames' tile offsets, then (exact bins) the uint64 histogram
+size_t acuw_work_bytes(uint32_t nframes, uint32_t size)
+{
+    return (size_t)(nframes + 1) * sizeof(uint64_t) + (size_t)size * 8;
+}
+
+// Shared tail of the AC_uw entry points: frame tiles, then the histogram
+// kernel (uint64 bins when every weight is an integer, else double bins), with
+// acuw_work_bytes of work memory at s1 (every word is written before it is read).
+int acuw_run(iac_ctx *ctx, void *s1, const float4 *rec, const uint64_t *d_frame_off, uint32_t nframes, uint32_t size,
+             float max_q, float dq, int interp, int self_pair_factor, bool exact, double *d_hist, bool timed)
+{
+    IAC_TRY(acuw_check(size, interp, self_pair_factor));
     auto *toff = static_cast<uint64_t *>(s1);
     auto *hu = reinterpret_cast<unsigned long long *>(static_cast<char *>(s1) + (size_t)(nframes + 1) * 8);
     hipLaunchKernelGGL(tile_scan_kernel, dim3(1), dim3(1024), 0, ctx->stream, d_frame_off, nframes, toff);
@@ -402,12 +440,12 @@ int acuw_run(iac_ctx *ctx, const float4 *rec, const uint64_t *d_frame_off, uint3
                               : (nd ? ac1d_acuw_kernel<true, false, true> : ac1d_acuw_kernel<true, false, false>))
                       : (fdiv ? (nd ? ac1d_acuw_kernel<false, true, true> : ac1d_acuw_kernel<false, true, false>)
                               : (nd ? ac1d_acuw_kernel<false, false, true> : ac1d_acuw_kernel<false, false, false>));
-    IAC_TRY(iac::timer_begin(ctx));
+    if (timed) IAC_TRY(iac::timer_begin(ctx));
     if (exact) hipLaunchKernelGGL(zero_u64_kernel, dim3(1), dim3(256), 0, ctx->stream, hu, size);
     hipLaunchKernelGGL(kern, dim3(persistent_blocks(ctx)), dim3(kBlock), acuw_lds(size), ctx->stream, rec,
                        d_frame_off, toff, nframes, size, max_q, dq, inv_dq, interp, sf, hu, d_hist, ctx->d_counters);
     IAC_HIP(hipGetLastError());
-    IAC_TRY(iac::timer_end(ctx));
+    if (timed) IAC_TRY(iac::timer_end(ctx));
     if (exact) {
         hipLaunchKernelGGL(add_u64_as_f64_kernel, dim3(std::max(1u, (size + 255) / 256)), dim3(256), 0, ctx->stream,
                            hu, size, d_hist);
@@ -486,8 +524,11 @@ int iac_ac1d_acuw(iac_ctx *ctx, const float *d_kmap, uint32_t npix, const uint32
                        ctx->d_counters + 1);
     IAC_HIP(hipGetLastError());
     // uint16 counts: every weight is an integer-valued float, exact in uint64 bins
-    return acuw_run(ctx, rec, d_frame_off, nframes, size, max_q, dq, interp, self_pair_factor,
-                    interp == IAC_NEAREST, d_hist);
+    IAC_TRY(acuw_check(size, interp, self_pair_factor));
+    void *s1 = nullptr;
+    IAC_TRY(iac::scratch(ctx, 1, acuw_work_bytes(nframes, size), &s1));
+    return acuw_run(ctx, s1, rec, d_frame_off, nframes, size, max_q, dq, interp, self_pair_factor,
+                    interp == IAC_NEAREST, d_hist, true);
 }
 
 int iac_ac1d_acuw_records(iac_ctx *ctx, const float *d_rec4, const uint64_t *d_frame_off, uint32_t nframes,
@@ -498,8 +539,28 @@ int iac_ac1d_acuw_records(iac_ctx *ctx, const float *d_rec4, const uint64_t *d_f
     if (!d_frame_off || !d_hist) return iac::fail(IAC_ERR_INVALID_ARG, "null input");
     if (nframes == 0 || n_total == 0) return IAC_OK;
     if (!d_rec4) return iac::fail(IAC_ERR_PRECONDITION, "no hit-list records");
-    return acuw_run(ctx, reinterpret_cast<const float4 *>(d_rec4), d_frame_off, nframes, size, max_q, dq, interp,
-                    self_pair_factor, false, d_hist);
+    IAC_TRY(acuw_check(size, interp, self_pair_factor));
+    void *s1 = nullptr;
+    IAC_TRY(iac::scratch(ctx, 1, acuw_work_bytes(nframes, size), &s1));
+    return acuw_run(ctx, s1, reinterpret_cast<const float4 *>(d_rec4), d_frame_off, nframes, size, max_q, dq, interp,
+                    self_pair_factor, false, d_hist, true);
+}
+
+int iac_ac1d_acuw_records_noalloc(iac_ctx *ctx, const float *d_rec4, const uint64_t *d_frame_off, uint32_t nframes,
+                                  uint64_t n_total, uint32_t size, float max_q, float dq, int interp,
+                                  int self_pair_factor, double *d_hist)
+{
+    IAC_TRY(iac::bind(ctx));
+    if (!d_frame_off || !d_hist) return iac::fail(IAC_ERR_INVALID_ARG, "null input");
+    if (nframes > kNoAllocMaxFrames)
+        return iac::fail(IAC_ERR_LIMIT, "iac_ac1d_acuw_records_noalloc: at most 65536 frames per call");
+    IAC_TRY(acuw_check(size, interp, self_pair_factor));
+    if (nframes == 0 || n_total == 0) return IAC_OK;
+    if (!d_rec4) return iac::fail(IAC_ERR_PRECONDITION, "no hit-list records");
+    void *s1 = nullptr;
+    IAC_TRY(noalloc_scratch(ctx->device, &s1));
+    return acuw_run(ctx, s1, reinterpret_cast<const float4 *>(d_rec4), d_frame_off, nframes, size, max_q, dq, interp,
+                    self_pair_factor, false, d_hist, false);
 }
 
 int iac_ac1d_calc_ac(iac_ctx *ctx, const double *d_ac_uw, const double *d_cq, uint32_t size, double *d_ac)
