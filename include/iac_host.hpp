@@ -25,6 +25,7 @@
 #include <functional>
 #include <mutex>
 #include <string>
+#include <unordered_map>
 #include <vector>
 
 #include "iac_hip.h"
@@ -36,6 +37,12 @@ public:
     ~Settings();
     Settings(const Settings &) = delete;
     Settings &operator=(const Settings &) = delete;
+
+    // IAC/Settings.h:40-42
+    const static int XML_HITLIST_VERSION = 9;
+    const static int INTERNAL_VERSION = 12;
+    const static int INTERNAL_VERSION_Revision = 2;
+    static std::string GetVersion();
 
     // IAC/Settings.h:105-119
     struct HitEvent {
@@ -51,6 +58,12 @@ public:
         const float *Data = nullptr;  // in-memory frame (DetectorSize[0] * DetectorSize[1] floats)
     };
     std::vector<HitEvent> HitEvents;
+    // IAC/Settings.cpp:473-676: the hit-event list file (XML, or JSON / INFO by extension)
+    void LoadHitEventListFromFile(std::string Filename);
+    void SafeHitEventListToFile(std::string Filename);
+    void SafeHitEventListToFile(std::string Filename, std::vector<HitEvent> &HitEventList,
+                                bool AdditionalInformations = false,
+                                std::unordered_map<std::string, std::string> AdditioInfoMap = {});
     bool echo = true;
     void Echo(const std::string &output);
 
@@ -320,11 +333,23 @@ public:
 
 // IAC/RunIAC.cpp:120-158
 void Merge_ACandCQ(double *&Output, ACMesh &AC, ACMesh &CQ, Settings &PrgSettings);
-// IAC/RunIAC.cpp:185-632; Settings::HitEvents is the event list (XML_Path is not read)
+// IAC/RunIAC.cpp:185-632; the event list is read from XML_Path when it is set, else
+// Settings::HitEvents is used as it stands
 IAC_Report Run_AutoCorr_DataEval(Settings &PrgSettings, CreateDataEval_Settings EvalSettings);
 // An evaluation whose detector the caller prepared (pixel map, k-map, mask loaded).
 IAC_Report Run_AutoCorr_DataEval(Settings &PrgSettings, CreateDataEval_Settings EvalSettings, Detector &Det);
 }  // namespace RunIAC
+
+// IAC/MainRunModes.h: the evaluation settings file and the evaluate run mode
+namespace MainRunModes {
+// IAC/MainRunModes.cpp:32-91 (plus root.EvalSettings.InputFiles.PixelMap_Size "SS; FS" for raw
+// pixel maps; without it the size is taken from the file)
+RunIAC::CreateDataEval_Settings LoadEvaluationSettings(std::string Filename, Settings &Options);
+// IAC/MainRunModes.cpp:354-500
+int Create_Example_Evaluation_Config_File(std::string Filename, Settings &Options);
+// IAC/MainRunModes.cpp:1254-1330: settings -> Run_AutoCorr_DataEval -> report file
+int AutoCorrelateData(std::string ConfigFile, Settings &Options);
+}  // namespace MainRunModes
 
 // IAC/PPP.h:11-101: the per-frame operations of pattern pre-processing (the HDF5 / XML handling
 // of ProcessData_PF_LAP stays with the caller; iac_ppp_lap runs a whole stack with mask, gain

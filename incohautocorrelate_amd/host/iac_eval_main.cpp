@@ -1,6 +1,6 @@
-// iac_eval_main.cpp -- command-line driver of RunIAC::Run_AutoCorr_DataEval on
-// raw binary inputs (the evaluation run mode of IAC/MainRunModes.cpp:1254-1330
-// without the XML / HDF5 readers):
+// iac_eval_main.cpp -- command-line driver of RunIAC::Run_AutoCorr_DataEval: the evaluate
+// run mode of IAC/MainRunModes.cpp:1254-1330 over a settings file (iac_eval -e Settings.xml,
+// the hit-event list named in it; arrays are raw binary files), or flags over raw inputs:
 //   iac_eval --pixelmap PM --size SS FS --frames STACK --nframes N [options]
 // PM: float32 [SS][FS][3]; STACK: float32 [N][SS][FS]; --mask: int32 [SS][FS];
 // --rotations: float32 [N][9] row major (default identity).  Outputs are raw
@@ -24,6 +24,8 @@ static void usage()
                  "  [--bounds LO HI] [--invert-rot] [--ignore-orientation] [--mask-flatfield] [--batch B]\n"
                  "  [--out-avint F] [--out-acuw F] [--out-cq F] [--out-cq-small F] [--out-final F] [--out-q F]\n"
                  "  [--echo L] [--version]\n"
+                 "       iac_eval -e Settings.xml      (evaluation settings file, hit-event list from it)\n"
+                 "       iac_eval -eec Example.xml     (write an example evaluation settings file)\n"
                  "       iac_eval --lap STACK N NPIX PANELS ADU SEED COMB OUT\n"
                  "  (PPP::PhotonFinder_LargestAdjacentPixel frame by frame; PANELS = first:fs:ss,...)\n";
 }
@@ -67,6 +69,29 @@ static int lap_main(int argc, char **argv)
 int main(int argc, char **argv)
 {
     if (argc >= 2 && std::string(argv[1]) == "--lap") return lap_main(argc, argv);
+    if (argc >= 2) {
+        // the reference's run modes over settings files (IAC/main.cpp:560-600)
+        const std::string a1 = argv[1];
+        const bool eval = a1 == "evaluate" || a1 == "-evaluate" || a1 == "e" || a1 == "-e";
+        const bool example = a1 == "exampleevaluationconfig" || a1 == "-exampleevaluationconfig" || a1 == "eec" ||
+                             a1 == "-eec";
+        if (eval || example) {
+            if (argc < 3) {
+                std::cerr << (eval ? "-Evaluate" : "-ExampleEvaluateConfig")
+                          << " requires one additional argument (\"Settings.xml\")\n";
+                return -1;
+            }
+            try {
+                Settings S;
+                if (example) return MainRunModes::Create_Example_Evaluation_Config_File(argv[2], S);
+                std::cout << "Run ICA in evaluation mode - autocorrelation\n";
+                return MainRunModes::AutoCorrelateData(argv[2], S);
+            } catch (const std::exception &ex) {
+                std::cerr << "ERROR: " << ex.what() << "\n";
+                return 1;
+            }
+        }
+    }
     RunIAC::CreateDataEval_Settings E;
     E.EchoLevel = 0;
     std::string frames, rotations;

@@ -6,6 +6,7 @@
 // vector, ratio.  Outputs are raw binary arrays as SafeArrayToFile(Binary) writes.
 #include <algorithm>
 #include <cmath>
+#include <fstream>
 #include <iostream>
 #include <stdexcept>
 #include <vector>
@@ -47,8 +48,15 @@ IAC_Report Run_AutoCorr_DataEval(Settings &PrgSettings, CreateDataEval_Settings 
 {
     if (EvalSettings.PixelMap_Path.empty())
         throw std::runtime_error("Run_AutoCorr_DataEval: no pixel map path set => can't load Hit Events");
-    if (!EvalSettings.DetectorSize[0] || !EvalSettings.DetectorSize[1])
-        throw std::runtime_error("Run_AutoCorr_DataEval: DetectorSize of the raw pixel map not set");
+    if (!EvalSettings.DetectorSize[0] || !EvalSettings.DetectorSize[1]) {
+        // the raw pixel map holds [ss][fs][3] floats; without a stated size, one row of all pixels
+        std::ifstream f(EvalSettings.PixelMap_Path, std::ios::binary | std::ios::ate);
+        if (!f) throw std::runtime_error("Run_AutoCorr_DataEval: cannot open pixel map " + EvalSettings.PixelMap_Path);
+        const long long bytes = (long long)f.tellg();
+        if (bytes <= 0 || bytes % 12) throw std::runtime_error("Run_AutoCorr_DataEval: pixel map is not [n][3] floats");
+        EvalSettings.DetectorSize[0] = 1;
+        EvalSettings.DetectorSize[1] = (unsigned int)(bytes / 12);
+    }
     Detector Det(PrgSettings);
     Det.LoadPixelMap(EvalSettings.PixelMap_Path, EvalSettings.DetectorSize[0], EvalSettings.DetectorSize[1]);
     if (EvalSettings.DetDisturb)
@@ -64,8 +72,11 @@ IAC_Report Run_AutoCorr_DataEval(Settings &PrgSettings, CreateDataEval_Settings 
     const int echo = EvalSettings.EchoLevel;
     if (!Det.Checklist.KMap) throw std::runtime_error("Run_AutoCorr_DataEval: detector k-map not calculated");
     if (!Det.Checklist.PixelMask) Det.GenerateFlatOnesPixelMask();
-    if (!EvalSettings.XML_Path.empty() && PrgSettings.HitEvents.empty())
-        throw std::runtime_error("Run_AutoCorr_DataEval: XML hit-event lists are not read here; fill Settings::HitEvents");
+    if (!EvalSettings.XML_Path.empty()) {  // IAC/RunIAC.cpp:227-240
+        PrgSettings.Echo("Load Hit Event List");
+        PrgSettings.LoadHitEventListFromFile(EvalSettings.XML_Path);
+    }
+    if (PrgSettings.HitEvents.empty()) throw std::runtime_error("Run_AutoCorr_DataEval: no hit events");
     if (EvalSettings.InvertRotMatrix) PrgSettings.InvertRotationMatrices();
     unsigned int lowerBound = 0, upperBound = (unsigned int)PrgSettings.HitEvents.size();
     if (EvalSettings.RestrictStackToBoundaries) {

@@ -14,6 +14,16 @@
 
 Settings::Settings() {}
 
+// IAC/Settings.h:86-99: "M.mm.rr" from INTERNAL_VERSION and its revision
+std::string Settings::GetVersion()
+{
+    std::string ret = std::to_string(INTERNAL_VERSION / 100) + ".";
+    if (INTERNAL_VERSION % 100 < 10) ret += "0";
+    ret += std::to_string(INTERNAL_VERSION % 100) + ".";
+    if (INTERNAL_VERSION_Revision / 10 == 0) ret += "0";
+    return ret + std::to_string(INTERNAL_VERSION_Revision);
+}
+
 Settings::~Settings()
 {
     if (owns_ctxs_)

@@ -178,3 +178,59 @@ def test_host_mirror_lap_photon_finder(tmp_path):
     ref, _, _ = O.ppp_lap(I, panels, adu=adu)
     assert np.array_equal(out.view(np.int32), ref.view(np.int32))
     assert np.array_equal(out[:, full:], I[:, full:])
+
+
+def test_eval_settings_file_vs_oracle(tmp_path):
+    """The evaluate run mode over files in the reference's layouts: iac_eval -e Settings.xml
+    (MainRunModes::AutoCorrelateData, LoadEvaluationSettings) reading a hit-event list XML
+    (Settings::LoadHitEventListFromFile: root.Info.Size / Version 9, root.Content.N<i> with
+    Filename / Event / Rotation strings) whose events point into a raw frame stack; every
+    output against the oracle, as in the flag-driven runs, and the report file written."""
+    import incohautocorrelate_amd as iac
+
+    n, nf, edge, off, step = 40, 17, 31, 3.2, 6.4
+    pm, km, mq, mask, frames, rots, f = _inputs(tmp_path, n, nf, seed=5, rotate=True)
+    events = "".join(
+        f"<N{i}><Filename>{f['frames']}</Filename><Dataset>/data</Dataset><Event>{i}</Event>"
+        f"<SerialNumber>{i}</SerialNumber><MeanIntensity>-1</MeanIntensity><PhotonCount>-1</PhotonCount>"
+        f"<SupplementInfo></SupplementInfo><Rotation>{'; '.join('%.9g' % x for x in rots[i])}</Rotation></N{i}>"
+        for i in range(nf))
+    hl = tmp_path / "EventList.xml"
+    hl.write_text(f'<?xml version="1.0" encoding="utf-8"?>\n<root><Info><Size>{nf}</Size><Version>9</Version>'
+                  f"</Info><Content>{events}</Content></root>")
+    out = {k: tmp_path / f"{k}.bin" for k in ("avint", "acuw", "cq", "final")}
+    settings = tmp_path / "Settings.xml"
+    settings.write_text(f"""<?xml version="1.0" encoding="utf-8"?>
+<root><EvalSettings>
+<InputFiles><EventList_Path>{hl}</EventList_Path><PixelMap_Path>{f['pm']}</PixelMap_Path>
+<PixelMap_Size>{n}; {n}</PixelMap_Size><PixelMask_Path>{f['mask']}</PixelMask_Path></InputFiles>
+<OutputFiles><AC_Final_Path>{out['final']}</AC_Final_Path><AvIntensity_Path>{out['avint']}</AvIntensity_Path>
+<Cq_Merged_Path>{out['cq']}</Cq_Merged_Path><ACuw_Path>{out['acuw']}</ACuw_Path></OutputFiles>
+<OutputMesh><MeshSize>{edge}</MeshSize><QZoom>1</QZoom><DoubleMap>true</DoubleMap></OutputMesh>
+<PhotonThreshold><PhotonOffset>{off}</PhotonOffset><PhotonStep>{step}</PhotonStep></PhotonThreshold>
+<Misc><EchoLevel>0</EchoLevel><Report_Path>{tmp_path / 'Report.txt'}</Report_Path></Misc>
+</EvalSettings></root>
+""")
+    r = subprocess.run([str(EVAL), "-e", str(settings)], capture_output=True, text=True, timeout=120)
+    assert r.returncode == 0, r.stderr + r.stdout
+    avg, mean_i, _ = O.average_intensity(frames, off, step, mask)
+    assert np.array_equal(np.fromfile(out["avint"], np.float32), avg)
+    I = avg * mask.astype(np.float32)
+    ss = iac.mesh_shape_small_cofq(mq, edge)
+    mult = O.cq_small_multiplicator(I)
+    small = O.cq_small(I, km, ss, 0, mult)[0].astype(np.float64) / mult
+    rw, m2 = O.merge_prepare(rots, mean_i, small)
+    big = O.merge_cq(small, ss, rw, 0, m2)[0].astype(np.float64) / m2
+    assert np.array_equal(np.fromfile(out["cq"], np.float64), big)
+    bs = iac.mesh_shape_big(mq, edge)
+    mesh = np.zeros(bs.size_ab ** 2 * bs.size_c, np.uint64)
+    for e in range(nf):
+        mesh, _ = O.ac3d_sparse(O.sparse_hit_list(frames[e], km, off, step, mask), bs, rots[e], 0, 0, True,
+                                mesh=mesh)
+    assert mesh.sum() > 0
+    acuw = (mesh.astype(np.float32) * np.float32(0.001)).astype(np.float64)
+    assert np.array_equal(np.fromfile(out["acuw"], np.float64), acuw)
+    assert np.array_equal(np.fromfile(out["final"], np.float64), O.merge_ac_cq(mesh, big))
+    report = (tmp_path / "Report.txt").read_text()
+    assert "AC-Volume (3D - Mode)" in report and f"Final Mesh Size: {bs.size_ab} x" in report
+    assert f"Evaluated Stack: Events: 0 - {nf}" in report
