@@ -78,6 +78,10 @@ int iac_dev_free(iac_ctx *ctx, void *d_ptr);
 int iac_memcpy_h2d(iac_ctx *ctx, void *d_dst, const void *h_src, size_t bytes);
 int iac_memcpy_d2h(iac_ctx *ctx, void *h_dst, const void *d_src, size_t bytes);
 int iac_memset_zero(iac_ctx *ctx, void *d_ptr, size_t bytes);
+/* Page-locked host memory for frame staging (the reference stages through pageable
+ * new[] arrays, e.g. IAC/Detector.cpp:1093-1110): copies from it run at full link speed. */
+int iac_pinned_alloc(size_t bytes, void **h_ptr);
+int iac_pinned_free(void *h_ptr);
 /* Kernel timing for measurement: while enabled, the dominant kernel of every
  * call is bracketed by HIP events on the context's stream.  iac_ctx_timing()
  * synchronizes, returns the summed milliseconds and the number of timed

@@ -63,6 +63,8 @@ SIGNATURES: dict[str, tuple] = {
     "iac_memcpy_h2d": (I32, [P, P, P, C.c_size_t]),
     "iac_memcpy_d2h": (I32, [P, P, P, C.c_size_t]),
     "iac_memset_zero": (I32, [P, P, C.c_size_t]),
+    "iac_pinned_alloc": (I32, [C.c_size_t, C.POINTER(P)]),
+    "iac_pinned_free": (I32, [P]),
     "iac_ctx_enable_timing": (I32, [P, I32]),
     "iac_ctx_set_option": (I32, [P, I32, C.c_int64]),
     "iac_ctx_timing": (I32, [P, C.POINTER(C.c_double), C.POINTER(U64), I32]),

@@ -210,6 +210,21 @@ int iac_dev_free(iac_ctx *ctx, void *d_ptr)
     return IAC_OK;
 }
 
+int iac_pinned_alloc(size_t bytes, void **h_ptr)
+{
+    if (!h_ptr || bytes == 0) return fail(IAC_ERR_INVALID_ARG, "null pointer or zero size");
+    *h_ptr = nullptr;
+    IAC_HIP(hipHostMalloc(h_ptr, bytes, hipHostMallocDefault));
+    return IAC_OK;
+}
+
+int iac_pinned_free(void *h_ptr)
+{
+    if (!h_ptr) return IAC_OK;
+    IAC_HIP(hipHostFree(h_ptr));
+    return IAC_OK;
+}
+
 int iac_memcpy_h2d(iac_ctx *ctx, void *d_dst, const void *h_src, size_t bytes)
 {
     IAC_TRY(iac::bind(ctx));

@@ -94,14 +94,14 @@ void AC1D::Calculate_AC_UW_MR(Settings &Options, Detector &RefDet, Settings::Int
     iac_host::DevBuf frames(ctx, (size_t)B * P * 4), rec(ctx, (size_t)B * P * 16), off(ctx, (size_t)(B + 1) * 8);
     iac_host::DevBuf hist(ctx, (size_t)Shape.Size * 8);
     hist.zero();
-    std::vector<float> host((size_t)B * P);
     const float *kmap = RefDet.DeviceKMap(ctx);
     const int *mask = RefDet.DeviceMask(ctx);
-    for (uint32_t b0 = 0; b0 < N; b0 += B) {
-        const uint32_t nb = std::min(B, N - b0);
-        for (uint32_t f = 0; f < nb; f++)
-            Options.LoadFrame(Options.HitEvents[LowerBoundary + b0 + f], host.data() + (size_t)f * P, P);
-        frames.upload(host.data(), (size_t)nb * P * 4);
+    // frames read one stack ahead on a reader thread into page-locked stacks
+    iac_host::FrameStream stream(Options, Options.HitEvents, LowerBoundary, N, P, B);
+    uint32_t nb = 0;
+    while (const float *stack = stream.next(nb)) {
+        frames.upload(stack, (size_t)nb * P * 4);
+        stream.release();
         uint64_t n = 0;
         check(iac_sparse_hit_list(ctx, kmap, (uint32_t)P, frames.as<float>(), nb, mask, Photonisation[0],
                                   Photonisation[1], rec.as<float>(), off.as<uint64_t>(), &n),

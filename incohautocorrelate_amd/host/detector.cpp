@@ -2,6 +2,7 @@
 // frames, and the device passes of the evaluation (averaged intensity,
 // sparsification, AC_uw, C(q) small, rotation merge).
 #include <algorithm>
+#include <chrono>
 #include <cmath>
 #include <cstdlib>
 #include <cstring>
@@ -284,15 +285,21 @@ void Detector::LoadAndAverageIntensity(std::vector<Settings::HitEvent> &Events, 
     const uint32_t B = std::max<uint32_t>(1, std::min<uint32_t>(O.BatchFrames, (uint32_t)((1ull << 30) / (P * 4))));
     iac_host::DevBuf acc(ctx, P * 8), frames(ctx, (size_t)B * P * 4), mi(ctx, (size_t)B * 4), pc(ctx, (size_t)B * 4);
     acc.zero();
-    std::vector<float> host((size_t)B * P), mean(B);
+    std::vector<float> mean(B);
     std::vector<int> count(B);
     const int *d_mask = DeviceMask(ctx);
     float *sum = PhotonSamplingStep > 0 ? nullptr : acc.as<float>();
     int64_t *tally = PhotonSamplingStep > 0 ? acc.as<int64_t>() : nullptr;
-    for (uint32_t b0 = 0; b0 < N; b0 += B) {
-        const uint32_t nb = std::min(B, N - b0);
-        for (uint32_t f = 0; f < nb; f++) O.LoadFrame(Events[LowerBound + b0 + f], host.data() + (size_t)f * P, P);
-        frames.upload(host.data(), (size_t)nb * P * 4);
+    // frames read one stack ahead on a reader thread into page-locked stacks
+    iac_host::FrameStream stream(O, Events, (size_t)LowerBound, N, P, B);
+    const auto t_loop = std::chrono::steady_clock::now();
+    double up_s = 0;
+    uint32_t b0 = 0, nb = 0;
+    while (const float *stack = stream.next(nb)) {
+        const auto t_up = std::chrono::steady_clock::now();
+        frames.upload(stack, (size_t)nb * P * 4);
+        up_s += std::chrono::duration<double>(std::chrono::steady_clock::now() - t_up).count();
+        stream.release();
         check(iac_intensity_accumulate(ctx, frames.as<float>(), nb, (uint32_t)P, d_mask, Threshold,
                                        PhotonSamplingStep, sum, tally, mi.as<float>(), pc.as<int>()),
               "iac_intensity_accumulate in Detector::LoadAndAverageIntensity()");
@@ -303,7 +310,14 @@ void Detector::LoadAndAverageIntensity(std::vector<Settings::HitEvent> &Events, 
             ev.MeanIntensity = mean[f];
             if (PhotonSamplingStep > 0) ev.PhotonCount = count[f];
         }
+        b0 += nb;
     }
+    iac_host::IngestStats &st = iac_host::ingest_stats();
+    st.frames += N;
+    st.read_seconds += stream.read_seconds();
+    st.wait_seconds += stream.wait_seconds();
+    st.upload_seconds += up_s;
+    st.loop_seconds += std::chrono::duration<double>(std::chrono::steady_clock::now() - t_loop).count();
     iac_host::DevBuf avg(ctx, P * 4);
     check(iac_intensity_finish(ctx, (uint32_t)P, N, PhotonSamplingStep, sum, tally, avg.as<float>()),
           "iac_intensity_finish in Detector::LoadAndAverageIntensity()");

@@ -3,11 +3,13 @@
 #pragma once
 
 #include <cstddef>
+#include <cstdint>
 #include <string>
+#include <vector>
 
 #include "iac_hip.h"
+#include "iac_host.hpp"
 
-class Settings;
 
 namespace iac_host {
 
@@ -55,5 +57,42 @@ private:
 
 // exit(EXIT_FAILURE) with the library's message, as Settings::checkErr does
 void check(int rc, const char *where);
+
+// Frames of events [first, first + count) read on a reader thread (Settings::LoadFrame, so a
+// FrameLoader runs on that thread, one call at a time) into two page-locked stacks of up to
+// `stack` frames: while the caller uploads and processes one stack, the next is read.  The
+// stacks come from a process-wide pool of page-locked blocks, kept for reuse until exit.
+class FrameStream {
+public:
+    FrameStream(const Settings &S, const std::vector<Settings::HitEvent> &events, size_t first, size_t count,
+                size_t npix, uint32_t stack);
+    ~FrameStream();
+    FrameStream(const FrameStream &) = delete;
+    FrameStream &operator=(const FrameStream &) = delete;
+    // The next stack in event order (nullptr after the last): nb frames of npix floats,
+    // valid until release() or the next call; rethrows a reader error.
+    const float *next(uint32_t &nb);
+    // The current stack's memory may be refilled (e.g. after its upload).
+    void release();
+    double read_seconds() const;  // reader busy time (file reads)
+    double wait_seconds() const;  // time next() waited for the reader
+    bool pinned() const { return pinned_; }
+
+private:
+    struct Impl;
+    Impl *impl_;
+    bool pinned_ = false;
+    size_t bytes_ = 0;
+};
+
+// Timing of the last evaluation's frame passes (for the driver's report).
+struct IngestStats {
+    uint64_t frames = 0;        // frames streamed (each pass counts)
+    double read_seconds = 0;    // reader thread busy reading
+    double wait_seconds = 0;    // device loop waiting for frames
+    double loop_seconds = 0;    // wall time of the streamed passes
+    double upload_seconds = 0;  // host-to-device copies from the pinned stacks
+};
+IngestStats &ingest_stats();
 
 }  // namespace iac_host

@@ -166,10 +166,14 @@ int main(int argc, char **argv)
             S.HitEvents.push_back(ev);
         }
         RunIAC::IAC_Report R = RunIAC::Run_AutoCorr_DataEval(S, E);
+        const iac_host::IngestStats &st = iac_host::ingest_stats();
         std::printf("{\"dQperVox\": %.9g, \"FinalMeshSize\": %u, \"SmallCqMeshSize\": [%u, %u, %u], "
-                    "\"MeanPhotonCount\": %.9g, \"LowerBound\": %u, \"UpperBound\": %u}\n",
+                    "\"MeanPhotonCount\": %.9g, \"LowerBound\": %u, \"UpperBound\": %u, \"ingest\": "
+                    "{\"frames\": %llu, \"read_s\": %.6f, \"wait_s\": %.6f, \"upload_s\": %.6f, "
+                    "\"loop_s\": %.6f}}\n",
                     R.dQperVox, R.FinalMeshSize, R.SmallCqMeshSize[0], R.SmallCqMeshSize[1], R.SmallCqMeshSize[2],
-                    R.MeanPhotonCount, R.LowerBound, R.UpperBound);
+                    R.MeanPhotonCount, R.LowerBound, R.UpperBound, (unsigned long long)st.frames, st.read_seconds,
+                    st.wait_seconds, st.upload_seconds, st.loop_seconds);
     } catch (const std::exception &ex) {
         std::cerr << "ERROR: " << ex.what() << "\n";
         return 1;

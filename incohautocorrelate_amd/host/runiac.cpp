@@ -5,6 +5,7 @@
 // stack into the resident mesh) -> AC = AC_uw / C(q).  1D: C(q) vector, AC_uw
 // vector, ratio.  Outputs are raw binary arrays as SafeArrayToFile(Binary) writes.
 #include <algorithm>
+#include <chrono>
 #include <cmath>
 #include <fstream>
 #include <iostream>
@@ -69,6 +70,7 @@ IAC_Report Run_AutoCorr_DataEval(Settings &PrgSettings, CreateDataEval_Settings 
 IAC_Report Run_AutoCorr_DataEval(Settings &PrgSettings, CreateDataEval_Settings EvalSettings, Detector &Det)
 {
     IAC_Report Report;
+    iac_host::ingest_stats() = iac_host::IngestStats();
     const int echo = EvalSettings.EchoLevel;
     if (!Det.Checklist.KMap) throw std::runtime_error("Run_AutoCorr_DataEval: detector k-map not calculated");
     if (!Det.Checklist.PixelMask) Det.GenerateFlatOnesPixelMask();
@@ -171,22 +173,28 @@ IAC_Report Run_AutoCorr_DataEval(Settings &PrgSettings, CreateDataEval_Settings 
         1, std::min<uint32_t>(PrgSettings.BatchFrames, (uint32_t)((1ull << 31) / (P * 20))));
     iac_host::DevBuf frames(ctx, (size_t)B * P * 4), rec(ctx, (size_t)B * P * 16), off(ctx, (size_t)(B + 1) * 8),
         rots(ctx, (size_t)B * 36);
-    std::vector<float> host((size_t)B * P), hrot((size_t)B * 9);
+    std::vector<float> hrot((size_t)B * 9);
     const float *kmap = Det.DeviceKMap(ctx);
     const int *mask = Det.DeviceMask(ctx);
     iac_mesh_shape shape = AC_uw.CShape();
     uint64_t *mesh = AC_uw.DeviceMesh();
-    for (uint32_t b0 = 0; b0 < N; b0 += B) {
-        const uint32_t nb = std::min(B, N - b0);
+    // frames read one stack ahead on a reader thread into page-locked stacks
+    iac_host::FrameStream stream(PrgSettings, PrgSettings.HitEvents, lowerBound, N, P, B);
+    const auto t_loop = std::chrono::steady_clock::now();
+    double up_s = 0;
+    uint32_t b0 = 0, nb = 0;
+    while (const float *stack = stream.next(nb)) {
         bool identity = true;
         for (uint32_t f = 0; f < nb; f++) {
             const Settings::HitEvent &ev = PrgSettings.HitEvents[lowerBound + b0 + f];
-            PrgSettings.LoadFrame(ev, host.data() + (size_t)f * P, P);
             std::copy(ev.RotMatrix, ev.RotMatrix + 9, hrot.begin() + 9 * f);
             static const float id[9] = {1, 0, 0, 0, 1, 0, 0, 0, 1};
             identity = identity && std::equal(ev.RotMatrix, ev.RotMatrix + 9, id);
         }
-        frames.upload(host.data(), (size_t)nb * P * 4);
+        const auto t_up = std::chrono::steady_clock::now();
+        frames.upload(stack, (size_t)nb * P * 4);
+        up_s += std::chrono::duration<double>(std::chrono::steady_clock::now() - t_up).count();
+        stream.release();
         if (!identity) rots.upload(hrot.data(), (size_t)nb * 36);
         uint64_t n = 0;
         check(iac_sparse_hit_list(ctx, kmap, (uint32_t)P, frames.as<float>(), nb, mask, EvalSettings.PhotonOffset,
@@ -199,8 +207,15 @@ IAC_Report Run_AutoCorr_DataEval(Settings &PrgSettings, CreateDataEval_Settings 
                                      mesh),
               "iac_ac3d_records_batch in Run_AutoCorr_DataEval()");
         if (echo > 2) std::cout << "AC_uw: " << (b0 + nb) << "/" << N << " events\n";
+        b0 += nb;
     }
     check(iac_ctx_sync(ctx), "iac_ctx_sync");
+    iac_host::IngestStats &st = iac_host::ingest_stats();
+    st.frames += N;
+    st.read_seconds += stream.read_seconds();
+    st.wait_seconds += stream.wait_seconds();
+    st.upload_seconds += up_s;
+    st.loop_seconds += std::chrono::duration<double>(std::chrono::steady_clock::now() - t_loop).count();
     }
     // Report.MeanPhotonCount += PhotonCount per event, then / N (IAC/RunIAC.cpp:528, 553)
     {
