@@ -148,7 +148,7 @@ def main():
         apply_lds_config(ctx, iac, tuned)
         tuned["fixed"] = True
     elif args.kernel != 1 and not args.no_tune:
-        tuned = autotune(ctx, iac, d_kmap, d_pix, d_cnt, data, shape, d_mesh, dev)
+        tuned = autotune(ctx, iac, d_kmap, d_pix, d_cnt, d_off, shape)
 
     # the prepared batch with the tuned settings: plan, chunk table and buffers made once;
     # each step enqueues mesh clear + gather + cell sort + pair pass + mirror with no host
@@ -314,49 +314,24 @@ def cfg_str(cfg):
     return " ".join(f"{k}={cfg.get(k, 0)}" for k in TUNE_KEYS)
 
 
-# (pair mode, box, cell side) points; mode 1: lanes on consecutive pairs,
-# 2: a contiguous pair range per thread
-TUNE_GRID = [dict(mode=m, box=b, cell_vox=c, cell_vox_v=c, frame_chunk=fc)
-             for m, b, c, fc in ((1, 1920, 4, 32), (1, 1920, 5, 32), (1, 1920, 6, 32), (1, 1920, 6, 64),
-                                 (1, 4096, 8, 32), (1, 1024, 4, 32), (1, 1024, 5, 32), (2, 1920, 4, 32))]
-
-
-UNEVEN_GRID = [dict(mode=m, box=b, cell_vox=c, cell_vox_v=c, frame_chunk=32)
-               for m, b, c in ((1, 1920, 4), (1, 1920, 5), (1, 1920, 6), (1, 4096, 8))]
-
-
 # around the defaults (mode 2, 3-voxel cells, 1920-voxel boxes, 2x2-cell regions); frames per
 # block do not matter here (a dense frame is one chunk)
 CQ_GRID = [dict(mode=m, box=b, cell_vox=c, cell_vox_v=c, frame_chunk=32, region=r)
            for m in (1, 2) for b, c in ((1920, 3), (1024, 3), (1920, 4), (4096, 4), (1920, 5)) for r in (2, 4)]
 
 
-def autotune(ctx, iac, d_kmap, d_pix, d_cnt, data, shape, d_mesh, dev):
-    """Pick the LDS kernel's pair mode, cell side (voxels), box and frames per
-    block on a 1,000-frame prefix by measured kernel time."""
-    import torch
-
-    off = data["frame_off"]
-    nf = min(1000, len(off) - 1)
-    n = int(off[nf])
-    d_off = torch.from_numpy(off[: nf + 1].view(np.int64)).to(dev)
-    best = None
-    ctx.enable_timing(True)
-    for cfg in TUNE_GRID:
-        apply_lds_config(ctx, iac, cfg)
-        ctx.ac3d_sparse_batch(d_kmap, d_pix[:n], d_cnt[:n], d_off, n, shape, d_mesh)  # plan
-        ctx.kernel_time(reset=True)
-        for _ in range(2):
-            ctx.ac3d_sparse_batch(d_kmap, d_pix[:n], d_cnt[:n], d_off, n, shape, d_mesh)
-        ms, k = ctx.kernel_time(reset=True)
-        t = ms / max(k, 1)
-        log(f"[tune] {cfg_str(cfg)}: {t:.3f} ms / {nf} frames")
-        if best is None or t < best[0]:
-            best = (t, cfg)
-    ctx.enable_timing(False)
-    apply_lds_config(ctx, iac, best[1])
-    log(f"[tune] chose {cfg_str(best[1])}")
-    return dict(best[1])
+def autotune(ctx, iac, d_kmap, d_pix, d_cnt, d_off, shape, label="tune"):
+    """The library's autotuner (iac_ac3d_tune) on a 1,000-frame prefix: it times the pair
+    kernel of each candidate setting and leaves the fastest applied to the context."""
+    best, cands = ctx.ac3d_tune(d_kmap, d_pix, d_cnt, d_off, shape, prefix_frames=1000, force=True)
+    for c in cands:
+        log(f"[{label}] mode={c['pair_mode']} box={c['box_cap']} cell_vox={c['cell_vox']} "
+            f"frame_chunk={c['frame_chunk']}: {c['ms']:.3f} ms / 1000 frames")
+    log(f"[{label}] library tuner chose mode={best['pair_mode']} box={best['box_cap']} cell_vox={best['cell_vox']} "
+        f"frame_chunk={best['frame_chunk']}")
+    return {"mode": best["pair_mode"], "box": best["box_cap"], "cell_vox": best["cell_vox"],
+            "cell_vox_v": best["cell_vox_v"], "region": best["task_region"], "frame_chunk": best["frame_chunk"],
+            "by": "iac_ac3d_tune (library)", "candidates": cands}
 
 
 def check_headline(ctx, iac, kmap, data, shape, dev, d_mesh):
@@ -1097,22 +1072,7 @@ def uneven_leg(args, iac, W, dev):
     ctx = iac.Context(0)
     tuned = None
     if not args.no_tune:
-        best = None
-        ctx.enable_timing(True)
-        for cfg in UNEVEN_GRID:
-            apply_lds_config(ctx, iac, cfg)
-            b = ctx.ac3d_batch(d_kmap, d_pix, d_cnt, d_off, n_total, shape)
-            b.launch(d_mesh, True)
-            ctx.kernel_time(reset=True)
-            b.launch(d_mesh, True)
-            ms, k = ctx.kernel_time(reset=True)
-            b.destroy()
-            log(f"[uneven tune] {cfg_str(cfg)}: {ms:.2f} ms")
-            if best is None or ms < best[0]:
-                best = (ms, cfg)
-        ctx.enable_timing(False)
-        apply_lds_config(ctx, iac, best[1])
-        tuned = dict(best[1])
+        tuned = autotune(ctx, iac, d_kmap, d_pix, d_cnt, d_off, shape, label="uneven tune")
     b = ctx.ac3d_batch(d_kmap, d_pix, d_cnt, d_off, n_total, shape)
     b.capture(d_mesh, True)
     b.launch(d_mesh, True)

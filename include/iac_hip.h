@@ -178,6 +178,26 @@ int iac_ac3d_batch_launch(iac_ctx *ctx, iac_ac3d_batch *batch, uint64_t *d_mesh,
 int iac_ac3d_batch_capture(iac_ctx *ctx, iac_ac3d_batch *batch, uint64_t *d_mesh, int zero_mesh);
 int iac_ac3d_batch_destroy(iac_ac3d_batch *batch);
 
+/* The engine's autotuner: sweeps the LDS cell-box engine's settings (pair walk, cell side,
+ * box capacity, frames per block) on the first prefix_frames frames of a batch (0: 1,000)
+ * with the given binning modes, timing the pair kernel of each with HIP events (best of two
+ * runs after a planning run, into a scratch mesh the call allocates and frees), and makes
+ * the fastest the context's options (IAC_OPT_PAIR_MODE, IAC_OPT_CELL_VOX(_V), IAC_OPT_BOX_CAP,
+ * IAC_OPT_FRAME_CHUNK, IAC_OPT_TASK_REGION), which every later engine call of the context
+ * uses.  The winner is cached in the context per (k-map, mesh shape, modes): a second call
+ * with the same key applies it without timing (force_retune = 0).  *best gets the winner;
+ * up to max_all candidate results go to all (may be NULL), their count to *n_all.  Pending
+ * kernel timings of the context (iac_ctx_timing) are discarded.  No counterpart in the
+ * reference (it has no tunable engine). */
+typedef struct {
+    int32_t pair_mode, cell_vox, cell_vox_v, box_cap, task_region, frame_chunk;
+    float ms; /* pair-kernel milliseconds on the prefix (-1: taken from the cache) */
+} iac_tune_result;
+int iac_ac3d_tune(iac_ctx *ctx, const float *d_kmap, uint32_t npix, const uint32_t *d_pix, const uint16_t *d_cnt,
+                  const uint64_t *d_frame_off, uint32_t nframes, const iac_mesh_shape *shape, int interp1,
+                  int interp2, int double_map, uint32_t prefix_frames, int force_retune, iac_tune_result *best,
+                  iac_tune_result *all, uint32_t max_all, uint32_t *n_all);
+
 /* A frame queue: the per-event call of the reference's frame loop,
  * Detector::AutoCorrelateSparseList once per event (IAC/RunIAC.cpp:524-550,
  * IAC/Detector.cpp:1010-1181), at the speed of the batched pass.  Each push appends one

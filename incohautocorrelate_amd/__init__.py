@@ -12,7 +12,7 @@ import weakref
 
 import numpy as np
 
-from ._abi import IACError, MeshShape, Panel, check, fptr, lib
+from ._abi import IACError, MeshShape, Panel, TuneResult, check, fptr, lib
 
 __all__ = [
     "IACError", "MeshShape", "Panel", "lib", "generate_pixel_map", "calc_kmap", "mesh_shape_big",
@@ -301,6 +301,19 @@ class Context:
         b = Ac3dBatch(self, h, (kmap, pix, cnt, frame_off))
         self._plans.add(b)
         return b
+
+    def ac3d_tune(self, kmap, pix, cnt, frame_off, shape: MeshShape, interp1: int = NEAREST, interp2: int = NEAREST,
+                  double_map: bool = True, prefix_frames: int = 1000, force: bool = False) -> tuple[dict, list]:
+        """The library's autotuner on a frame prefix; applies the winner to this context.
+        Returns (winner, every candidate) as dicts (ms = -1 for a cached winner)."""
+        best = TuneResult()
+        allr = (TuneResult * 32)()
+        n = C.c_uint32()
+        check(lib().iac_ac3d_tune(self._h, _ptr(kmap), kmap.numel() // 3, _ptr(pix), _ptr(cnt), _ptr(frame_off),
+                                  frame_off.numel() - 1, C.byref(shape), interp1, interp2, int(double_map),
+                                  int(prefix_frames), int(force), C.byref(best), allr, 32, C.byref(n)),
+              "iac_ac3d_tune")
+        return best.as_dict(), [allr[i].as_dict() for i in range(n.value)]
 
     def ac3d_queue(self, kmap, shape: MeshShape, mesh, interp1: int = NEAREST, interp2: int = NEAREST,
                    double_map: bool = True, max_records: int = 0, max_frames: int = 0) -> "Ac3dQueue":

@@ -34,6 +34,17 @@ class MeshShape(C.Structure):
         }
 
 
+class TuneResult(C.Structure):
+    """iac_tune_result: one setting of the pair engine and its pair-kernel time."""
+
+    _fields_ = [("pair_mode", C.c_int32), ("cell_vox", C.c_int32), ("cell_vox_v", C.c_int32),
+                ("box_cap", C.c_int32), ("task_region", C.c_int32), ("frame_chunk", C.c_int32),
+                ("ms", C.c_float)]
+
+    def as_dict(self) -> dict:
+        return {k: getattr(self, k) for k, _ in self._fields_}
+
+
 class Panel(C.Structure):
     """iac_panel: PPP::DetectorPanel (IncohAutoCorrelate/PPP.h:14-18)."""
 
@@ -103,6 +114,8 @@ SIGNATURES: dict[str, tuple] = {
     "iac_ac3d_batch_launch": (I32, [P, P, P, I32]),
     "iac_ac3d_batch_capture": (I32, [P, P, P, I32]),
     "iac_ac3d_batch_destroy": (I32, [P]),
+    "iac_ac3d_tune": (I32, [P, P, U32, P, P, P, U32, C.POINTER(MeshShape), I32, I32, I32, U32, I32,
+                            C.POINTER(TuneResult), C.POINTER(TuneResult), U32, C.POINTER(U32)]),
     "iac_ac3d_queue_create": (I32, [P, P, U32, C.POINTER(MeshShape), I32, I32, I32, P, U64, U32, C.POINTER(P)]),
     "iac_ac3d_queue_push": (I32, [P, P, U32, F3]),
     "iac_ac3d_queue_push_device": (I32, [P, P, U32, F3]),

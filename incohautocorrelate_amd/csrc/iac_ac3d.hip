@@ -546,6 +546,118 @@ int iac_ac3d_sparse(iac_ctx *ctx, const float *d_hitlist4, uint32_t n, const flo
                         nullptr, R, P, d_mesh);
 }
 
+// ---- autotuner ---------------------------------------------------------------------
+
+}  // extern "C"
+
+namespace {
+
+struct TunePoint {
+    int pair_mode, cell_vox, box_cap, frame_chunk;
+};
+// Candidates around the measured defaults: nearest-neighbour binning (one LDS add per pair,
+// symmetric: unordered pairs and a mirror pass) and linear binning (eight corner adds per
+// ordered pair, 4096-voxel boxes); walks 1 = lanes on consecutive pairs, 2 = a pair range
+// per thread.
+const TunePoint kTuneNearest[] = {{1, 6, 1920, 32}, {1, 5, 1920, 32}, {1, 4, 1920, 32}, {1, 6, 1920, 64},
+                                  {1, 8, 4096, 32}, {1, 4, 1024, 32}, {2, 4, 1920, 32}};
+const TunePoint kTuneLinear[] = {{1, 5, 4096, 32}, {1, 4, 4096, 32}, {1, 6, 4096, 32},
+                                 {1, 4, 1920, 32}, {2, 4, 4096, 32}, {1, 5, 4096, 64}};
+
+void apply_point(iac_ctx *ctx, const iac_tune_result &r)
+{
+    ctx->pair_mode = r.pair_mode;
+    ctx->cell_vox = r.cell_vox;
+    ctx->cell_vox_v = r.cell_vox_v;
+    ctx->box_cap = r.box_cap;
+    ctx->task_region = r.task_region;
+    ctx->frame_chunk = (uint32_t)r.frame_chunk;
+}
+
+bool same_shape(const iac_mesh_shape &a, const iac_mesh_shape &b) { return std::memcmp(&a, &b, sizeof(a)) == 0; }
+
+}  // namespace
+
+extern "C" {
+
+int iac_ac3d_tune(iac_ctx *ctx, const float *d_kmap, uint32_t npix, const uint32_t *d_pix, const uint16_t *d_cnt,
+                  const uint64_t *d_frame_off, uint32_t nframes, const iac_mesh_shape *shape, int interp1,
+                  int interp2, int double_map, uint32_t prefix_frames, int force_retune, iac_tune_result *best,
+                  iac_tune_result *all, uint32_t max_all, uint32_t *n_all)
+{
+    IAC_TRY(iac::bind(ctx));
+    if (!d_kmap || npix == 0 || !d_pix || !d_cnt || !d_frame_off || !best)
+        return iac::fail(IAC_ERR_INVALID_ARG, "null tuner argument");
+    if (nframes == 0) return iac::fail(IAC_ERR_INVALID_ARG, "the tuner needs frames");
+    iac::BinParams P;
+    IAC_TRY(make_params(shape, interp1, interp2, double_map, &P));
+    if (n_all) *n_all = 0;
+    for (const auto &e : ctx->tune_cache)
+        if (!force_retune && e.kmap == d_kmap && e.npix == npix && same_shape(e.shape, *shape) &&
+            e.interp1 == interp1 && e.interp2 == interp2 && e.double_map == (double_map ? 1 : 0)) {
+            *best = e.best;
+            best->ms = -1.0f;
+            apply_point(ctx, e.best);
+            return IAC_OK;
+        }
+    const uint32_t nf = std::min(nframes, prefix_frames ? prefix_frames : 1000u);
+    uint64_t n = 0;
+    IAC_HIP(hipMemcpyAsync(&n, d_frame_off + nf, sizeof(n), hipMemcpyDeviceToHost, ctx->stream));
+    IAC_HIP(hipStreamSynchronize(ctx->stream));
+    if (n == 0) return iac::fail(IAC_ERR_PRECONDITION, "no photons in the tuning prefix");
+    if (n >= (1ull << 32)) return iac::fail(IAC_ERR_LIMIT, "tuning prefix above 2^32 - 1 photons");
+    const bool linear = !symmetric_mode(engine_params(P));
+    const TunePoint *pts = linear ? kTuneLinear : kTuneNearest;
+    const size_t npts = linear ? sizeof(kTuneLinear) / sizeof(kTuneLinear[0])
+                               : sizeof(kTuneNearest) / sizeof(kTuneNearest[0]);
+    const size_t nvox = (size_t)P.sab * P.sab * P.sc;
+    uint64_t *mesh = nullptr;
+    IAC_HIP(hipMalloc(&mesh, nvox * sizeof(uint64_t)));
+    // the caller's timing switch and kernel choice are restored on every exit
+    const bool timing = ctx->timing;
+    const int kernel = ctx->ac3d_kernel;
+    ctx->ac3d_kernel = 2;
+    int rc = IAC_OK;
+    iac_tune_result win{};
+    win.ms = -1.0f;
+    uint32_t count = 0;
+    for (size_t k = 0; k < npts && rc == IAC_OK; k++) {
+        iac_tune_result r{pts[k].pair_mode, pts[k].cell_vox, pts[k].cell_vox, pts[k].box_cap, 0, pts[k].frame_chunk,
+                          0.0f};
+        apply_point(ctx, r);
+        float t_best = -1.0f;
+        for (int rep = 0; rep < 3 && rc == IAC_OK; rep++) {  // the first run plans
+            ctx->timing = rep > 0;
+            ctx->ev_used = 0;
+            rc = iac_ac3d_sparse_batch(ctx, d_kmap, npix, d_pix, d_cnt, d_frame_off, nf, n, nullptr, shape, interp1,
+                                       interp2, double_map, mesh);
+            if (rc == IAC_OK && rep > 0) {
+                double ms = 0;
+                uint64_t launches = 0;
+                rc = iac_ctx_timing(ctx, &ms, &launches, 1);
+                if (rc == IAC_OK && (t_best < 0 || ms < t_best)) t_best = (float)ms;
+            }
+        }
+        if (rc != IAC_OK) break;
+        r.ms = t_best;
+        if (all && count < max_all) all[count] = r;
+        count++;
+        if (win.ms < 0 || r.ms < win.ms) win = r;
+    }
+    ctx->timing = timing;
+    ctx->ev_used = 0;  // pending kernel timings of the caller are discarded
+    ctx->ac3d_kernel = kernel;
+    (void)hipStreamSynchronize(ctx->stream);
+    (void)hipFree(mesh);
+    if (rc != IAC_OK) return rc;
+    if (n_all) *n_all = std::min(count, max_all);
+    apply_point(ctx, win);
+    *best = win;
+    iac_ctx::TuneEntry e{d_kmap, npix, *shape, interp1, interp2, double_map ? 1 : 0, win};
+    ctx->tune_cache.push_back(e);
+    return IAC_OK;
+}
+
 // ---- frame queue: the per-event loop at batch speed --------------------------------
 
 }  // extern "C"

@@ -37,6 +37,15 @@ struct iac_ctx {
     uint64_t sim_chunk_rows = 0;  // simulator: rows (pixel x subpixel) per GEMM launch with subpixels or modes
     int box_cap = 0;        // pair engine: LDS box voxels, 1024, 1920 or 4096 (0 = per kind)
     int pair_prefetch = 0;  // pair engine: pairs per load batch (1, 4 or 8; 0 = 1)
+    // autotuner winners per (k-map, mesh shape, binning modes)
+    struct TuneEntry {
+        const float *kmap;
+        uint32_t npix;
+        iac_mesh_shape shape;
+        int interp1, interp2, double_map;
+        iac_tune_result best;
+    };
+    std::vector<TuneEntry> tune_cache;
 };
 
 namespace iac {

@@ -247,3 +247,42 @@ def test_nn_linear_double_map_symmetric(ctx, dev, mode):
     finally:
         ctx.set_option(iac.OPT_AC3D_KERNEL, 0)
         ctx.set_option(iac.OPT_PAIR_MODE, 0)
+
+
+@pytest.mark.parametrize("interp1,interp2", [(0, 0), (1, 1)])
+def test_library_tuner(ctx, dev, interp1, interp2):
+    """iac_ac3d_tune times every candidate on a frame prefix, applies the fastest, caches it
+    per (k-map, mesh, modes); every setting it tries gives the oracle's mesh (the tuner is
+    also how the benchmark picks its settings)."""
+    import torch
+
+    import incohautocorrelate_amd as iac
+
+    n = 128
+    _, km, mq = W.detector(n)
+    shape = iac.mesh_shape_big(mq, 63)
+    data = W.lognormal_photon_frames(n * n, 60, 80, 1.0, 2, 800, seed=23)
+    d_km = to_dev(km, dev)
+    pix, cnt, off = _dev_frames(data, dev)
+    c = iac.Context(0)
+    try:
+        best, cands = c.ac3d_tune(d_km, pix, cnt, off, shape, interp1, interp2, True, prefix_frames=40)
+        assert len(cands) >= 5 and all(x["ms"] > 0 for x in cands)
+        assert best["ms"] == min(x["ms"] for x in cands)
+        if interp1 == 1:  # linear binning: the 4096-voxel boxes lead the candidates
+            assert cands[0]["box_cap"] == 4096
+        again, none = c.ac3d_tune(d_km, pix, cnt, off, shape, interp1, interp2, True, prefix_frames=40)
+        assert none == [] and again["ms"] == -1.0
+        assert {k: v for k, v in again.items() if k != "ms"} == {k: v for k, v in best.items() if k != "ms"}
+        ref, _ = O.ac3d_frames(km, data["pix"], data["cnt"], data["frame_off"], shape, None, interp1, interp2, True)
+        for x in cands:  # each candidate setting, applied by hand, is bit-exact
+            c.set_option(iac.OPT_PAIR_MODE, x["pair_mode"])
+            c.set_option(iac.OPT_CELL_VOX, x["cell_vox"])
+            c.set_option(iac.OPT_BOX_CAP, x["box_cap"])
+            c.set_option(iac.OPT_FRAME_CHUNK, x["frame_chunk"])
+            m = torch.zeros(shape.size_ab ** 2 * shape.size_c, dtype=torch.int64, device=dev)
+            c.ac3d_sparse_batch(d_km, pix, cnt, off, int(data["frame_off"][-1]), shape, m,
+                                interp1=interp1, interp2=interp2, double_map=True)
+            assert np.array_equal(mesh_np(m), ref), x
+    finally:
+        c.close()
