@@ -191,3 +191,33 @@ def test_ac1d_cq_and_acuw_with_drops(ctx, dev, interp):
     else:
         assert np.allclose(got2, ref2, rtol=1e-12, atol=0)
     assert ctx.dropped() - before == dref2
+
+
+@pytest.mark.parametrize("interp", [0, 1])
+def test_ac1d_acuw_sparse_1024_4096_bins(ctx, dev, interp):
+    """AC1D AC_uw at the sparse benchmark's shape: 1024^2 detector, Poisson(0.002) photon
+    frames (~2,100 photons each), 4096 bins; the first 16 frames against the oracle's CPU
+    branch (AC1D.cpp:256-293): bit for bit with nearest bins (integer weights, exact in
+    double), to the double rounding of the add order with linear bins."""
+    import torch
+
+    import incohautocorrelate_amd as iac
+
+    n, size, nf = 1024, 4096, 16
+    _, km, mq = W.detector(n)
+    max_q, dq = iac.ac1d_shape(mq, size)
+    d = W.poisson_photon_frames(n * n, nf, 0.002, seed=2)
+    off = d["frame_off"]
+    ref = np.zeros(size)
+    for f in range(nf):
+        hl = O.hit_list_from_photons(km, d["pix"][off[f]:off[f + 1]], d["cnt"][off[f]:off[f + 1]])
+        ref, _ = O.ac1d_acuw(hl, size, max_q, dq, interp, 2, ac=ref)
+    h = torch.zeros(size, dtype=torch.float64, device=dev)
+    ctx.ac1d_acuw(to_dev(km, dev), to_dev(d["pix"], dev), to_dev(d["cnt"], dev), to_dev(off, dev),
+                  int(off[-1]), size, max_q, dq, interp, h)
+    got = h.cpu().numpy()
+    assert ref.sum() > 0 and np.count_nonzero(ref) > 1000
+    if interp == 0:
+        assert np.array_equal(got, ref)
+    else:
+        assert np.allclose(got, ref, rtol=1e-12, atol=1e-9)
