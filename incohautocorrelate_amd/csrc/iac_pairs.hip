@@ -565,25 +565,41 @@ __device__ __forceinline__ void tri_decode(uint32_t r, uint32_t n, uint32_t &ia,
 // Phase 2 of a group on the specialised path with lanes on consecutive pairs:
 // wave w walks the pair range [w0, w1) 64 at a time, lane l on pair P + l, so the
 // records a wave loads lie in the few rows those 64 pairs span rather than in
-// 64 unrelated rows (one or two cache lines per load instead of up to 64).  The
-// group's items are compacted (every item has pairs, so item starts are
-// distinct): the next 64 starts mark their lanes in a per-wave LDS flag row, and
-// a lane's item is the number of marked lanes up to it (ballot + mbcnt).  A
-// non-triangle item carries 1/nb (correctly rounded) in place of na, so (ia, ib)
-// is a float-reciprocal division with one correction step while the group's
-// pair total is below 2^24.
-// a volatile LDS byte: cross-lane flags stay LDS accesses, in program order
-typedef volatile __attribute__((address_space(3))) uint8_t lds_flag_t;
+// 64 unrelated rows (one or two cache lines per load instead of up to 64).
+//
+// Item lookup: the group's items are compacted (every item has pairs, so item
+// starts are distinct) and followed by 64 sentinel starts (~0).  The starts in
+// [P, P + 64) tag their positions with P in a per-wave LDS row; a lane's item is
+// the item before the step plus the number of tagged positions up to the lane
+// (ballot + mbcnt, the count's base and the next step's item in scalar registers).
+//
+// Division: an item of a non-triangle group of at most 2^22 pairs carries
+// h = RN(1 / (2 nb)) in place of na, and ia = trunc(f32(2r + 1) * h) is exactly
+// r / nb for every r < 2^22: (r + 1/2) / nb lies at least 1 / (2 nb) from the
+// integers, and the two roundings move it by at most (r + 1/2) / nb * 2^-23.
+//
+// Records are read at 32-bit byte offsets from the chunk's first record, one step
+// ahead of the binning.  Lanes past the range read a real pair (their position
+// clamped to the group's last pair) and do not bin.
+// a volatile LDS word: cross-lane tags stay LDS accesses, in program order
+typedef volatile __attribute__((address_space(3))) uint32_t lds_tag_t;
 
 // The binning: FASTBIN, the AC_uw specialised inline path (SMALL and INBOX apply
 // there); otherwise bin_ordered for ordered runs and bin_sym for symmetric ones.
-template <int KIND, bool ORD, bool FASTBIN, bool SMALL, bool INBOX>
+// EXACT: the group divides by h (non-triangle, at most 2^22 pairs); otherwise the
+// triangle decode or an integer division.
+template <int KIND, bool ORD, bool FASTBIN, bool SMALL, bool INBOX, bool EXACT>
 __device__ __forceinline__ void fast_walk(const float4 *__restrict__ src, const uint4 *item, const uint32_t *pre,
-                                          uint32_t nnz, uint32_t w0, uint32_t w1, bool tri, bool big,
+                                          uint32_t u0, uint32_t w0, uint32_t w1, uint32_t total, bool tri,
                                           const PairSpec &S, unsigned long long *box, const Task &T, BoxSink &sink,
-                                          lds_flag_t *fl)
+                                          lds_tag_t *fl)
 {
     const uint32_t lane = threadIdx.x & 63u;
+    // wave-uniform values in scalar registers
+    w0 = __builtin_amdgcn_readfirstlane(w0);
+    w1 = __builtin_amdgcn_readfirstlane(w1);
+    const uint32_t last = __builtin_amdgcn_readfirstlane(total) - 1u;
+    uint32_t ub = __builtin_amdgcn_readfirstlane(u0);  // the item before the step's first start
     const int fox = S.P.c0 - T.bx0, foy = S.P.c1 - T.by0, foz = S.P.c2 - T.bz0;
     const int bx8 = 8 * T.bx, bxby8 = 8 * T.bx * T.by;
     const int c8 = 8 * fox + bx8 * foy + bxby8 * foz;
@@ -592,50 +608,65 @@ __device__ __forceinline__ void fast_walk(const float4 *__restrict__ src, const 
     const int cst1 = S.ka == 1 ? 8 : (S.kb == 1 ? bx8 : bxby8);
     const int cst2 = S.ka == 2 ? 8 : (S.kb == 2 ? bx8 : bxby8);
     const float vox = S.P.vox, cut_sq = S.P.cut_sq;
+    const char *const srcb = reinterpret_cast<const char *>(src);
     char *const boxb = reinterpret_cast<char *>(box);
-    uint32_t lo = 0, hi = nnz;  // the item holding w0: largest u with pre[u] <= w0
-    while (hi - lo > 1) {
-        const uint32_t mid = (lo + hi) >> 1;
-        if (pre[mid] <= w0) lo = mid; else hi = mid;
-    }
-    uint32_t ub = lo;  // the item holding P (its end is at or after P)
+
+    fl[lane] = ~0u;  // untagged: a tag is a pair index, below 2^32 - 1
+    // the item of pair P + lane; ub moves to the item of pair P + 63
     auto item_at = [&](uint32_t P) -> uint32_t {
-        const uint32_t si = ub + 1 + lane;
-        const uint32_t t = si < nnz ? pre[si] - P : 64u;
-        fl[lane] = 0u;
-        if (t < 64u) fl[t] = 1u;
-        const uint64_t starts = __ballot(fl[lane] != 0u);
-        const uint32_t below = __builtin_amdgcn_mbcnt_hi((uint32_t)(starts >> 32),
-                                                         __builtin_amdgcn_mbcnt_lo((uint32_t)starts, 0u));
-        return ub + below + (uint32_t)((starts >> lane) & 1ull);
+        const uint32_t t = pre[ub + 1u + lane] - P;
+        if (t < 64u) fl[t] = P;
+        const uint64_t starts = __ballot(fl[lane] == P);
+        const uint32_t base = ub + (uint32_t)(starts & 1ull);
+        ub += (uint32_t)__popcll(starts);
+        return __builtin_amdgcn_mbcnt_hi((uint32_t)(starts >> 33),
+                                         __builtin_amdgcn_mbcnt_lo((uint32_t)(starts >> 1), base));
     };
-    uint32_t u = item_at(w0);
-    for (uint32_t P = w0; P < w1; P += 64) {
-        const bool active = P + lane < w1;
-        const uint32_t p = active ? P + lane : w1 - 1;
+    // byte offsets of the two records of pair min(P + lane, last), which item u holds
+    auto decode = [&](uint32_t u, uint32_t P, uint32_t &oa, uint32_t &ob) {
         const uint4 d = item[u];
-        // an inactive lane's item belongs to its unclamped position, not to p: its
-        // offset is meaningless, so it decodes pair 0 (and loads nothing)
-        const uint32_t r = active ? p - pre[u] : 0u;
+        const uint32_t r = min(P + lane, last) - pre[u];
         uint32_t ia, ib;
-        if (tri) {
+        if (EXACT) {
+            ia = (uint32_t)((float)(2u * r + 1u) * __uint_as_float(d.y));
+            ib = r - __umul24(ia, d.w);
+        } else if (tri) {
             tri_decode(r, d.y, ia, ib);
-        } else if (!big) {
-            divmod24(r, d.w, __uint_as_float(d.y), ia, ib);
         } else {
             ia = r / d.w;
             ib = r - ia * d.w;
         }
-        ub = __builtin_amdgcn_readlane(u, 63);
-        float4 ca = make_float4(0.f, 0.f, 0.f, 0.f), cb = ca;  // the pair's records (active lanes)
-        if (active) {
-            ca = src[d.x + ia];
-            cb = src[d.z + ib];
+        oa = (d.x + ia) << 4;
+        ob = (d.z + ib) << 4;
+    };
+    auto load = [&](uint32_t o) { return *reinterpret_cast<const float4 *>(srcb + o); };
+
+    auto bin = [&](const float4 &ca, const float4 &cb, bool active) {
+        if (FASTBIN) {
+            asm volatile("" ::"v"(cb.w), "v"(ca.w));  // 16-byte loads
+            const float q0 = ca.x - cb.x, q1 = ca.y - cb.y, q2 = ca.z - cb.z;
+            const float sq = (q0 * q0 + q1 * q1) + q2 * q2;
+            if (active && sq < cut_sq) {
+                const int dx = round_away(q0 * vox);
+                const int dy = round_away(q1 * vox);
+                const int dz = round_away(q2 * vox);
+                const uint32_t v = SMALL ? (uint32_t)((ca.w * 1000.0f) * cb.w) : pair_val_small(ca.w * cb.w);
+                if (INBOX) {
+                    const int o = __mul24(dz, bxby8) + (__mul24(dy, bx8) + (dx * 8 + c8));
+                    atomicAdd(reinterpret_cast<unsigned long long *>(boxb + o), (unsigned long long)v);
+                } else {
+                    const uint32_t lx = (uint32_t)(dx + fox), ly = (uint32_t)(dy + foy), lz = (uint32_t)(dz + foz);
+                    if (lx < (uint32_t)T.bx && ly < (uint32_t)T.by && lz < (uint32_t)T.bz)
+                        atomicAdd(box + lx + __umul24(ly, (uint32_t)T.bx) + __umul24(lz, (uint32_t)(T.bx * T.by)),
+                                  (unsigned long long)v);
+                    else
+                        sink.add(dx + S.P.c0, dy + S.P.c1, dz + S.P.c2, v);
+                }
+            }
+            return;
         }
-        // the next step's item (flag row, ballot) while this step's records load: its LDS
-        // round trips overlap the global-load latency instead of following it
-        if (P + 64 < w1) u = item_at(P + 64);
-        if (active && ORD && KIND == kKindAc3d && INBOX) {
+        if (!active) return;
+        if (ORD && KIND == kKindAc3d && INBOX) {
             // a linear first level (identity) on a task whose box holds every corner: add_q_entry's
             // eight corners at one base byte offset plus constant strides, their values
             // ((v wx) wy) wz from a product tree in the same order
@@ -664,10 +695,10 @@ __device__ __forceinline__ void fast_walk(const float4 *__restrict__ src, const 
                 };
                 if (S.P.interp2) corners(std::true_type{}); else corners(std::false_type{});
             }
-        } else if (active && ORD) {
+        } else if (ORD) {
             // ordered runs (linear second level): every corner through the checked box
             bin_ordered<KIND>(S, ca, cb, sink);
-        } else if (active && KIND == kKindCq && INBOX) {
+        } else if (KIND == kKindCq && INBOX) {
             // C(q) of a task whose box holds every voxel its pairs reach: the bin of
             // bin_sym (its value and axis order: round(q[ka] vox) + c0 is round_away
             // of the same product plus c0) as one byte offset, component i scaled by
@@ -682,82 +713,101 @@ __device__ __forceinline__ void fast_walk(const float4 *__restrict__ src, const 
                 atomicAdd(reinterpret_cast<unsigned long long *>(boxb + o),
                           (unsigned long long)(uint64_t)((double)w * S.mult));
             }
-        } else if (active && KIND == kKindCq) {
-            // C(q): its own value and axis order, through the checked box
-            int fs, ms, ss;
-            uint64_t v;
-            if (bin_sym<kKindCq>(S, ca, cb, fs, ms, ss, v)) sink.add(fs, ms, ss, v);
-        } else if (active && !FASTBIN) {
+        } else {
+            // C(q) through the checked box, or the general symmetric binning
             int fs, ms, ss;
             uint64_t v;
             if (bin_sym<KIND>(S, ca, cb, fs, ms, ss, v)) sink.add(fs, ms, ss, v);
-        } else if (active) {
-            asm volatile("" ::"v"(cb.w), "v"(ca.w));  // 16-byte loads
-            const float q0 = ca.x - cb.x, q1 = ca.y - cb.y, q2 = ca.z - cb.z;
-            const float sq = (q0 * q0 + q1 * q1) + q2 * q2;
-            if (sq < cut_sq) {
-                const int dx = round_away(q0 * vox);
-                const int dy = round_away(q1 * vox);
-                const int dz = round_away(q2 * vox);
-                const uint32_t v = SMALL ? (uint32_t)((ca.w * 1000.0f) * cb.w) : pair_val_small(ca.w * cb.w);
-                if (INBOX) {
-                    const int o = __mul24(dz, bxby8) + (__mul24(dy, bx8) + (dx * 8 + c8));
-                    atomicAdd(reinterpret_cast<unsigned long long *>(boxb + o), (unsigned long long)v);
-                } else {
-                    const uint32_t lx = (uint32_t)(dx + fox), ly = (uint32_t)(dy + foy), lz = (uint32_t)(dz + foz);
-                    if (lx < (uint32_t)T.bx && ly < (uint32_t)T.by && lz < (uint32_t)T.bz)
-                        atomicAdd(box + lx + __umul24(ly, (uint32_t)T.bx) + __umul24(lz, (uint32_t)(T.bx * T.by)),
-                                  (unsigned long long)v);
-                    else
-                        sink.add(dx + S.P.c0, dy + S.P.c1, dz + S.P.c2, v);
-                }
-            }
         }
+    };
+
+    // two steps per iteration, the next step's records in flight while one bins.  The
+    // look-ahead is unconditional (past the range it reads the group's last pair, or the
+    // next wave's), and so is the second step (past the range no lane bins): a load
+    // behind a branch, or a second exit, would make the binning wait for the loads in
+    // flight too.  The scheduling barriers keep the binning's first use of its records
+    // after the issue of the next step's loads.
+    auto lanes = [&](uint32_t P) { return w1 > P ? w1 - P : 0u; };  // lanes of step P in range
+    uint32_t oa, ob;
+    decode(item_at(w0), w0, oa, ob);
+    float4 a0 = load(oa), b0 = load(ob);
+    for (uint32_t P = w0;; P += 128u) {
+        decode(item_at(P + 64u), P + 64u, oa, ob);
+        const float4 a1 = load(oa), b1 = load(ob);
+        __builtin_amdgcn_sched_barrier(0);
+        bin(a0, b0, lane < lanes(P));
+        decode(item_at(P + 128u), P + 128u, oa, ob);
+        a0 = load(oa);
+        b0 = load(ob);
+        __builtin_amdgcn_sched_barrier(0);
+        bin(a1, b1, lane < lanes(P + 64u));
+        if (P + 128u >= w1) break;
     }
 }
 
 template <int KIND, bool ORD, bool FAST>
-__device__ __forceinline__ void fast_walk_w(bool small, bool inbox, const float4 *__restrict__ src, const uint4 *item,
-                                            const uint32_t *pre, uint32_t nnz, uint32_t w0, uint32_t w1, bool tri,
-                                            bool big, const PairSpec &S, unsigned long long *box, const Task &T,
-                                            BoxSink &sink, lds_flag_t *fl)
+__device__ __forceinline__ void fast_walk_w(bool small, bool inbox, bool exact, const float4 *__restrict__ src,
+                                            const uint4 *item, const uint32_t *pre, uint32_t u0, uint32_t w0,
+                                            uint32_t w1, uint32_t total, bool tri, const PairSpec &S,
+                                            unsigned long long *box, const Task &T, BoxSink &sink, lds_tag_t *fl)
 {
+    // the triangle task and groups of more than 2^22 pairs: one general instance (the
+    // checked box and the general value), so the rare decodes add one walk loop, not four
+    if (!exact) {
+        fast_walk<KIND, ORD, FAST, false, false, false>(src, item, pre, u0, w0, w1, total, tri, S, box, T, sink, fl);
+        return;
+    }
+    auto go = [&](auto fastbin_c, auto small_c, auto inbox_c) {
+        constexpr bool kFastbin = decltype(fastbin_c)::value, kSmall = decltype(small_c)::value;
+        constexpr bool kInbox = decltype(inbox_c)::value;
+        fast_walk<KIND, ORD, kFastbin, kSmall, kInbox, true>(src, item, pre, u0, w0, w1, total, tri, S, box, T, sink,
+                                                             fl);
+    };
+    using T_ = std::true_type;
+    using F_ = std::false_type;
     if (!FAST) {
         if (((KIND == kKindCq && !ORD && S.interp == 0) || (KIND == kKindAc3d && ORD && lin_first_identity(S.P))) &&
             inbox)
-            fast_walk<KIND, ORD, false, false, true>(src, item, pre, nnz, w0, w1, tri, big, S, box, T, sink, fl);
+            go(F_{}, F_{}, T_{});
         else
-            fast_walk<KIND, ORD, false, false, false>(src, item, pre, nnz, w0, w1, tri, big, S, box, T, sink, fl);
+            go(F_{}, F_{}, F_{});
     } else if (inbox) {
-        if (small) fast_walk<KIND, ORD, true, true, true>(src, item, pre, nnz, w0, w1, tri, big, S, box, T, sink, fl);
-        else fast_walk<KIND, ORD, true, false, true>(src, item, pre, nnz, w0, w1, tri, big, S, box, T, sink, fl);
+        if (small) go(T_{}, T_{}, T_{});
+        else go(T_{}, F_{}, T_{});
     } else {
-        if (small) fast_walk<KIND, ORD, true, true, false>(src, item, pre, nnz, w0, w1, tri, big, S, box, T, sink, fl);
-        else fast_walk<KIND, ORD, true, false, false>(src, item, pre, nnz, w0, w1, tri, big, S, box, T, sink, fl);
+        if (small) go(T_{}, T_{}, F_{});
+        else go(T_{}, F_{}, F_{});
     }
 }
 
+// PF: 0 lanes on consecutive pairs (the walk), 1 a pair range per thread, pair by
+// pair, 4 or 8 a range per thread in batches of PF pairs
 template <int KIND, bool ORD, int PF, bool FAST, int BOX>
 __global__ void __launch_bounds__(kBlock, (Occupancy<FAST, BOX>::blocks))
     pairs_kernel(const float4 *__restrict__ srec, const uint64_t *__restrict__ off,
                  const uint32_t *__restrict__ cs, const Chunk *__restrict__ chunks,
                  const Task *__restrict__ tasks, uint32_t ntasks, CellGeom g, PairSpec S,
                  unsigned long long *__restrict__ target, size_t nvox,
-                 unsigned long long *__restrict__ dropped, const uint32_t *__restrict__ wflag, int walk,
+                 unsigned long long *__restrict__ dropped, const uint32_t *__restrict__ wflag,
                  const float *__restrict__ rots)
 {
     constexpr int kGroup = BoxCfg<BOX>::group;
     constexpr int kItemsPerThread = BoxCfg<BOX>::ipt;
     __shared__ unsigned long long box[BOX];
-    __shared__ uint4 item[kGroup];        // {a base, na, b base, nb}
-    __shared__ uint32_t pre[kGroup + 1];  // exclusive prefix of pair counts
+    __shared__ uint4 item[kGroup];         // {a base, na (walk: h), b base, nb}, bases from the chunk's first record
+    __shared__ uint32_t pre[kGroup + 64];  // exclusive prefix of pair counts (walk: then 64 sentinels)
     __shared__ uint32_t wtot[kBlock / 64];
-    __shared__ uint32_t wnz[kBlock / 64];       // walk: items with pairs per wave
-    __shared__ uint8_t wflags[kBlock / 64][64];  // walk: item-start flags per wave
-    __shared__ uint32_t s_nnz;
+    __shared__ uint32_t wnz[kBlock / 64];         // walk: items with pairs per wave
+    __shared__ uint32_t wtags[kBlock / 64][64];   // walk: item-start tags per wave
+    __shared__ uint32_t wfirst[kBlock / 64];      // walk: the item holding each wave's first pair
+    __shared__ uint32_t s_total;                  // walk: pairs of the group
 
     const Task T = tasks[blockIdx.x % ntasks];
     const Chunk CH = chunks[blockIdx.x / ntasks];
+    // the chunk's records: item bases count from its first record, so record byte
+    // offsets stay below 2^32 (a chunk holds far fewer than 2^28 records)
+    const uint64_t rec0 = off[CH.f0];
+    const float4 *__restrict__ crec = srec + rec0;
     const int ncell = g.ncu * g.ncv;
     const bool zero_delta = (T.du == 0 && T.dv == 0);
     const bool tri = !ORD && zero_delta;  // distinct unordered pairs within a cell
@@ -792,7 +842,7 @@ __global__ void __launch_bounds__(kBlock, (Occupancy<FAST, BOX>::blocks))
     const float inv_cells = 1.0f / (float)cells, inv_nu = 1.0f / (float)T.nu;
     const uint32_t lane = threadIdx.x & 63u, wave = threadIdx.x >> 6;
     // lanes on consecutive pairs (every mode unless a pair range per thread is asked for)
-    const bool walking = PF == 1 && walk;
+    constexpr bool walking = PF == 0;
     for (uint32_t ub = 0; ub < nitems; ub += kGroup) {
         // phase 1: item descriptors and pair counts (contiguous items per thread)
         uint32_t mycnt[kItemsPerThread];
@@ -813,13 +863,14 @@ __global__ void __launch_bounds__(kBlock, (Occupancy<FAST, BOX>::blocks))
                 const int ci = iv * g.ncu + iu;
                 const int cj = (iv - T.dv) * g.ncu + (iu - T.du);
                 const uint32_t *csf = cs + (size_t)f * (ncell + 1);
-                const uint32_t base = (uint32_t)off[f];
+                const uint32_t base = (uint32_t)(off[f] - rec0);
                 const uint32_t a0 = csf[ci], na = csf[ci + 1] - a0;
                 const uint32_t b0 = csf[cj], nb = csf[cj + 1] - b0;
                 d = make_uint4(base + a0, na, base + b0, nb);
                 c = tri ? na * (na - 1) / 2 : na * nb;
-                // the walk decodes (ia, ib) of a non-triangle item with 1/nb in place of na
-                if (walking && !tri && c) d.y = __float_as_uint(1.0f / (float)nb);
+                // the walk decodes (ia, ib) of a non-triangle item with h = RN(1 / (2 nb)) in
+                // place of na (2 nb is exact in float: nb < 2^24)
+                if (walking && !tri && c) d.y = __float_as_uint(1.0f / (float)(2u * nb));
             }
             if (!walking) item[li] = d;
             myd[k] = d;
@@ -846,23 +897,34 @@ __global__ void __launch_bounds__(kBlock, (Occupancy<FAST, BOX>::blocks))
         __syncthreads();
         if (walking) {
             // items with pairs only, in order: item starts become distinct
-            uint32_t run = x - tsum, rz = z - nz;
-            for (uint32_t w = 0; w < wave; w++) {
-                run += wtot[w];
-                rz += wnz[w];
+            uint32_t run = x - tsum, rz = z - nz, tot = 0, nzt = 0;
+            for (uint32_t w = 0; w < kBlock / 64; w++) {
+                if (w < wave) {
+                    run += wtot[w];
+                    rz += wnz[w];
+                }
+                tot += wtot[w];
+                nzt += wnz[w];
             }
 #pragma unroll
             for (int k = 0; k < kItemsPerThread; k++) {
                 if (mycnt[k]) {
+                    // the item holding the first pair of waves 1..3 (the walk splits the
+                    // group's pairs in quarters)
+                    for (uint32_t w = 1; w < kBlock / 64; w++) {
+                        const uint32_t first = (uint32_t)(((uint64_t)tot * w) / (kBlock / 64));
+                        if (first >= run && first - run < mycnt[k]) wfirst[w] = rz;
+                    }
                     item[rz] = myd[k];
                     pre[rz] = run;
                     rz++;
                     run += mycnt[k];
                 }
             }
-            if (threadIdx.x == kBlock - 1) {
-                pre[kGroup] = run;
-                s_nnz = rz;
+            if (threadIdx.x < 64) pre[nzt + threadIdx.x] = ~0u;  // sentinel starts
+            if (threadIdx.x == 0) {
+                wfirst[0] = 0;
+                s_total = tot;
             }
         } else {
             uint32_t run = x - tsum;
@@ -879,15 +941,14 @@ __global__ void __launch_bounds__(kBlock, (Occupancy<FAST, BOX>::blocks))
         // phase 2: thread t bins the contiguous pair range [p0, p1).  PF == 1: pair
         // by pair, one load each.  PF > 1: PF pairs at a time with an integer-only
         // enumeration, so the 2 PF record loads of a batch are in flight together.
-        const uint32_t total = pre[kGroup];
+        const uint32_t total = walking ? s_total : pre[kGroup];
         if (walking) {
             // waves on quarters of the group's pairs, lanes on consecutive pairs
             const uint32_t w0 = (uint32_t)(((uint64_t)total * wave) / (kBlock / 64));
             const uint32_t w1 = (uint32_t)(((uint64_t)total * (wave + 1)) / (kBlock / 64));
             if (w0 < w1)
-                fast_walk_w<KIND, ORD, FAST>(small, inbox, srec, item, pre, s_nnz, w0, w1, tri, total >= (1u << 24), S,
-                                             box, T, sink,
-                            (lds_flag_t *)wflags[wave]);
+                fast_walk_w<KIND, ORD, FAST>(small, inbox, !tri && total <= (1u << 22), crec, item, pre, wfirst[wave],
+                                             w0, w1, total, tri, S, box, T, sink, (lds_tag_t *)wtags[wave]);
             __syncthreads();
             continue;
         }
@@ -915,16 +976,16 @@ __global__ void __launch_bounds__(kBlock, (Occupancy<FAST, BOX>::blocks))
                 ib = r % d.w;
             }
             if (FAST && inbox) {
-                fast_pairs_w<true>(small, srec, item, pre, u, d, ia, ib, p1 - p0, tri, S, box, T, sink);
+                fast_pairs_w<true>(small, crec, item, pre, u, d, ia, ib, p1 - p0, tri, S, box, T, sink);
             } else if (PF == 1) {
                 // one record load per pair: the a record stays in a register along its row
-                float4 ra = srec[d.x + ia];
+                float4 ra = crec[d.x + ia];
                 // in-box C(q): consecutive pairs of a row fall in runs on one voxel, so
                 // a run's values add in a register and reach the box in one atomic
                 int co = -1;
                 unsigned long long cacc = 0;
                 for (uint32_t p = p0;;) {
-                    const float4 rb = srec[d.z + ib];
+                    const float4 rb = crec[d.z + ib];
                     if (KIND == kKindCq && !ORD && inbox) {
                         asm volatile("" ::"v"(rb.w));  // one 16-byte load
                         const float w = ra.w * rb.w;
@@ -982,19 +1043,19 @@ __global__ void __launch_bounds__(kBlock, (Occupancy<FAST, BOX>::blocks))
                             ia = 0;
                             ib = tri ? 1 : 0;
                         }
-                        ra = srec[d.x + ia];
+                        ra = crec[d.x + ia];
                     }
                 }
                 if (co >= 0) atomicAdd(reinterpret_cast<unsigned long long *>(cboxb + co), cacc);
             } else
             for (uint32_t p = p0; p < p1;) {
                 const uint32_t nb = min((uint32_t)PF, p1 - p);
-                float4 A[PF], B[PF];
+                float4 A[PF > 1 ? PF : 1], B[PF > 1 ? PF : 1];
 #pragma unroll
                 for (int k = 0; k < PF; k++) {
                     if ((uint32_t)k < nb) {
-                        A[k] = srec[d.x + ia];
-                        B[k] = srec[d.z + ib];
+                        A[k] = crec[d.x + ia];
+                        B[k] = crec[d.z + ib];
                         if (p + k + 1 < p1) {  // a next pair exists: advance the enumeration
                             ib++;
                             if (ib >= (tri ? d.y : d.w)) {
@@ -1418,18 +1479,22 @@ template <int K, bool O, int BOX>
 void launch_pairs_box(int pf, bool fast, int walk, dim3 grid, hipStream_t st, const PairRun &R, const Bufs &b,
                       unsigned long long *target, size_t nvox, unsigned long long *dropped)
 {
-    if ((O || pf == 1) && fast)
-        hipLaunchKernelGGL((pairs_kernel<K, O, 1, K == kKindAc3d && !O, BOX>), grid, dim3(kBlock), 0, st, b.srec,
-                           R.d_frame_off, b.cs, b.d_chunks, b.d_tasks, b.ntasks, b.g, R.spec, target, nvox, dropped, b.d_flag, walk, R.d_rots);
-    else if (O || pf == 1 || BOX != kBoxMax)
-        hipLaunchKernelGGL((pairs_kernel<K, O, 1, false, BOX>), grid, dim3(kBlock), 0, st, b.srec, R.d_frame_off,
-                           b.cs, b.d_chunks, b.d_tasks, b.ntasks, b.g, R.spec, target, nvox, dropped, b.d_flag, walk, R.d_rots);
-    else if (pf == 4)
-        hipLaunchKernelGGL((pairs_kernel<K, O, O ? 1 : 4, false, BOX>), grid, dim3(kBlock), 0, st, b.srec,
-                           R.d_frame_off, b.cs, b.d_chunks, b.d_tasks, b.ntasks, b.g, R.spec, target, nvox, dropped, b.d_flag, walk, R.d_rots);
-    else
-        hipLaunchKernelGGL((pairs_kernel<K, O, O ? 1 : 8, false, BOX>), grid, dim3(kBlock), 0, st, b.srec,
-                           R.d_frame_off, b.cs, b.d_chunks, b.d_tasks, b.ntasks, b.g, R.spec, target, nvox, dropped, b.d_flag, walk, R.d_rots);
+    auto go = [&](auto kernel) {
+        hipLaunchKernelGGL(kernel, grid, dim3(kBlock), 0, st, b.srec, R.d_frame_off, b.cs, b.d_chunks, b.d_tasks,
+                           b.ntasks, b.g, R.spec, target, nvox, dropped, b.d_flag, R.d_rots);
+    };
+    constexpr bool kFast = K == kKindAc3d && !O;
+    if ((O || pf == 1) && fast) {
+        if (walk) go(pairs_kernel<K, O, 0, kFast, BOX>);
+        else go(pairs_kernel<K, O, 1, kFast, BOX>);
+    } else if (O || pf == 1 || BOX != kBoxMax) {
+        if (walk) go(pairs_kernel<K, O, 0, false, BOX>);
+        else go(pairs_kernel<K, O, 1, false, BOX>);
+    } else if (pf == 4) {
+        go(pairs_kernel<K, O, O ? 1 : 4, false, BOX>);
+    } else {
+        go(pairs_kernel<K, O, O ? 1 : 8, false, BOX>);
+    }
 }
 
 // walk: lanes on consecutive pairs (1) or a contiguous pair range per thread (0)
