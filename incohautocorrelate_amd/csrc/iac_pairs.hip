@@ -454,19 +454,24 @@ struct BoxSink {
     unsigned long long drop_weight;
     const float *rot;  // the chunk's rotation of first-level voxels, or null
     int c0, c1, c2;
+    // an add outside the LDS box: the mesh voxel under the chunk's rotation, or the dropped
+    // count when it leaves the mesh.  Target and value by selection, one atomic: nested
+    // divergent branches here kept values live across the whole group loop (spills).
+    __device__ __forceinline__ void add_outside(int fs, int ms, int ss, uint64_t v)
+    {
+        if (rot) rotate_voxel(rot, c0, c1, c2, fs, ms, ss);
+        const bool ok = (unsigned)fs < (unsigned)S && (unsigned)ms < (unsigned)S && (unsigned)ss < (unsigned)SC;
+        const size_t idx = ok ? (size_t)fs + (size_t)ms * S + (size_t)ss * S * S : 0;
+        atomicAdd(ok ? out + idx : dropped, ok ? (unsigned long long)v : drop_weight);
+    }
     __device__ __forceinline__ void add(int fs, int ms, int ss, uint64_t v)
     {
         if (!v) return;
         const unsigned lx = (unsigned)(fs - bx0), ly = (unsigned)(ms - by0), lz = (unsigned)(ss - bz0);
-        if (lx < (unsigned)bx && ly < (unsigned)by && lz < (unsigned)bz) {
+        if (lx < (unsigned)bx && ly < (unsigned)by && lz < (unsigned)bz)
             atomicAdd(box + lx + (size_t)bx * (ly + (size_t)by * lz), (unsigned long long)v);
-        } else {
-            if (rot) rotate_voxel(rot, c0, c1, c2, fs, ms, ss);
-            if ((unsigned)fs < (unsigned)S && (unsigned)ms < (unsigned)S && (unsigned)ss < (unsigned)SC)
-                atomicAdd(out + (size_t)fs + (size_t)ms * S + (size_t)ss * S * S, (unsigned long long)v);
-            else
-                atomicAdd(dropped, drop_weight);
-        }
+        else
+            add_outside(fs, ms, ss, v);
     }
 };
 
@@ -573,10 +578,11 @@ __device__ __forceinline__ void tri_decode(uint32_t r, uint32_t n, uint32_t &ia,
 // the item before the step plus the number of tagged positions up to the lane
 // (ballot + mbcnt, the count's base and the next step's item in scalar registers).
 //
-// Division: an item of a non-triangle group of at most 2^22 pairs carries
-// h = RN(1 / (2 nb)) in place of na, and ia = trunc(f32(2r + 1) * h) is exactly
-// r / nb for every r < 2^22: (r + 1/2) / nb lies at least 1 / (2 nb) from the
-// integers, and the two roundings move it by at most (r + 1/2) / nb * 2^-23.
+// Division: a non-triangle item carries h = RN(1 / (2 nb)) in place of na and -nb
+// in place of nb, and ia = trunc(f32(2r + 1) * h) is exactly r / nb for every
+// r < 2^22: (r + 1/2) / nb lies at least 1 / (2 nb) from the integers, and the two
+// roundings move it by at most (r + 1/2) / nb * 2^-23.  A group of more than 2^22
+// pairs takes a correction step (below 2^24) or divides in integers.
 //
 // Records are read at 32-bit byte offsets from the chunk's first record, one step
 // ahead of the binning.  Lanes past the range read a real pair (their position
@@ -587,7 +593,8 @@ typedef volatile __attribute__((address_space(3))) uint32_t lds_tag_t;
 // The binning: FASTBIN, the AC_uw specialised inline path (SMALL and INBOX apply
 // there); otherwise bin_ordered for ordered runs and bin_sym for symmetric ones.
 // EXACT: the group divides by h (non-triangle, at most 2^22 pairs); otherwise the
-// triangle decode or an integer division.
+// triangle decode, or the reciprocal 2h with one correction step below 2^24 pairs,
+// or an integer division.
 template <int KIND, bool ORD, bool FASTBIN, bool SMALL, bool INBOX, bool EXACT>
 __device__ __forceinline__ void fast_walk(const float4 *__restrict__ src, const uint4 *item, const uint32_t *pre,
                                           uint32_t u0, uint32_t w0, uint32_t w1, uint32_t total, bool tri,
@@ -628,13 +635,21 @@ __device__ __forceinline__ void fast_walk(const float4 *__restrict__ src, const 
         const uint32_t r = min(P + lane, last) - pre[u];
         uint32_t ia, ib;
         if (EXACT) {
+            // d.w = -nb: b's record index d.z + r - ia nb in one 24-bit multiply-add
+            // (ia, nb <= r < 2^22)
             ia = (uint32_t)((float)(2u * r + 1u) * __uint_as_float(d.y));
-            ib = r - __umul24(ia, d.w);
-        } else if (tri) {
+            oa = (d.x + ia) << 4;
+            ob = (uint32_t)(__mul24((int)ia, (int)d.w) + (int)(d.z + r)) << 4;
+            return;
+        }
+        if (tri) {
             tri_decode(r, d.y, ia, ib);
+        } else if (last < (1u << 24)) {
+            divmod24(r, 0u - d.w, 2.0f * __uint_as_float(d.y), ia, ib);  // 2h = RN(1 / nb)
         } else {
-            ia = r / d.w;
-            ib = r - ia * d.w;
+            const uint32_t nb = 0u - d.w;
+            ia = r / nb;
+            ib = r - ia * nb;
         }
         oa = (d.x + ia) << 4;
         ob = (d.z + ib) << 4;
@@ -659,8 +674,8 @@ __device__ __forceinline__ void fast_walk(const float4 *__restrict__ src, const 
                     if (lx < (uint32_t)T.bx && ly < (uint32_t)T.by && lz < (uint32_t)T.bz)
                         atomicAdd(box + lx + __umul24(ly, (uint32_t)T.bx) + __umul24(lz, (uint32_t)(T.bx * T.by)),
                                   (unsigned long long)v);
-                    else
-                        sink.add(dx + S.P.c0, dy + S.P.c1, dz + S.P.c2, v);
+                    else if (v)
+                        sink.add_outside(dx + S.P.c0, dy + S.P.c1, dz + S.P.c2, v);
                 }
             }
             return;
@@ -869,8 +884,11 @@ __global__ void __launch_bounds__(kBlock, (Occupancy<FAST, BOX>::blocks))
                 d = make_uint4(base + a0, na, base + b0, nb);
                 c = tri ? na * (na - 1) / 2 : na * nb;
                 // the walk decodes (ia, ib) of a non-triangle item with h = RN(1 / (2 nb)) in
-                // place of na (2 nb is exact in float: nb < 2^24)
-                if (walking && !tri && c) d.y = __float_as_uint(1.0f / (float)(2u * nb));
+                // place of na (2 nb is exact in float: nb < 2^24) and -nb in place of nb
+                if (walking && !tri && c) {
+                    d.y = __float_as_uint(1.0f / (float)(2u * nb));
+                    d.w = 0u - nb;
+                }
             }
             if (!walking) item[li] = d;
             myd[k] = d;
