@@ -81,6 +81,7 @@ struct Task {
     int bx0, by0, bz0;  // LDS box origin (voxel indices)
     int bx, by, bz;     // LDS box extent; bx == 0 -> no box, every add goes to global
     int inbox;          // the box lies inside the mesh: every nearest-neighbour pair lands in it
+    int allin;          // the task's q box lies inside the cut sphere: no pair fails the cut
 };
 
 struct Chunk {
@@ -594,8 +595,8 @@ typedef volatile __attribute__((address_space(3))) uint32_t lds_tag_t;
 // there); otherwise bin_ordered for ordered runs and bin_sym for symmetric ones.
 // EXACT: the group divides by h (non-triangle, at most 2^22 pairs); otherwise the
 // triangle decode, or the reciprocal 2h with one correction step below 2^24 pairs,
-// or an integer division.
-template <int KIND, bool ORD, bool FASTBIN, bool SMALL, bool INBOX, bool EXACT>
+// or an integer division.  CUT: FASTBIN tests the cut (false on an all-in task).
+template <int KIND, bool ORD, bool FASTBIN, bool SMALL, bool INBOX, bool EXACT, bool CUT = true>
 __device__ __forceinline__ void fast_walk(const float4 *__restrict__ src, const uint4 *item, const uint32_t *pre,
                                           uint32_t u0, uint32_t w0, uint32_t w1, uint32_t total, bool tri,
                                           const PairSpec &S, unsigned long long *box, const Task &T, BoxSink &sink,
@@ -661,7 +662,7 @@ __device__ __forceinline__ void fast_walk(const float4 *__restrict__ src, const 
             asm volatile("" ::"v"(cb.w), "v"(ca.w));  // 16-byte loads
             const float q0 = ca.x - cb.x, q1 = ca.y - cb.y, q2 = ca.z - cb.z;
             const float sq = (q0 * q0 + q1 * q1) + q2 * q2;
-            if (active && sq < cut_sq) {
+            if (active && (!CUT || sq < cut_sq)) {
                 const int dx = round_away(q0 * vox);
                 const int dy = round_away(q1 * vox);
                 const int dz = round_away(q2 * vox);
@@ -787,7 +788,10 @@ __device__ __forceinline__ void fast_walk_w(bool small, bool inbox, bool exact, 
         else
             go(F_{}, F_{}, F_{});
     } else if (inbox) {
-        if (small) go(T_{}, T_{}, T_{});
+        if (small && T.allin)  // the headline case: no cut test
+            fast_walk<KIND, ORD, true, true, true, true, false>(src, item, pre, u0, w0, w1, total, tri, S, box, T,
+                                                                sink, fl);
+        else if (small) go(T_{}, T_{}, T_{});
         else go(T_{}, F_{}, T_{});
     } else {
         if (small) go(T_{}, T_{}, F_{});
@@ -1301,6 +1305,11 @@ void build_tasks(iac_pair_plan &pl, bool ordered)
                         dist2 += m * m;
                     }
                     if (std::sqrt(dist2) > maxq * (1.0 + 1e-5) + 1e-7) continue;
+                    // the farthest corner of the q box inside the cut, with margin for the float
+                    // rounding of the kernel's q and |q|^2 (a relative 3.1e-7 at most)
+                    double far2 = 0;
+                    for (int d = 0; d < 3; d++) far2 += std::max(qlo[d] * qlo[d], qhi[d] * qhi[d]);
+                    t.allin = far2 * (1.0 + 1e-5) < (double)S.P.cut_sq ? 1 : 0;
                     double lo[3], hi[3];
                     if (fast || tight_cq) {
                         // the pair bins at round_away(fl(fl(ka - kb) * vox)): within eps of x in [l, h]
