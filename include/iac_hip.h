@@ -105,6 +105,9 @@ int iac_ctx_timing(iac_ctx *ctx, double *ms, uint64_t *launches, int reset);
 #define IAC_OPT_SIM_CHUNK_ROWS 4 /* simulator with subpixels or modes: detector rows (pixel x subpixel) per GEMM
                                     launch, rounded to whole groups of 256 pixels, 0 = auto (1 GiB of FP32
                                     magnitudes); one subpixel and one mode need no chunks */
+#define IAC_OPT_LAP_GROUPS 10 /* LAP finder: two-frame groups a tile's waves take in a row of the block order
+                                 (the XCD's L2 keeps the tile's mask, offset and gain rows for them),
+                                 0 = default (16) */
 int iac_ctx_set_option(iac_ctx *ctx, int option, int64_t value);
 /* Updates whose voxel fell outside the mesh since the context was created.
  * The reference writes such updates out of bounds; this path drops them. */

@@ -32,6 +32,7 @@ OPT_BOX_CAP = 6
 OPT_CELL_VOX_V = 7
 OPT_TASK_REGION = 8
 OPT_PAIR_MODE = 9
+OPT_LAP_GROUPS = 10
 SIM_FP32_DIRECT, SIM_BF16X3, SIM_FP16X3, SIM_BF16, SIM_FP16 = 0, 1, 2, 3, 4
 SIM_AMPLITUDE, SIM_INTENSITY = 0, 1
 

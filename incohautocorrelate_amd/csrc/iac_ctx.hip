@@ -316,6 +316,10 @@ int iac_ctx_set_option(iac_ctx *ctx, int option, int64_t value)
             return fail(IAC_ERR_INVALID_ARG, "IAC_OPT_PAIR_PREFETCH is 0, 1, 4 or 8");
         ctx->pair_prefetch = (int)value;
         return IAC_OK;
+    case IAC_OPT_LAP_GROUPS:
+        if (value < 0 || value > 65536) return fail(IAC_ERR_INVALID_ARG, "IAC_OPT_LAP_GROUPS is 0..65536");
+        ctx->lap_groups = (uint32_t)value;
+        return IAC_OK;
     case IAC_OPT_SIM_CHUNK_ROWS:
         if (value < 0 || value > (1ll << 31)) return fail(IAC_ERR_INVALID_ARG, "IAC_OPT_SIM_CHUNK_ROWS is 0..2^31");
         ctx->sim_chunk_rows = (uint64_t)value;

@@ -34,6 +34,7 @@ struct iac_ctx {
     int task_region = 0;  // pair engine: cells per task region side (0 = per kind)
     int pair_mode = 0;    // pair engine: 0/1 lanes on consecutive pairs, 2 a pair range per thread
     uint32_t frame_chunk = 0;
+    uint32_t lap_groups = 0;      // LAP finder: two-frame groups per tile in a row of the block order (0: 16)
     uint64_t sim_chunk_rows = 0;  // simulator: rows (pixel x subpixel) per GEMM launch with subpixels or modes
     int box_cap = 0;        // pair engine: LDS box voxels, 1024, 1920 or 4096 (0 = per kind)
     int pair_prefetch = 0;  // pair engine: pairs per load batch (1, 4 or 8; 0 = 1)

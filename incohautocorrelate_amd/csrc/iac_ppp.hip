@@ -224,7 +224,13 @@ __device__ __forceinline__ void load_corr_w(const int *mask, const float *offset
 // The LAP finder over the panels, one wave per tile (a strip of kCols * kW columns and kRows
 // rows of one panel) and group of kF frames.  Lane l holds the kW columns from
 // strip * kCols * kW + (l - 1) * kW (kW = 2 only where every panel starts and ends on even
-// pixels, so the pairs are 8-byte aligned).
+// pixels, so the pairs are 8-byte aligned).  Grid: ntiles * ngrp blocks for ngrp frame groups
+// (each block then steps by ngrp groups), a multiple of G.  Block order: chunks of G groups,
+// in each chunk tile after tile, the G groups of a tile together.  Blocks go to the 8 XCDs
+// round robin; the remap hands each XCD a contiguous range of that order, so a tile's G groups
+// run on one XCD and read the tile's mask, offset and gain rows from its L2 (a frame group's
+// whole detector at a time keeps the frame reads spread over the HBM channels; all groups of
+// one tile at once would read one offset of hundreds of frames, 4 MB apart).
 // IAC/PPP.cpp:53-87: where a pixel's fraction reaches the seed threshold, the largest
 // fraction among its left / right / upper / lower neighbours inside the panel (the upper one
 // only for i > fs, i = row * fs + column) is added and one more photon counted when the sum
@@ -233,9 +239,14 @@ template <bool kMask, bool kGain, int kW>
 __global__ void __launch_bounds__(kWave)
     lap_rows_kernel(const float *__restrict__ in, float *__restrict__ out, uint32_t nframes, uint32_t npix,
                     const int *__restrict__ mask, const float *__restrict__ offset, const float *__restrict__ gain,
-                    PanelTable T, float adu, float seed, float comb, FrameAcc *__restrict__ acc)
+                    PanelTable T, float adu, float seed, float comb, FrameAcc *__restrict__ acc, uint32_t ngrp,
+                    uint32_t G)
 {
-    const uint32_t tile = blockIdx.x;
+    const uint32_t nb = gridDim.x, b = blockIdx.x, xcd = b % 8, local = b / 8;
+    const uint32_t q8 = nb / 8, r8 = nb % 8;
+    const uint32_t lin = xcd < r8 ? xcd * (q8 + 1) + local : r8 * (q8 + 1) + (xcd - r8) * q8 + local;
+    const uint32_t ntiles = T.tile0[T.n], chunk = lin / (ntiles * G), rem = lin % (ntiles * G);
+    const uint32_t tile = rem / G, grp = chunk * G + rem % G;
     uint32_t lo = 0, hi = T.n;
     while (hi - lo > 1) {
         const uint32_t mid = (lo + hi) / 2;
@@ -252,7 +263,7 @@ __global__ void __launch_bounds__(kWave)
     const uint32_t cc = valid ? (uint32_t)c : 0u;
     const uint32_t r0 = seg * kRows, r1 = min(ss, r0 + kRows);
 
-    for (uint32_t fg = blockIdx.y * kF; fg < nframes; fg += gridDim.y * kF) {
+    for (uint32_t fg = grp * kF; fg < nframes; fg += ngrp * kF) {
         const uint32_t nf = min(kF, nframes - fg);
         // fractions (and whole photons) of row r for the kF frames; 0 off the panel
         auto load_row = [&](uint32_t r, float (*frag)[kW], float (*whole)[kW]) {
@@ -454,9 +465,16 @@ int iac_ppp_lap(iac_ctx *ctx, const float *d_in, float *d_out, uint32_t nframes,
                                 : (hg ? lap_rows_kernel<false, true, 2> : lap_rows_kernel<false, false, 2>))
                           : (hm ? (hg ? lap_rows_kernel<true, true, 1> : lap_rows_kernel<true, false, 1>)
                                 : (hg ? lap_rows_kernel<false, true, 1> : lap_rows_kernel<false, false, 1>));
-        const dim3 grid(T.tile0[T.n], std::min<uint32_t>(65535, (nframes + kF - 1) / kF));
-        hipLaunchKernelGGL(kern, grid, dim3(kWave), 0, ctx->stream, src, d_out, nframes, npix, d_mask, d_offset,
-                           d_gain, T, adu_per_photon, seed_threshold, combined_threshold, acc);
+        // every frame group its own block while the grid stays below 2^31 blocks; groups in
+        // chunks of G (blocks of a padded chunk past the last frame do nothing)
+        const uint32_t ntiles = T.tile0[T.n];
+        const uint64_t nfg = (nframes + kF - 1) / kF;
+        const uint32_t G = (uint32_t)std::min<uint64_t>(ctx->lap_groups ? ctx->lap_groups : 16, nfg);
+        const uint64_t cap = 0x7fffffffu / ((uint64_t)ntiles * G);  // chunks per grid
+        if (cap == 0) return iac::fail(IAC_ERR_LIMIT, "detector panels too large for IAC_OPT_LAP_GROUPS");
+        const uint32_t ngrp = (uint32_t)(std::min<uint64_t>((nfg + G - 1) / G, cap) * G);
+        hipLaunchKernelGGL(kern, dim3(ntiles * ngrp), dim3(kWave), 0, ctx->stream, src, d_out, nframes, npix, d_mask,
+                           d_offset, d_gain, T, adu_per_photon, seed_threshold, combined_threshold, acc, ngrp, G);
         IAC_HIP(hipGetLastError());
     }
     if (full < npix) {

@@ -100,6 +100,28 @@ def test_lap_full_detector_modules(ctx, dev):
     assert np.array_equal(pc, rpc) and np.array_equal(mi.view(np.int32), rmi.view(np.int32))
 
 
+@pytest.mark.parametrize("groups", [1, 3, 16, 1000])
+def test_lap_block_order_groups(dev, groups):
+    """IAC_OPT_LAP_GROUPS only reorders the blocks: 37 frames (19 two-frame groups: not a
+    multiple of 3 or 16, fewer than 1000) on the odd and the even layouts equal the oracle."""
+    import incohautocorrelate_amd as iac
+
+    c = iac.Context(0)
+    try:
+        c.set_option(iac.OPT_LAP_GROUPS, groups)
+        for name in ("odd", "even"):
+            npix, panels = layout(name)
+            print(f"lap groups={groups} layout={name}", flush=True)
+            I = frames_for(npix, 37, seed=groups)
+            mask, offset, gain = maps_for(npix, groups)
+            ref, rpc, rmi = O.ppp_lap(I, panels, adu=ADU, mask=mask, offset=offset, gain=gain)
+            out, pc, mi = run_gpu(c, dev, I, panels, mask, offset, gain, False, False)
+            assert np.array_equal(out.view(np.int32), ref.view(np.int32))
+            assert np.array_equal(pc, rpc) and np.array_equal(mi.view(np.int32), rmi.view(np.int32))
+    finally:
+        c.close()
+
+
 def test_lap_thresholds(ctx, dev):
     """Other seed / combined thresholds than the defaults."""
     npix, panels = layout("eight")
