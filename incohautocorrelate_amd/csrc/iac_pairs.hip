@@ -630,10 +630,19 @@ __device__ __forceinline__ void fast_walk(const float4 *__restrict__ src, const 
         return __builtin_amdgcn_mbcnt_hi((uint32_t)(starts >> 33),
                                          __builtin_amdgcn_mbcnt_lo((uint32_t)(starts >> 1), base));
     };
-    // byte offsets of the two records of pair min(P + lane, last), which item u holds
-    auto decode = [&](uint32_t u, uint32_t P, uint32_t &oa, uint32_t &ob) {
-        const uint4 d = item[u];
-        const uint32_t r = min(P + lane, last) - pre[u];
+    // the descriptor and first pair of the item of pair P + lane, a step before its decode
+    struct Ahead {
+        uint4 d;
+        uint32_t pu;
+    };
+    auto lookup = [&](uint32_t P) -> Ahead {
+        const uint32_t u = item_at(P);
+        return Ahead{item[u], pre[u]};
+    };
+    // byte offsets of the two records of pair min(P + lane, last), which item A holds
+    auto decode = [&](const Ahead &A, uint32_t P, uint32_t &oa, uint32_t &ob) {
+        const uint4 d = A.d;
+        const uint32_t r = min(P + lane, last) - A.pu;
         uint32_t ia, ib;
         if (EXACT) {
             // d.w = -nb: b's record index d.z + r - ia nb in one 24-bit multiply-add
@@ -662,7 +671,20 @@ __device__ __forceinline__ void fast_walk(const float4 *__restrict__ src, const 
             asm volatile("" ::"v"(cb.w), "v"(ca.w));  // 16-byte loads
             const float q0 = ca.x - cb.x, q1 = ca.y - cb.y, q2 = ca.z - cb.z;
             const float sq = (q0 * q0 + q1 * q1) + q2 * q2;
-            if (active && (!CUT || sq < cut_sq)) {
+            const bool in = active && (!CUT || sq < cut_sq);
+            if (INBOX && SMALL) {
+                // every pair of an in-box task (lanes past the range read pairs of the same task)
+                // lands in the box: branch-free, a lane outside the range or the cut adds 0, so
+                // the step is one basic block and the next step's item lookup interleaves with it
+                const int dx = round_away(q0 * vox);
+                const int dy = round_away(q1 * vox);
+                const int dz = round_away(q2 * vox);
+                const uint32_t v = (uint32_t)((ca.w * 1000.0f) * cb.w);
+                const int o = __mul24(dz, bxby8) + (__mul24(dy, bx8) + (dx * 8 + c8));
+                atomicAdd(reinterpret_cast<unsigned long long *>(boxb + o), (unsigned long long)(in ? v : 0u));
+                return;
+            }
+            if (in) {
                 const int dx = round_away(q0 * vox);
                 const int dy = round_away(q1 * vox);
                 const int dz = round_away(q2 * vox);
@@ -745,17 +767,20 @@ __device__ __forceinline__ void fast_walk(const float4 *__restrict__ src, const 
     // after the issue of the next step's loads.
     auto lanes = [&](uint32_t P) { return w1 > P ? w1 - P : 0u; };  // lanes of step P in range
     uint32_t oa, ob;
-    decode(item_at(w0), w0, oa, ob);
+    decode(lookup(w0), w0, oa, ob);
     float4 a0 = load(oa), b0 = load(ob);
+    Ahead A = lookup(w0 + 64u);
     for (uint32_t P = w0;; P += 128u) {
-        decode(item_at(P + 64u), P + 64u, oa, ob);
+        decode(A, P + 64u, oa, ob);
         const float4 a1 = load(oa), b1 = load(ob);
         __builtin_amdgcn_sched_barrier(0);
+        A = lookup(P + 128u);
         bin(a0, b0, lane < lanes(P));
-        decode(item_at(P + 128u), P + 128u, oa, ob);
+        decode(A, P + 128u, oa, ob);
         a0 = load(oa);
         b0 = load(ob);
         __builtin_amdgcn_sched_barrier(0);
+        A = lookup(P + 192u);
         bin(a1, b1, lane < lanes(P + 64u));
         if (P + 128u >= w1) break;
     }
