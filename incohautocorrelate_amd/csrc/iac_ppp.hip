@@ -4,11 +4,12 @@
 // finder per detector panel, and each event's PhotonCount / MeanIntensity, with the
 // reference's float arithmetic bit for bit.
 //
-// A panel is walked row by row: one wave owns a strip of 62 columns (its two outer lanes
-// hold the neighbouring columns) over 32 rows, keeping the rows above and below in
-// registers and taking the left and right neighbours from the adjacent lanes, so each pixel
-// of a frame is read about 1.07 times and written once; the mask, offset and gain of a row
-// are read once for four frames.
+// A panel is walked row by row: one wave owns a strip of 62 lanes (two columns per lane
+// where the panels allow 8-byte accesses; its two outer lanes hold the neighbouring
+// columns) over 64 rows and two frames, keeping the rows above and below in registers and
+// taking the left and right neighbours from the lane's other column or the adjacent lanes,
+// so each pixel of a frame is read about 1.06 times and written once.  The mask, offset and
+// gain of a tile are read by its waves for 16 frame pairs in a row, from their XCD's L2.
 #include <algorithm>
 #include <cstring>
 #include <type_traits>
