@@ -157,3 +157,27 @@ def test_lds_dense_and_sparse_frames(lds, dev, box, mode):
     lds.set_option(iac.OPT_PAIR_MODE, mode)
     got = mesh_np(gpu_batch(lds, dev, km, pix, cnt, off, shape))
     assert np.array_equal(got, ref)
+
+
+@pytest.mark.parametrize("zoom", [0.75, 1.5, 2.0])
+def test_lds_zoomed_mesh_cut(lds, dev, zoom):
+    """A mesh zoomed in (q_zoom 1.5, 2) cuts through many tasks: tasks beyond the cut
+    sphere are dropped, tasks across it test every pair, tasks inside it skip the test;
+    zoomed out (0.75) every task lies inside.  All equal the oracle bit for bit, on the
+    sparse 1024^2 detector and on dense frames (items of many pairs)."""
+    import incohautocorrelate_amd as iac
+
+    _, km, mq = W.detector(1024)
+    shape = iac.mesh_shape_big(mq, 127, zoom)
+    print(f"zoomed mesh q_zoom={zoom}: sparse 1024^2, 12 frames", flush=True)
+    d = W.poisson_photon_frames(1024 * 1024, 12, 0.002, seed=7)
+    ref, _ = O.ac3d_frames(km, d["pix"], d["cnt"], d["frame_off"], shape)
+    got = mesh_np(gpu_batch(lds, dev, km, d["pix"], d["cnt"], d["frame_off"], shape))
+    assert np.array_equal(got, ref)
+    _, km2, mq2 = W.detector(128)
+    shape2 = iac.mesh_shape_big(mq2, 63, zoom)
+    print(f"zoomed mesh q_zoom={zoom}: dense 128^2, 20 frames", flush=True)
+    d2 = W.dense_poisson_frames(128 * 128, 20, 0.05, seed=8)
+    ref2, _ = O.ac3d_frames(km2, d2["pix"], d2["cnt"], d2["frame_off"], shape2)
+    got2 = mesh_np(gpu_batch(lds, dev, km2, d2["pix"], d2["cnt"], d2["frame_off"], shape2))
+    assert np.array_equal(got2, ref2)
