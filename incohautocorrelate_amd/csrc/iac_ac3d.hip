@@ -662,6 +662,25 @@ int iac_ac3d_tune(iac_ctx *ctx, const float *d_kmap, uint32_t npix, const uint32
 
 }  // extern "C"
 
+// Two slots of frames: while one slot's uploads and engine run are in flight, pushes fill
+// the other.  A slot's uploads run on the queue's copy stream after the last engine run
+// that read the slot (event done), and its engine run on the context's stream after them
+// (event copied), so the uploads of one flush overlap the previous flush's pair pass.
+struct QueueSlot {
+    // device: records | offsets | rotations | tile offsets (tile kernel fallback)
+    float4 *d_rec = nullptr;
+    uint64_t *d_off = nullptr;
+    float *d_rot = nullptr;
+    uint64_t *d_toff = nullptr;
+    // pinned host staging of the same (records only for host pushes)
+    float4 *h_rec = nullptr;
+    uint64_t *h_off = nullptr;
+    float *h_rot = nullptr;
+    hipEvent_t copied = nullptr;  // this slot's last uploads are done
+    hipEvent_t done = nullptr;    // the last engine run that read this slot is done
+    bool uploading = false;       // an upload from the host staging may still be in flight
+};
+
 struct iac_ac3d_queue {
     iac_ctx *ctx = nullptr;
     const float *d_kmap = nullptr;
@@ -670,19 +689,19 @@ struct iac_ac3d_queue {
     uint64_t *d_mesh = nullptr;
     uint64_t cap_rec = 0;
     uint32_t cap_frames = 0;
-    // device: records | offsets | rotations | tile offsets (tile kernel fallback)
-    void *dmem = nullptr;
+    void *dmem = nullptr;  // both slots' device buffers
+    void *hmem = nullptr;  // both slots' pinned staging
+    QueueSlot slot[2];
+    int cur = 0;                     // the slot pushes fill
+    hipStream_t copy_st = nullptr;   // uploads (default priority)
+    // the current slot's buffers
     float4 *d_rec = nullptr;
     uint64_t *d_off = nullptr;
     float *d_rot = nullptr;
     uint64_t *d_toff = nullptr;
-    // pinned host staging of the same (records only for host pushes)
-    void *hmem = nullptr;
     float4 *h_rec = nullptr;
     uint64_t *h_off = nullptr;
     float *h_rot = nullptr;
-    hipEvent_t staged = nullptr;  // the last flush's uploads from the staging are done
-    bool staging_free = true;     // no upload from the staging is pending
     // pending frames
     uint32_t nframes = 0;
     uint64_t nrec = 0;
@@ -693,14 +712,75 @@ struct iac_ac3d_queue {
 
 namespace {
 
-// Waits (once per flush) until the previous flush has read the pinned staging.
+// Waits for both streams, then frees everything the queue holds.
+void queue_release(iac_ac3d_queue *q)
+{
+    (void)hipStreamSynchronize(q->ctx->stream);
+    if (q->copy_st) (void)hipStreamSynchronize(q->copy_st);
+    iac::pairs_queue_free(q->pq);
+    if (q->dmem) (void)hipFree(q->dmem);
+    if (q->hmem) (void)hipHostFree(q->hmem);
+    for (QueueSlot &S : q->slot) {
+        if (S.copied) (void)hipEventDestroy(S.copied);
+        if (S.done) (void)hipEventDestroy(S.done);
+    }
+    if (q->copy_st) (void)hipStreamDestroy(q->copy_st);
+    delete q;
+}
+
+void queue_point(iac_ac3d_queue *q)
+{
+    const QueueSlot &S = q->slot[q->cur];
+    q->d_rec = S.d_rec;
+    q->d_off = S.d_off;
+    q->d_rot = S.d_rot;
+    q->d_toff = S.d_toff;
+    q->h_rec = S.h_rec;
+    q->h_off = S.h_off;
+    q->h_rot = S.h_rot;
+}
+
+// Waits (once per flush of a slot) until the slot's last uploads have read its staging.
 int queue_claim_staging(iac_ac3d_queue *q)
 {
-    if (!q->staging_free) {
-        IAC_HIP(hipEventSynchronize(q->staged));
-        q->staging_free = true;
+    QueueSlot &S = q->slot[q->cur];
+    if (S.uploading) {
+        IAC_HIP(hipEventSynchronize(S.copied));
+        S.uploading = false;
     }
     return IAC_OK;
+}
+
+// The pair pass over one flushed slot (engine, or the tile kernel where the engine does not apply).
+int queue_run(iac_ac3d_queue *q, QueueSlot &S, uint32_t n, uint64_t nrec, bool rot)
+{
+    iac_ctx *ctx = q->ctx;
+    hipStream_t st = ctx->stream;
+    // the LDS engine takes the identity, or per-frame rotations whose voxels stay in the mesh
+    std::vector<float> hrot;
+    bool engine = q->pq != nullptr;
+    if (engine && rot) {
+        hrot.assign(S.h_rot, S.h_rot + (size_t)n * 9);
+        engine = rotations_on_engine(q->P, hrot, n);
+    }
+    if (engine) {
+        iac::PairRun R;
+        std::memset(&R.spec, 0, sizeof(R.spec));
+        R.rec = S.d_rec;
+        R.d_frame_off = S.d_off;
+        R.nframes = n;
+        R.n_total = nrec;
+        R.d_mesh = q->d_mesh;
+        if (rot) {
+            R.d_rots = S.d_rot;
+            R.h_rots = hrot.data();
+        }
+        return iac::pairs_queue_run(q->pq, R, S.h_off);
+    }
+    hipLaunchKernelGGL(tile_scan_kernel, dim3(1), dim3(1024), 0, st, S.d_off, n, S.d_toff);
+    IAC_HIP(hipGetLastError());
+    const Rot9 ident{{1, 0, 0, 0, 1, 0, 0, 0, 1}};
+    return launch_tiles(ctx, S.d_rec, S.d_off, S.d_toff, n, 0, rot ? S.d_rot : nullptr, ident, q->P, q->d_mesh);
 }
 
 int queue_flush(iac_ac3d_queue *q)
@@ -710,43 +790,31 @@ int queue_flush(iac_ac3d_queue *q)
     hipStream_t st = ctx->stream;
     const uint32_t n = q->nframes;
     const uint64_t nrec = q->nrec;
+    QueueSlot &S = q->slot[q->cur];
+    // uploads on the copy stream once the slot's previous engine run is done, the pair pass
+    // on the context's stream once they are (device records came in on that stream already)
+    IAC_HIP(hipStreamWaitEvent(q->copy_st, S.done, 0));
     if (q->source == 1 && nrec)
-        IAC_HIP(hipMemcpyAsync(q->d_rec, q->h_rec, nrec * sizeof(float4), hipMemcpyHostToDevice, st));
-    IAC_HIP(hipMemcpyAsync(q->d_off, q->h_off, (n + 1) * sizeof(uint64_t), hipMemcpyHostToDevice, st));
+        IAC_HIP(hipMemcpyAsync(S.d_rec, S.h_rec, nrec * sizeof(float4), hipMemcpyHostToDevice, q->copy_st));
+    IAC_HIP(hipMemcpyAsync(S.d_off, S.h_off, (n + 1) * sizeof(uint64_t), hipMemcpyHostToDevice, q->copy_st));
     bool rot = q->any_rot;
-    if (rot) IAC_HIP(hipMemcpyAsync(q->d_rot, q->h_rot, (size_t)n * 9 * sizeof(float), hipMemcpyHostToDevice, st));
-    IAC_HIP(hipEventRecord(q->staged, st));
-    q->staging_free = false;
+    if (rot)
+        IAC_HIP(hipMemcpyAsync(S.d_rot, S.h_rot, (size_t)n * 9 * sizeof(float), hipMemcpyHostToDevice, q->copy_st));
+    IAC_HIP(hipEventRecord(S.copied, q->copy_st));
+    IAC_HIP(hipStreamWaitEvent(st, S.copied, 0));
+    S.uploading = true;
     q->nframes = 0;
     q->nrec = 0;
     q->any_rot = false;
     q->source = 0;
-    if (nrec == 0) return IAC_OK;
-    // the LDS engine takes the identity, or per-frame rotations whose voxels stay in the mesh
-    std::vector<float> hrot;
-    bool engine = q->pq != nullptr;
-    if (engine && rot) {
-        hrot.assign(q->h_rot, q->h_rot + (size_t)n * 9);
-        engine = rotations_on_engine(q->P, hrot, n);
-    }
-    if (engine) {
-        iac::PairRun R;
-        std::memset(&R.spec, 0, sizeof(R.spec));
-        R.rec = q->d_rec;
-        R.d_frame_off = q->d_off;
-        R.nframes = n;
-        R.n_total = nrec;
-        R.d_mesh = q->d_mesh;
-        if (rot) {
-            R.d_rots = q->d_rot;
-            R.h_rots = hrot.data();
-        }
-        return iac::pairs_queue_run(q->pq, R, q->h_off);
-    }
-    hipLaunchKernelGGL(tile_scan_kernel, dim3(1), dim3(1024), 0, st, q->d_off, n, q->d_toff);
-    IAC_HIP(hipGetLastError());
-    const Rot9 ident{{1, 0, 0, 0, 1, 0, 0, 0, 1}};
-    return launch_tiles(ctx, q->d_rec, q->d_off, q->d_toff, n, 0, rot ? q->d_rot : nullptr, ident, q->P, q->d_mesh);
+    // pushes go to the other slot from here on; this slot's run is recorded as done below
+    q->cur ^= 1;
+    queue_point(q);
+    int rc = IAC_OK;
+    if (nrec) rc = queue_run(q, S, n, nrec, rot);
+    const hipError_t e = hipEventRecord(S.done, st);
+    if (rc == IAC_OK && e != hipSuccess) rc = iac::hip_fail(e, "queue flush", __FILE__, __LINE__);
+    return rc;
 }
 
 // Room for one more frame of n records (flushing first when full); writes its rotation.
@@ -804,22 +872,32 @@ int iac_ac3d_queue_create(iac_ctx *ctx, const float *d_kmap, uint32_t npix, cons
     auto up = [](size_t x) { return (x + 255) / 256 * 256; };
     const size_t brec = up(max_records * sizeof(float4)), boff = up(((size_t)max_frames + 1) * sizeof(uint64_t)),
                  brot = up((size_t)max_frames * 9 * sizeof(float));
-    hipError_t e = hipMalloc(&q->dmem, brec + 2 * boff + brot);
-    if (e == hipSuccess) e = hipHostMalloc(&q->hmem, brec + boff + brot, hipHostMallocDefault);
-    if (e == hipSuccess) e = hipEventCreateWithFlags(&q->staged, hipEventDisableTiming);
+    const size_t dslot = brec + 2 * boff + brot, hslot = brec + boff + brot;
+    hipError_t e = hipMalloc(&q->dmem, 2 * dslot);
+    if (e == hipSuccess) e = hipHostMalloc(&q->hmem, 2 * hslot, hipHostMallocDefault);
+    if (e == hipSuccess) e = hipStreamCreateWithFlags(&q->copy_st, hipStreamNonBlocking);
+    for (int k = 0; k < 2 && e == hipSuccess; k++) {
+        e = hipEventCreateWithFlags(&q->slot[k].copied, hipEventDisableTiming);
+        if (e == hipSuccess) e = hipEventCreateWithFlags(&q->slot[k].done, hipEventDisableTiming);
+        if (e == hipSuccess) e = hipEventRecord(q->slot[k].done, ctx->stream);
+    }
     int rc = IAC_OK;
     if (e != hipSuccess) rc = iac::hip_fail(e, "iac_ac3d_queue_create allocation", __FILE__, __LINE__);
     if (rc == IAC_OK) {
-        char *d = static_cast<char *>(q->dmem);
-        q->d_rec = reinterpret_cast<float4 *>(d);
-        q->d_off = reinterpret_cast<uint64_t *>(d + brec);
-        q->d_rot = reinterpret_cast<float *>(d + brec + boff);
-        q->d_toff = reinterpret_cast<uint64_t *>(d + brec + boff + brot);
-        char *h = static_cast<char *>(q->hmem);
-        q->h_rec = reinterpret_cast<float4 *>(h);
-        q->h_off = reinterpret_cast<uint64_t *>(h + brec);
-        q->h_rot = reinterpret_cast<float *>(h + brec + boff);
-        q->h_off[0] = 0;
+        for (int k = 0; k < 2; k++) {
+            QueueSlot &S = q->slot[k];
+            char *d = static_cast<char *>(q->dmem) + k * dslot;
+            S.d_rec = reinterpret_cast<float4 *>(d);
+            S.d_off = reinterpret_cast<uint64_t *>(d + brec);
+            S.d_rot = reinterpret_cast<float *>(d + brec + boff);
+            S.d_toff = reinterpret_cast<uint64_t *>(d + brec + boff + brot);
+            char *h = static_cast<char *>(q->hmem) + k * hslot;
+            S.h_rec = reinterpret_cast<float4 *>(h);
+            S.h_off = reinterpret_cast<uint64_t *>(h + brec);
+            S.h_rot = reinterpret_cast<float *>(h + brec + boff);
+            S.h_off[0] = 0;
+        }
+        queue_point(q);
         // the LDS engine, unless the context asks for the tile kernel
         if (ctx->ac3d_kernel != 1) {
             iac::PairRun R;
@@ -834,10 +912,7 @@ int iac_ac3d_queue_create(iac_ctx *ctx, const float *d_kmap, uint32_t npix, cons
         }
     }
     if (rc != IAC_OK) {
-        if (q->dmem) (void)hipFree(q->dmem);
-        if (q->hmem) (void)hipHostFree(q->hmem);
-        if (q->staged) (void)hipEventDestroy(q->staged);
-        delete q;
+        queue_release(q);
         return rc;
     }
     *out = q;
@@ -888,12 +963,7 @@ int iac_ac3d_queue_destroy(iac_ac3d_queue *q)
     if (!q) return IAC_OK;
     int rc = iac::bind(q->ctx);
     if (rc == IAC_OK) rc = queue_flush(q);
-    (void)hipStreamSynchronize(q->ctx->stream);
-    iac::pairs_queue_free(q->pq);
-    if (q->dmem) (void)hipFree(q->dmem);
-    if (q->hmem) (void)hipHostFree(q->hmem);
-    if (q->staged) (void)hipEventDestroy(q->staged);
-    delete q;
+    queue_release(q);
     return rc;
 }
 
