@@ -210,8 +210,10 @@ int iac_ac3d_tune(iac_ctx *ctx, const float *d_kmap, uint32_t npix, const uint32
  * max_frames frames; 0 selects 2^22 and 4096) or is flushed.  A flush uploads the pending
  * frames and enqueues one pass of the LDS cell-box engine (per-frame rotations included
  * when the second level is nearest-neighbour and every rotated voxel stays in the mesh,
- * else the tile kernel), with no allocation; the host waits only for the previous flush's
- * uploads before it writes the staging again.  The mesh equals calling iac_ac3d_sparse once
+ * else the tile kernel), with no allocation.  Two slots (pinned staging and device records,
+ * 2 x 16 max_records bytes each side) alternate: a flush's uploads run on the queue's own
+ * stream and overlap the previous flush's pass, and the host waits only when it returns to
+ * a slot whose uploads are still running.  The mesh equals calling iac_ac3d_sparse once
  * per frame.  d_kmap ([npix][3]) sets the engine's cell grid (records may hold any k);
  * d_kmap and d_mesh must outlive the queue, and the queue must be destroyed before its
  * context.  After iac_ac3d_queue_flush the mesh is complete in stream order;
