@@ -343,18 +343,21 @@ __global__ void cell_ext_kernel(const float *__restrict__ kmap, uint32_t npix, C
 
 // One block per frame: counting sort of the frame's nonzero records by cell.
 // cs[f*(ncell+1) + c] = first sorted position of cell c relative to off[f];
-// cs[f*(ncell+1) + ncell] = number of kept records.  In symmetric mode the
-// frame's self-pair values are added to self[group of f].
+// cs[f*(ncell+1) + ncell] = number of kept records; cmax[f] = records of the
+// frame's fullest cell.  In symmetric mode the frame's self-pair values are added
+// to self[group of f].
 template <int KIND, bool ORD>
 __global__ void __launch_bounds__(kBlock)
     cell_sort_kernel(const float4 *__restrict__ rec, const uint64_t *__restrict__ off, CellGeom g,
                      float4 *__restrict__ srec, uint32_t *__restrict__ cs, PairSpec S,
                      const uint32_t *__restrict__ group_off, uint32_t ngroups,
-                     unsigned long long *__restrict__ self, uint32_t *__restrict__ wflag)
+                     unsigned long long *__restrict__ self, uint32_t *__restrict__ wflag,
+                     uint32_t *__restrict__ cmax)
 {
     __shared__ uint32_t cnt[kMaxCells];
     __shared__ uint32_t part[kBlock];
     __shared__ unsigned long long sred[kBlock / 64];
+    __shared__ uint32_t smax[kBlock / 64];
     const uint32_t f = blockIdx.x;
     const int ncell = g.ncu * g.ncv;
     const uint64_t base = off[f];
@@ -391,9 +394,14 @@ __global__ void __launch_bounds__(kBlock)
     // exclusive scan of cnt[0..ncell)
     const int per = (ncell + kBlock - 1) / kBlock;
     const int c0 = min(ncell, (int)threadIdx.x * per), c1 = min(ncell, c0 + per);
-    uint32_t s = 0;
-    for (int c = c0; c < c1; c++) s += cnt[c];
+    uint32_t s = 0, m = 0;
+    for (int c = c0; c < c1; c++) {
+        s += cnt[c];
+        m = max(m, cnt[c]);
+    }
     part[threadIdx.x] = s;
+    for (int o = 32; o > 0; o >>= 1) m = max(m, (uint32_t)__shfl_down(m, o, 64));
+    if ((threadIdx.x & 63) == 0) smax[threadIdx.x >> 6] = m;
     __syncthreads();
     for (int o = 1; o < kBlock; o <<= 1) {
         const uint32_t v = threadIdx.x >= (unsigned)o ? part[threadIdx.x - o] : 0u;
@@ -410,6 +418,7 @@ __global__ void __launch_bounds__(kBlock)
         run += v;
     }
     if (threadIdx.x == kBlock - 1) csf[ncell] = part[kBlock - 1];
+    if (threadIdx.x == 0) cmax[f] = max(max(smax[0], smax[1]), max(smax[2], smax[3]));
     __syncthreads();
     for (uint32_t i = threadIdx.x; i < n; i += kBlock) {
         const float4 r = rec[base + i];
@@ -833,7 +842,7 @@ __global__ void __launch_bounds__(kBlock, (Occupancy<FAST, BOX>::blocks))
                  const Task *__restrict__ tasks, uint32_t ntasks, CellGeom g, PairSpec S,
                  unsigned long long *__restrict__ target, size_t nvox,
                  unsigned long long *__restrict__ dropped, const uint32_t *__restrict__ wflag,
-                 const float *__restrict__ rots)
+                 const float *__restrict__ rots, const uint32_t *__restrict__ cmax)
 {
     constexpr int kGroup = BoxCfg<BOX>::group;
     constexpr int kItemsPerThread = BoxCfg<BOX>::ipt;
@@ -845,6 +854,7 @@ __global__ void __launch_bounds__(kBlock, (Occupancy<FAST, BOX>::blocks))
     __shared__ uint32_t wtags[kBlock / 64][64];   // walk: item-start tags per wave
     __shared__ uint32_t wfirst[kBlock / 64];      // walk: the item holding each wave's first pair
     __shared__ uint32_t s_total;                  // walk: pairs of the group
+    __shared__ uint32_t s_heavy[kBlock / 64];     // per wave: the chunk holds an item of 2^32 / kGroup pairs or more
 
     const Task T = tasks[blockIdx.x % ntasks];
     const Chunk CH = chunks[blockIdx.x / ntasks];
@@ -876,8 +886,17 @@ __global__ void __launch_bounds__(kBlock, (Occupancy<FAST, BOX>::blocks))
     const int cst1 = S.ka == 1 ? 8 : (S.kb == 1 ? cbx8 : cbxby8);
     const int cst2 = S.ka == 2 ? 8 : (S.kb == 2 ? cbx8 : cbxby8);
     char *const cboxb = reinterpret_cast<char *>(box);
+    // pair counts and prefixes are 32-bit: a group of kGroup items each holding fewer
+    // than 2^32 / kGroup pairs stays below 2^32.  A frame whose fullest cell holds M
+    // records has items of at most M^2 pairs.
+    constexpr uint64_t kHeavy = (1ull << 32) / kGroup;
+    bool hv = false;
+    for (uint32_t f = CH.f0 + threadIdx.x; f < CH.f1; f += kBlock) hv = hv || (uint64_t)cmax[f] * cmax[f] >= kHeavy;
+    const bool whv = __ballot(hv) != 0ull;
+    if ((threadIdx.x & 63u) == 0) s_heavy[threadIdx.x >> 6] = whv;
     for (int v = threadIdx.x; v < vbox; v += kBlock) box[v] = 0ull;
     __syncthreads();
+    const bool heavy = (s_heavy[0] | s_heavy[1] | s_heavy[2] | s_heavy[3]) != 0u;
 
     // items (frame, cell) of the chunk: fewer than 2^24 (at most 4096 frames x 4096 cells),
     // so quotients come from a float reciprocal with one correction step
@@ -887,7 +906,51 @@ __global__ void __launch_bounds__(kBlock, (Occupancy<FAST, BOX>::blocks))
     const uint32_t lane = threadIdx.x & 63u, wave = threadIdx.x >> 6;
     // lanes on consecutive pairs (every mode unless a pair range per thread is asked for)
     constexpr bool walking = PF == 0;
-    for (uint32_t ub = 0; ub < nitems; ub += kGroup) {
+    if (heavy) {
+        // items one after another, each a grid of rows along its shorter side (triangle
+        // items: row a, columns b > a) with the block's threads along a row and four
+        // record loads in flight per thread; general binning through the checked box
+        for (uint32_t u = 0; u < nitems; u++) {
+            uint32_t fr, rem, row, col;
+            divmod24(u, cells, inv_cells, fr, rem);
+            divmod24(rem, (uint32_t)T.nu, inv_nu, row, col);
+            const uint32_t f = CH.f0 + fr;
+            const int iv = T.iv0 + (int)row, iu = T.iu0 + (int)col;
+            const int ci = iv * g.ncu + iu;
+            const int cj = (iv - T.dv) * g.ncu + (iu - T.du);
+            const uint32_t *csf = cs + (size_t)f * (ncell + 1);
+            const uint32_t base = (uint32_t)(off[f] - rec0);
+            const uint32_t a0 = base + csf[ci], na = csf[ci + 1] - csf[ci];
+            const uint32_t b0 = base + csf[cj], nb = csf[cj + 1] - csf[cj];
+            const bool by_a = tri || na <= nb;  // rows are a records, columns b records
+            const uint32_t nrow = by_a ? na : nb, ncol = by_a ? nb : na;
+            const uint32_t rbase = by_a ? a0 : b0, cbase = by_a ? b0 : a0;
+            if (ncol == 0) continue;
+            for (uint32_t r = 0; r < nrow; r++) {
+                const float4 rr = crec[rbase + r];
+                for (uint32_t c = (tri ? r + 1 : 0) + threadIdx.x; c < ncol; c += 4 * kBlock) {
+                    float4 rc[4];
+#pragma unroll
+                    for (int k = 0; k < 4; k++) rc[k] = crec[cbase + min(c + k * kBlock, ncol - 1)];
+#pragma unroll
+                    for (int k = 0; k < 4; k++) {
+                        if (c + k * kBlock >= ncol) break;
+                        const float4 &ra = by_a ? rr : rc[k];
+                        const float4 &rb = by_a ? rc[k] : rr;
+                        if (ORD) {
+                            bin_ordered<KIND>(S, ra, rb, sink);
+                        } else {
+                            int fs, ms, ss;
+                            uint64_t v;
+                            if (bin_sym<KIND>(S, ra, rb, fs, ms, ss, v)) sink.add(fs, ms, ss, v);
+                        }
+                    }
+                }
+            }
+        }
+        __syncthreads();  // the box is complete before the flush
+    }
+    for (uint32_t ub = 0; ub < (heavy ? 0u : nitems); ub += kGroup) {
         // phase 1: item descriptors and pair counts (contiguous items per thread)
         uint32_t mycnt[kItemsPerThread];
         uint4 myd[kItemsPerThread];
@@ -1512,6 +1575,17 @@ std::vector<Chunk> make_chunks(const uint64_t *off, uint32_t nframes, const uint
     return chunks;
 }
 
+// The pair kernel reads a chunk's records at 32-bit byte offsets from its first record: every
+// chunk spans fewer than 2^28 records (the chunk budget keeps a chunk's other frames to a few
+// million records, so only a single frame of ~2^28 records can break this).
+int check_chunks(const uint64_t *off, const std::vector<Chunk> &chunks)
+{
+    for (const Chunk &c : chunks)
+        if (off[c.f1] - off[c.f0] >= (1ull << 28))
+            return fail(IAC_ERR_LIMIT, "pair engine: a frame chunk of 2^28 or more records (split the frame)");
+    return IAC_OK;
+}
+
 struct Bufs {
     float4 *srec;
     uint32_t *cs;
@@ -1527,13 +1601,19 @@ struct Bufs {
     int box_cap;
 };
 
+// per frame, the records of its fullest cell: after the frames' cell starts
+uint32_t *cell_max(const PairRun &R, const Bufs &b)
+{
+    return b.cs + (size_t)R.nframes * (b.g.ncu * b.g.ncv + 1);
+}
+
 template <int K, bool O, int BOX>
 void launch_pairs_box(int pf, bool fast, int walk, dim3 grid, hipStream_t st, const PairRun &R, const Bufs &b,
                       unsigned long long *target, size_t nvox, unsigned long long *dropped)
 {
     auto go = [&](auto kernel) {
         hipLaunchKernelGGL(kernel, grid, dim3(kBlock), 0, st, b.srec, R.d_frame_off, b.cs, b.d_chunks, b.d_tasks,
-                           b.ntasks, b.g, R.spec, target, nvox, dropped, b.d_flag, R.d_rots);
+                           b.ntasks, b.g, R.spec, target, nvox, dropped, b.d_flag, R.d_rots, cell_max(R, b));
     };
     constexpr bool kFast = K == kKindAc3d && !O;
     if ((O || pf == 1) && fast) {
@@ -1581,17 +1661,17 @@ int enqueue(iac_ctx *ctx, hipStream_t st, const PairRun &R, const Bufs &b, bool 
     if (S.kind == kKindAc3d) {
         if (ordered)
             hipLaunchKernelGGL((cell_sort_kernel<kKindAc3d, true>), dim3(R.nframes), dim3(kBlock), 0, st, R.rec,
-                               R.d_frame_off, b.g, b.srec, b.cs, S, b.d_goff, R.ngroups, b.d_self, b.d_flag);
+                               R.d_frame_off, b.g, b.srec, b.cs, S, b.d_goff, R.ngroups, b.d_self, b.d_flag, cell_max(R, b));
         else
             hipLaunchKernelGGL((cell_sort_kernel<kKindAc3d, false>), dim3(R.nframes), dim3(kBlock), 0, st, R.rec,
-                               R.d_frame_off, b.g, b.srec, b.cs, S, b.d_goff, R.ngroups, b.d_self, b.d_flag);
+                               R.d_frame_off, b.g, b.srec, b.cs, S, b.d_goff, R.ngroups, b.d_self, b.d_flag, cell_max(R, b));
     } else {
         if (ordered)
             hipLaunchKernelGGL((cell_sort_kernel<kKindCq, true>), dim3(R.nframes), dim3(kBlock), 0, st, R.rec,
-                               R.d_frame_off, b.g, b.srec, b.cs, S, b.d_goff, R.ngroups, b.d_self, b.d_flag);
+                               R.d_frame_off, b.g, b.srec, b.cs, S, b.d_goff, R.ngroups, b.d_self, b.d_flag, cell_max(R, b));
         else
             hipLaunchKernelGGL((cell_sort_kernel<kKindCq, false>), dim3(R.nframes), dim3(kBlock), 0, st, R.rec,
-                               R.d_frame_off, b.g, b.srec, b.cs, S, b.d_goff, R.ngroups, b.d_self, b.d_flag);
+                               R.d_frame_off, b.g, b.srec, b.cs, S, b.d_goff, R.ngroups, b.d_self, b.d_flag, cell_max(R, b));
     }
     IAC_HIP(hipGetLastError());
     auto *target = ordered ? reinterpret_cast<unsigned long long *>(R.d_mesh) : b.H;
@@ -1672,7 +1752,7 @@ Sizes sizes_of(const PairRun &R, int nc, size_t nchunks)
     const size_t nvox = (size_t)R.spec.P.sab * R.spec.P.sab * R.spec.P.sc;
     auto up = [](size_t x) { return (x + 255) / 256 * 256; };
     return Sizes{up(R.n_total * sizeof(float4)),
-                 up((size_t)R.nframes * (nc + 1) * sizeof(uint32_t)),
+                 up((size_t)R.nframes * (nc + 2) * sizeof(uint32_t)),  // cell starts, then the fullest cells
                  R.ordered ? 0 : nvox * R.ngroups * sizeof(unsigned long long),
                  up((R.ngroups + 1) * sizeof(uint32_t)),
                  up(R.ngroups * sizeof(unsigned long long) + sizeof(uint32_t)),
@@ -1700,6 +1780,7 @@ int pairs_run(iac_ctx *ctx, const PairRun &R)
     IAC_HIP(hipMemcpyAsync(off.data(), R.d_frame_off, off.size() * 8, hipMemcpyDeviceToHost, st));
     IAC_HIP(hipStreamSynchronize(st));
     const std::vector<Chunk> chunks = make_chunks(off.data(), R.nframes, R.h_group_off, R.ngroups, fchunk_of(ctx), R.h_rots);
+    IAC_TRY(check_chunks(off.data(), chunks));
     const int nc = pl->g.ncu * pl->g.ncv;
     const Sizes z = sizes_of(R, nc, chunks.size());
     // scratch: sorted records + cell offsets (slot 1), H + group offsets + self + chunks (slot 2)
@@ -1753,6 +1834,7 @@ int pairs_prepare(iac_ctx *ctx, const PairRun &R0, PairBatch **out)
     B->group_off.assign(R0.h_group_off, R0.h_group_off + R0.ngroups + 1);
     B->R.h_group_off = B->group_off.data();
     const std::vector<Chunk> chunks = make_chunks(off.data(), R0.nframes, B->R.h_group_off, R0.ngroups, fchunk_of(ctx));
+    IAC_TRY(check_chunks(off.data(), chunks));
     const int nc = pl->g.ncu * pl->g.ncv;
     const Sizes z = sizes_of(B->R, nc, chunks.size());
     const size_t tbytes = std::max<size_t>(1, pl->tasks.size()) * sizeof(Task);
@@ -1879,6 +1961,7 @@ int pairs_queue_run(PairQueue *Q, const PairRun &R, const uint64_t *h_off)
     hipStream_t st = Q->ctx->stream;
     const uint32_t goff[2] = {0, R.nframes};
     const std::vector<Chunk> chunks = make_chunks(h_off, R.nframes, goff, 1, fchunk_of(Q->ctx), R.h_rots);
+    IAC_TRY(check_chunks(h_off, chunks));
     // the previous run's uploads read the pinned table: wait for them before rewriting it
     IAC_HIP(hipEventSynchronize(Q->staged));
     std::copy(chunks.begin(), chunks.end(), Q->h_chunks);

@@ -181,3 +181,69 @@ def test_lds_zoomed_mesh_cut(lds, dev, zoom):
     ref2, _ = O.ac3d_frames(km2, d2["pix"], d2["cnt"], d2["frame_off"], shape2)
     got2 = mesh_np(gpu_batch(lds, dev, km2, d2["pix"], d2["cnt"], d2["frame_off"], shape2))
     assert np.array_equal(got2, ref2)
+
+
+@pytest.mark.parametrize("box,mode,interp1,chunk", [(1920, 1, 0, 1), (4096, 1, 0, 1), (1920, 2, 0, 0),
+                                                    (1920, 1, 1, 1)])
+def test_lds_heavy_cell_vs_oracle(lds, dev, box, mode, interp1, chunk):
+    """Cells of 64 voxels on a 31-voxel mesh hold the whole 96^2 detector: a frame of
+    5,000 photons is one item of 12.5M distinct (25M ordered) pairs, at or past
+    2^32 / group size, so its chunk bins item by item; with one-frame chunks the
+    other frames still walk.  Bit-exact against the oracle, nearest and linear."""
+    import incohautocorrelate_amd as iac
+
+    _, km, mq = W.detector(96)
+    shape = iac.mesh_shape_big(mq, 31)
+    rng = np.random.default_rng(21)
+    sizes = [300, 5000, 40, 0, 700]
+    pix, off = [], [0]
+    for n in sizes:
+        p = np.sort(rng.choice(96 * 96, n, replace=False)) if n else np.zeros(0, np.int64)
+        pix.append(p)
+        off.append(off[-1] + n)
+    pix = np.concatenate(pix).astype(np.uint32)
+    cnt = rng.integers(1, 3, pix.size).astype(np.uint16)
+    off = np.array(off, np.uint64)
+    print(f"heavy cell: box {box} mode {mode} interp1 {interp1} chunk {chunk}", flush=True)
+    ref, _ = O.ac3d_frames(km, pix, cnt, off, shape, interp1=interp1)
+    lds.set_option(iac.OPT_CELL_VOX, 64)
+    lds.set_option(iac.OPT_BOX_CAP, box)
+    lds.set_option(iac.OPT_PAIR_MODE, mode)
+    lds.set_option(iac.OPT_FRAME_CHUNK, chunk)
+    got = mesh_np(gpu_batch(lds, dev, km, pix, cnt, off, shape, interp1=interp1))
+    assert np.array_equal(got, ref)
+
+
+def test_lds_heavy_group_past_2_32(lds, dev):
+    """One frame of 95,000 unit photons on three k points, all in one cell: the cell's
+    4.5e9 distinct pairs pass 2^32 inside one group.  The mesh is a quadratic form in
+    the photon counts c_i per point, sum c_i E_i + sum_{i<j} c_i c_j P_ij +
+    sum C(c_i, 2) Q_i, whose terms the oracle gives from frames of one or two photons."""
+    import incohautocorrelate_amd as iac
+
+    _, km, mq = W.detector(96)
+    shape = iac.mesh_shape_big(mq, 31)
+    pts = km.reshape(-1, 3)[[48 * 96 + 48, 48 * 96 + 51, 52 * 96 + 47]]
+    counts = [93500, 1000, 500]
+    kmap = np.repeat(pts, counts, axis=0).astype(np.float32)
+    n = kmap.shape[0]
+
+    def oracle(classes):
+        m, dropped = O.ac3d_frames(pts, np.asarray(classes, np.uint32), np.ones(len(classes), np.uint16),
+                                   np.array([0, len(classes)], np.uint64), shape)
+        assert dropped == 0
+        return m
+
+    E = [oracle([i]) for i in range(3)]
+    want = np.zeros_like(E[0])
+    for i in range(3):
+        c = np.uint64(counts[i])
+        want += c * E[i] + (c * (c - np.uint64(1)) // np.uint64(2)) * (oracle([i, i]) - np.uint64(2) * E[i])
+        for j in range(i + 1, 3):
+            want += c * np.uint64(counts[j]) * (oracle([i, j]) - E[i] - E[j])
+    lds.set_option(iac.OPT_CELL_VOX, 64)
+    print(f"heavy group: one frame of {n} photons on 3 points", flush=True)
+    got = mesh_np(gpu_batch(lds, dev, kmap, np.arange(n, dtype=np.uint32), np.ones(n, np.uint16),
+                            np.array([0, n], np.uint64), shape))
+    assert want[want.size // 2] > 0
+    assert np.array_equal(got, want)

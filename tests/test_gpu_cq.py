@@ -276,3 +276,27 @@ def test_cq_dense_pair_modes(ctx, dev, mode, cell, box, region):
         ctx.set_option(iac.OPT_BOX_CAP, 0)
         ctx.set_option(iac.OPT_TASK_REGION, 0)
     assert np.array_equal(got, ref)
+
+
+@pytest.mark.parametrize("interp", [0, 1])
+def test_cq_heavy_cell(ctx, dev, interp):
+    """Cells of 64 voxels make the whole 80^2 detector one cell: one item of 20M
+    distinct pixel pairs, past 2^32 / group size, so the frame bins item by item.
+    Bit-exact against the oracle."""
+    import incohautocorrelate_amd as iac
+
+    n = 80
+    _, km, mq = W.detector(n)
+    s = iac.mesh_shape_small_cofq(mq, 21)
+    rng = np.random.default_rng(5)
+    I = rng.gamma(1.0, 0.5, n * n).astype(np.float32)
+    mult = iac.cq_small_multiplicator(I)
+    ref, _ = O.cq_small(I, km, s, interp, mult)
+    print(f"C(q) heavy cell: interp {interp}", flush=True)
+    ctx.set_option(iac.OPT_CELL_VOX, 64)
+    try:
+        m = _zeros(ref.size, dev)
+        ctx.cq_small(to_dev(km, dev), to_dev(I, dev), s, m, interp, mult)
+    finally:
+        ctx.set_option(iac.OPT_CELL_VOX, 0)
+    assert np.array_equal(mesh_np(m), ref)
