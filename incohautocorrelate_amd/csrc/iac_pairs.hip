@@ -774,7 +774,9 @@ __device__ __forceinline__ void fast_walk(const float4 *__restrict__ src, const 
     // behind a branch, or a second exit, would make the binning wait for the loads in
     // flight too.  The scheduling barriers keep the binning's first use of its records
     // after the issue of the next step's loads.
-    auto lanes = [&](uint32_t P) { return w1 > P ? w1 - P : 0u; };  // lanes of step P in range
+    // lane of step P in range: with the exact division every pair index stays below
+    // 2^22 + 192, so P + lane cannot wrap (one compare on the decode's own sum)
+    auto active = [&](uint32_t P) { return EXACT ? P + lane < w1 : lane < (w1 > P ? w1 - P : 0u); };
     uint32_t oa, ob;
     decode(lookup(w0), w0, oa, ob);
     float4 a0 = load(oa), b0 = load(ob);
@@ -784,13 +786,13 @@ __device__ __forceinline__ void fast_walk(const float4 *__restrict__ src, const 
         const float4 a1 = load(oa), b1 = load(ob);
         __builtin_amdgcn_sched_barrier(0);
         A = lookup(P + 128u);
-        bin(a0, b0, lane < lanes(P));
+        bin(a0, b0, active(P));
         decode(A, P + 128u, oa, ob);
         a0 = load(oa);
         b0 = load(ob);
         __builtin_amdgcn_sched_barrier(0);
         A = lookup(P + 192u);
-        bin(a1, b1, lane < lanes(P + 64u));
+        bin(a1, b1, active(P + 64u));
         if (P + 128u >= w1) break;
     }
 }
