@@ -1026,8 +1026,8 @@ def eval_leg(args, W):
 
 def probe_leg(iac, dev, nvox):
     """Measured ceilings: a 2 GiB device copy (HBM), uint64 atomic adds uniform over
-    the 257^3 mesh, over a hot window of the occupied slab's size (7.3e5 voxels), and
-    into LDS boxes."""
+    the 257^3 mesh, over a hot window of the occupied slab's size (7.3e5 voxels), in runs
+    of 8 consecutive voxels (one 64-byte segment per 8 lanes), and into LDS boxes."""
     import torch
 
     ctx = iac.Context(0)
@@ -1040,12 +1040,12 @@ def probe_leg(iac, dev, nvox):
     mesh = torch.zeros(nvox, dtype=torch.int64, device=dev)
     n = 1 << 30
     rates = {}
-    for mode, name, hot in ((0, "uniform", 0), (1, "hot_slab", 730000), (2, "lds_box", 0)):
+    for mode, name, hot in ((0, "uniform", 0), (1, "hot_slab", 730000), (3, "runs_of_8", 0), (2, "lds_box", 0)):
         ctx.probe_atomics(mesh, n >> 4, mode, hot)
         t = min(ctx.probe_atomics(mesh, n, mode, hot) for _ in range(3))
         rates[name] = n / (t * 1e-3)
     log(f"[probe] copy {copy_gbs:.0f} GB/s; u64 atomics/s uniform {rates['uniform']:.3e}, "
-        f"hot slab {rates['hot_slab']:.3e}, LDS {rates['lds_box']:.3e}")
+        f"hot slab {rates['hot_slab']:.3e}, runs of 8 {rates['runs_of_8']:.3e}, LDS {rates['lds_box']:.3e}")
     ctx.close()
     return {"hbm_copy_GBps": copy_gbs, "atomic_u64_per_s": rates,
             "copy": "2 GiB device-to-device, 16-byte vector loads/stores, best of 5",

@@ -20,14 +20,16 @@ __device__ __forceinline__ uint32_t mix32(uint64_t x)
     return (uint32_t)x;
 }
 
-// mode 0: uniform over nvox; mode 1: a hot window of `hot` voxels at the middle
+// mode 0: uniform over nvox; mode 1: a hot window of `hot` voxels at the middle;
+// mode 3: runs of 8 consecutive voxels (one 64-byte segment) at uniform random
+// segments, lanes 8k..8k+7 of a wave on one run
 __global__ void __launch_bounds__(256)
     global_atomic_kernel(unsigned long long *__restrict__ mesh, uint64_t nvox, uint64_t n, int mode, uint64_t hot)
 {
     const uint64_t base = mode == 1 ? (nvox - hot) / 2 : 0;
     const uint64_t span = mode == 1 ? hot : nvox;
     for (uint64_t i = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; i < n; i += (uint64_t)gridDim.x * blockDim.x)
-        atomicAdd(mesh + base + mix32(i) % span, 1ull);
+        atomicAdd(mesh + (mode == 3 ? (mix32(i >> 3) % (nvox >> 3)) * 8 + (i & 7) : base + mix32(i) % span), 1ull);
 }
 
 // LDS box of 4096 uint64 per block, flushed once
@@ -92,14 +94,14 @@ int iac_probe_atomics(iac_ctx *ctx, uint64_t *d_mesh, uint64_t nvox, uint64_t n_
 {
     IAC_TRY(iac::bind(ctx));
     if (!d_mesh || !ms || nvox < 4096 || n_updates == 0) return iac::fail(IAC_ERR_INVALID_ARG, "bad probe");
-    if (mode < 0 || mode > 2) return iac::fail(IAC_ERR_INVALID_ARG, "probe mode is 0, 1 or 2");
+    if (mode < 0 || mode > 3) return iac::fail(IAC_ERR_INVALID_ARG, "probe mode is 0, 1, 2 or 3");
     if (mode == 1 && (hot == 0 || hot > nvox)) return iac::fail(IAC_ERR_INVALID_ARG, "bad hot window");
     hipEvent_t a, b;
     IAC_HIP(hipEventCreate(&a));
     IAC_HIP(hipEventCreate(&b));
     const unsigned blocks = (unsigned)std::max(1, ctx->num_cus * 16);
     IAC_HIP(hipEventRecord(a, ctx->stream));
-    if (mode < 2) {
+    if (mode != 2) {
         hipLaunchKernelGGL(global_atomic_kernel, dim3(blocks), dim3(256), 0, ctx->stream,
                            reinterpret_cast<unsigned long long *>(d_mesh), nvox, n_updates, mode, hot);
     } else {
