@@ -247,3 +247,32 @@ def test_lds_heavy_group_past_2_32(lds, dev):
                             np.array([0, n], np.uint64), shape))
     assert want[want.size // 2] > 0
     assert np.array_equal(got, want)
+
+
+@pytest.mark.parametrize("interp1", [0, 1])
+def test_lds_heavy_cell_rotated(lds, dev, interp1):
+    """Per-frame rotations on chunks that bin item by item: a 5,000-photon frame in one
+    64-voxel cell between smaller frames, each frame its own rotation (rotated at the box
+    flush and on the way out of the box).  Bit-exact against the oracle, nothing dropped."""
+    import incohautocorrelate_amd as iac
+
+    _, km, mq = W.detector(96)
+    shape = iac.mesh_shape_big(mq, 31)
+    rng = np.random.default_rng(23)
+    sizes = [200, 5000, 1, 600]
+    pix, off = [], [0]
+    for n in sizes:
+        pix.append(np.sort(rng.choice(96 * 96, n, replace=False)))
+        off.append(off[-1] + n)
+    pix = np.concatenate(pix).astype(np.uint32)
+    cnt = rng.integers(1, 3, pix.size).astype(np.uint16)
+    off = np.array(off, np.uint64)
+    rots = W.random_rotations(len(sizes), 29).astype(np.float32)
+    print(f"heavy cell rotated: interp1 {interp1}", flush=True)
+    ref, dref = O.ac3d_frames(km, pix, cnt, off, shape, rots=rots, interp1=interp1)
+    assert dref == 0
+    lds.set_option(iac.OPT_CELL_VOX, 64)
+    before = lds.dropped()
+    got = mesh_np(gpu_batch(lds, dev, km, pix, cnt, off, shape, rots=rots, interp1=interp1))
+    assert lds.dropped() == before
+    assert np.array_equal(got, ref)
