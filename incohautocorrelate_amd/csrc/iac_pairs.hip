@@ -1830,13 +1830,13 @@ int pairs_prepare(iac_ctx *ctx, const PairRun &R0, PairBatch **out)
     std::vector<uint64_t> off(R0.nframes + 1);
     IAC_HIP(hipMemcpyAsync(off.data(), R0.d_frame_off, off.size() * 8, hipMemcpyDeviceToHost, st));
     IAC_HIP(hipStreamSynchronize(st));
+    const std::vector<Chunk> chunks = make_chunks(off.data(), R0.nframes, R0.h_group_off, R0.ngroups, fchunk_of(ctx));
+    IAC_TRY(check_chunks(off.data(), chunks));
     auto *B = new PairBatch();
     B->ctx = ctx;
     B->R = R0;
     B->group_off.assign(R0.h_group_off, R0.h_group_off + R0.ngroups + 1);
     B->R.h_group_off = B->group_off.data();
-    const std::vector<Chunk> chunks = make_chunks(off.data(), R0.nframes, B->R.h_group_off, R0.ngroups, fchunk_of(ctx));
-    IAC_TRY(check_chunks(off.data(), chunks));
     const int nc = pl->g.ncu * pl->g.ncv;
     const Sizes z = sizes_of(B->R, nc, chunks.size());
     const size_t tbytes = std::max<size_t>(1, pl->tasks.size()) * sizeof(Task);
