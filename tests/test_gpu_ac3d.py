@@ -73,6 +73,41 @@ def test_cfg2_sparse_subset(ctx, dev):
     assert np.array_equal(got, ref)
 
 
+def test_cfg2_full_size_engine_tiles_halves(ctx, dev):
+    """The sparse configuration at its full size (10,000 frames of the 1024^2 detector,
+    Poisson(0.002), 257^3 mesh), where the oracle is too slow: the LDS engine's mesh equals
+    the global-atomic tile kernel's (an independent path, pair by pair), equals the sum of
+    two half-batch passes (frames are independent), is point symmetric about the centre,
+    and its centre holds at least 1000 sum c^2 (every photon's self pair).  Nothing is
+    dropped."""
+    import torch
+
+    import incohautocorrelate_amd as iac
+    from incohautocorrelate_amd import workloads as W
+
+    _, km, mq = W.detector(1024)
+    shape = iac.mesh_shape_big(mq, 255)
+    d = W.poisson_photon_frames(1024 * 1024, 10000, 0.002, seed=2)
+    pix, cnt, off = d["pix"], d["cnt"], d["frame_off"]
+    print(f"cfg2 full size: {int(off[-1])} photons in 10,000 frames", flush=True)
+    dropped0 = ctx.dropped()
+    full = mesh_np(gpu_batch(ctx, dev, km, pix, cnt, off, shape))
+    h = int(off[5000])
+    m = gpu_batch(ctx, dev, km, pix[:h], cnt[:h], off[:5001], shape)
+    m = gpu_batch(ctx, dev, km, pix[h:], cnt[h:], off[5000:] - np.uint64(h), shape, mesh=m)
+    assert np.array_equal(mesh_np(m), full)
+    ctx.set_option(iac.OPT_AC3D_KERNEL, 1)
+    try:
+        tiles = mesh_np(gpu_batch(ctx, dev, km, pix, cnt, off, shape))
+    finally:
+        ctx.set_option(iac.OPT_AC3D_KERNEL, 0)
+    assert np.array_equal(tiles, full)
+    assert ctx.dropped() == dropped0
+    assert np.array_equal(full, full[::-1])  # point symmetry: voxel v and nvox - 1 - v
+    c = cnt.astype(np.uint64)
+    assert full[full.size // 2] >= np.uint64(1000) * int((c * c).sum())
+
+
 def test_batch_equals_per_frame_uneven(ctx, dev):
     """Uneven frames (lognormal sizes, empty and one-photon frames) through the
     persistent batch equal per-frame calls and the oracle."""
