@@ -300,3 +300,32 @@ def test_cq_heavy_cell(ctx, dev, interp):
     finally:
         ctx.set_option(iac.OPT_CELL_VOX, 0)
     assert np.array_equal(mesh_np(m), ref)
+
+
+@pytest.mark.timeout(400)
+def test_cfg3_full_size_halves_and_symmetry(ctx, dev):
+    """The dense-float configuration at its full size, where the oracle is too slow:
+    2,000 frames of 256^2 over 360 orientations (launches sliced into whole chunks).
+    The 360 uint64 C(q) small meshes equal the sum of two passes over the frame halves
+    (frames are independent, uint64 sums exact), each is point symmetric, and so is the
+    merged lab mesh (the merge rotates integral voxel offsets, which negation commutes
+    with)."""
+    import torch
+
+    km, s, frames, orient, order, goff, rw = _cfg3_setup(dev, 2000, 360, seed=41)
+    nsmall = s.size_ab ** 2 * s.size_c
+    print("cfg3 full size: 2,000 frames, 360 orientations", flush=True)
+    got_small, _, got_big = _cfg3_gpu(ctx, dev, km, s, frames, order, goff, rw, 1e10, 1e6)
+    # halves: frames f < 1000 and f >= 1000, each grouped by orientation
+    d_km = to_dev(km, dev)
+    acc = torch.zeros(360 * nsmall, dtype=torch.int64, device=dev)
+    for half in (np.arange(1000), np.arange(1000, 2000)):
+        o = orient[half]
+        hord = half[np.argsort(o, kind="stable")]
+        hgoff = np.searchsorted(np.sort(o, kind="stable"), np.arange(361)).astype(np.uint32)
+        ctx.cq_dense_frames(d_km, to_dev(frames[hord], dev), hgoff, s, acc, 0, 1e10)
+    assert np.array_equal(mesh_np(acc), got_small)
+    sm = got_small.reshape(360, nsmall)
+    assert np.array_equal(sm, sm[:, ::-1])
+    assert np.array_equal(got_big, got_big[::-1])
+    assert got_big.sum() > 0
