@@ -211,8 +211,8 @@ def main():
     ev = None if args.no_eval else eval_leg(args, W)
     probes = None if args.no_probes else probe_leg(iac, dev, nvox)
     if probes and ac1d:
-        # AC1D bins in LDS: each unordered pair (twice-weighted) is two linear-split LDS adds
-        lds = probes["atomic_u64_per_s"]["lds_box"]
+        # AC1D bins in LDS: each unordered pair (twice-weighted) is two linear-split LDS double adds
+        lds = probes["atomic_u64_per_s"]["lds_box_f64"]
         for leg in (ac1d, ac1d["cq"]):
             upairs = leg["value"] / 2 if leg is ac1d else leg["value"]
             leg["lds_adds_per_s"] = 2 * upairs
@@ -1040,12 +1040,14 @@ def probe_leg(iac, dev, nvox):
     mesh = torch.zeros(nvox, dtype=torch.int64, device=dev)
     n = 1 << 30
     rates = {}
-    for mode, name, hot in ((0, "uniform", 0), (1, "hot_slab", 730000), (3, "runs_of_8", 0), (2, "lds_box", 0)):
+    for mode, name, hot in ((0, "uniform", 0), (1, "hot_slab", 730000), (3, "runs_of_8", 0), (2, "lds_box", 0),
+                            (4, "lds_box_f64", 0)):
         ctx.probe_atomics(mesh, n >> 4, mode, hot)
         t = min(ctx.probe_atomics(mesh, n, mode, hot) for _ in range(3))
         rates[name] = n / (t * 1e-3)
     log(f"[probe] copy {copy_gbs:.0f} GB/s; u64 atomics/s uniform {rates['uniform']:.3e}, "
-        f"hot slab {rates['hot_slab']:.3e}, runs of 8 {rates['runs_of_8']:.3e}, LDS {rates['lds_box']:.3e}")
+        f"hot slab {rates['hot_slab']:.3e}, runs of 8 {rates['runs_of_8']:.3e}, LDS {rates['lds_box']:.3e}, "
+        f"LDS double {rates['lds_box_f64']:.3e}")
     ctx.close()
     return {"hbm_copy_GBps": copy_gbs, "atomic_u64_per_s": rates,
             "copy": "2 GiB device-to-device, 16-byte vector loads/stores, best of 5",

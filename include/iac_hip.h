@@ -410,8 +410,8 @@ int iac_sim_postprocess(iac_ctx *ctx, float *d_frames, uint32_t npix, uint32_t n
 /* n_updates uint64 atomic adds of 1 into d_mesh (nvox >= 4096): mode 0 uniform over the
  * mesh, mode 1 over a window of `hot` voxels in its middle, mode 2 into a 4096-voxel LDS
  * box per block flushed once, mode 3 in runs of 8 consecutive voxels (one 64-byte segment
- * per 8 lanes) at uniform random segments.  *ms = kernel time (HIP events on the context's
- * stream). */
+ * per 8 lanes) at uniform random segments, mode 4 as mode 2 with double adds of 1.0 in the
+ * box (flushed as integers).  *ms = kernel time (HIP events on the context's stream). */
 int iac_probe_atomics(iac_ctx *ctx, uint64_t *d_mesh, uint64_t nvox, uint64_t n_updates, int mode, uint64_t hot,
                       float *ms);
 /* A plain 16-byte-vector copy of `bytes` (a multiple of 16); *ms = kernel time. */

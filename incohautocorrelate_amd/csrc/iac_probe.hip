@@ -32,19 +32,21 @@ __global__ void __launch_bounds__(256)
         atomicAdd(mesh + (mode == 3 ? (mix32(i >> 3) % (nvox >> 3)) * 8 + (i & 7) : base + mix32(i) % span), 1ull);
 }
 
-// LDS box of 4096 uint64 per block, flushed once
+// LDS box of 4096 uint64 (T = unsigned long long) or double (the AC1D histograms' adds)
+// per block, flushed once
+template <typename T>
 __global__ void __launch_bounds__(256)
     lds_atomic_kernel(unsigned long long *__restrict__ mesh, uint64_t nvox, uint64_t n_per_block)
 {
-    __shared__ unsigned long long box[4096];
-    for (int v = threadIdx.x; v < 4096; v += 256) box[v] = 0ull;
+    __shared__ T box[4096];
+    for (int v = threadIdx.x; v < 4096; v += 256) box[v] = T(0);
     __syncthreads();
     const uint64_t i0 = (uint64_t)blockIdx.x * n_per_block;
-    for (uint64_t i = threadIdx.x; i < n_per_block; i += 256) atomicAdd(&box[mix32(i0 + i) & 4095u], 1ull);
+    for (uint64_t i = threadIdx.x; i < n_per_block; i += 256) atomicAdd(&box[mix32(i0 + i) & 4095u], T(1));
     __syncthreads();
     const uint64_t off = ((uint64_t)blockIdx.x * 4096) % (nvox - 4096 + 1);
     for (int v = threadIdx.x; v < 4096; v += 256)
-        if (box[v]) atomicAdd(mesh + off + v, box[v]);
+        if (box[v] != T(0)) atomicAdd(mesh + off + v, (unsigned long long)box[v]);
 }
 
 __global__ void __launch_bounds__(256) copy_kernel(const uint4 *__restrict__ src, uint4 *__restrict__ dst, size_t n)
@@ -94,20 +96,24 @@ int iac_probe_atomics(iac_ctx *ctx, uint64_t *d_mesh, uint64_t nvox, uint64_t n_
 {
     IAC_TRY(iac::bind(ctx));
     if (!d_mesh || !ms || nvox < 4096 || n_updates == 0) return iac::fail(IAC_ERR_INVALID_ARG, "bad probe");
-    if (mode < 0 || mode > 3) return iac::fail(IAC_ERR_INVALID_ARG, "probe mode is 0, 1, 2 or 3");
+    if (mode < 0 || mode > 4) return iac::fail(IAC_ERR_INVALID_ARG, "probe mode is 0 .. 4");
     if (mode == 1 && (hot == 0 || hot > nvox)) return iac::fail(IAC_ERR_INVALID_ARG, "bad hot window");
     hipEvent_t a, b;
     IAC_HIP(hipEventCreate(&a));
     IAC_HIP(hipEventCreate(&b));
     const unsigned blocks = (unsigned)std::max(1, ctx->num_cus * 16);
     IAC_HIP(hipEventRecord(a, ctx->stream));
-    if (mode != 2) {
+    if (mode == 0 || mode == 1 || mode == 3) {
         hipLaunchKernelGGL(global_atomic_kernel, dim3(blocks), dim3(256), 0, ctx->stream,
                            reinterpret_cast<unsigned long long *>(d_mesh), nvox, n_updates, mode, hot);
     } else {
         const uint64_t per = std::max<uint64_t>(1, n_updates / blocks);
-        hipLaunchKernelGGL(lds_atomic_kernel, dim3(blocks), dim3(256), 0, ctx->stream,
-                           reinterpret_cast<unsigned long long *>(d_mesh), nvox, per);
+        if (mode == 2)
+            hipLaunchKernelGGL(lds_atomic_kernel<unsigned long long>, dim3(blocks), dim3(256), 0, ctx->stream,
+                               reinterpret_cast<unsigned long long *>(d_mesh), nvox, per);
+        else
+            hipLaunchKernelGGL(lds_atomic_kernel<double>, dim3(blocks), dim3(256), 0, ctx->stream,
+                               reinterpret_cast<unsigned long long *>(d_mesh), nvox, per);
     }
     IAC_HIP(hipGetLastError());
     IAC_HIP(hipEventRecord(b, ctx->stream));

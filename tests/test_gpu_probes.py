@@ -22,11 +22,12 @@ def test_global_atomic_probe_counts_every_add(ctx, dev, mode, hot):
         assert int(m[nvox // 8 * 8:].abs().sum()) == 0
 
 
-def test_lds_atomic_probe_flushes_its_boxes(ctx, dev):
+@pytest.mark.parametrize("mode", [2, 4])
+def test_lds_atomic_probe_flushes_its_boxes(ctx, dev, mode):
     import torch
 
     nvox = 65 ** 3
     m = torch.zeros(nvox, dtype=torch.int64, device=dev)
-    ctx.probe_atomics(m, 1 << 20, 2, 0)
+    ctx.probe_atomics(m, 1 << 20, mode, 0)
     total = int(m.sum())
     assert 0 < total <= 1 << 20
