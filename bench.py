@@ -273,8 +273,11 @@ def main():
                                  "read-modify-write (see traffic)"},
             "frac_of_measured_copy": achieved / probes["hbm_copy_GBps"] if probes else None,
             "updates_vs_uniform_u64_atomics": value / probes["atomic_u64_per_s"]["uniform"] if probes else None,
-            # the kernel's own update rate against random uint64 adds into a 4096-voxel LDS box
-            "frac_of_lds_atomic_probe": (updates / (k_avg_ms * 1e-3) / probes["atomic_u64_per_s"]["lds_box"]
+            # the kernel's LDS adds (one per distinct unordered pair: (updates - photons) / 2, the self
+            # pairs go to the centre separately) against random uint64 adds into a 4096-voxel LDS box
+            "lds_adds_per_s": (updates - n_total) / 2 / (k_avg_ms * 1e-3) if k_avg_ms > 0 else None,
+            "frac_of_lds_atomic_probe": ((updates - n_total) / 2 / (k_avg_ms * 1e-3)
+                                         / probes["atomic_u64_per_s"]["lds_box"]
                                          if probes and k_avg_ms > 0 else None),
         },
         "parity": parity,
