@@ -8,8 +8,9 @@
 //      widest k axes (cell side `cell_vox` voxels); per-cell k extents bound
 //      the voxel box reached by the pairs of every cell offset delta.
 //   2. cell_sort: per frame, counting sort of its records by cell, dropping
-//      zero-valued records (they add nothing in any mode); in symmetric mode
-//      the frame's self-pair value is summed for its group.
+//      zero-valued records (they add nothing in any mode), and the record count
+//      of its fullest cell; in symmetric mode the frame's self-pair value is
+//      summed for its group.
 //   3. pairs: one block per (delta [, region of cells], chunk of frames of one
 //      group).  The block privatises delta's voxel box in LDS across the chunk
 //      and bins
@@ -20,9 +21,11 @@
 //      the box adds to global memory directly, except on in-box tasks of the
 //      AC_uw specialised path, whose exact box provably holds every pair.
 //      Work inside a block goes by groups of (frame, cell) items: the items'
-//      pair counts are scanned, then symmetric nearest-neighbour runs walk the
-//      group's pairs with the lanes of a wave on consecutive pairs (coalesced
-//      record loads), other runs give each thread a contiguous pair range.
+//      pair counts are scanned (32-bit), then AC_uw runs walk the group's pairs
+//      with the lanes of a wave on consecutive pairs (coalesced record loads)
+//      and C(q) runs give each thread a contiguous pair range (either on
+//      request).  A chunk with a frame whose fullest cell could push a group
+//      past 2^32 pairs bins its items one after another instead.
 //   4. symmetric modes: mesh[v] += H[v] + H[mirror v], centre += self.
 // Nearest-neighbour binning is odd in q (iac_binning.h): the (b,a) pair lands
 // on the point reflection of (a,b) with the same value, so step 4 restores
