@@ -1210,12 +1210,21 @@ __global__ void __launch_bounds__(kBlock, (Occupancy<FAST, BOX>::blocks))
     // voxel shares its first-level voxel, so the rotated second level bins them alike)
     unsigned long long *out = target + (size_t)CH.group * nvox;
     float rr[9];
-    if (crot)
+    int ax = 0;  // the box axis consecutive lanes walk
+    if (crot) {
         for (int i = 0; i < 9; i++) rr[i] = crot[i];
+        // the axis whose rotated image moves most along the mesh's fast axis, so that
+        // neighbouring lanes add to neighbouring voxels (one L2 request per 64-byte segment
+        // a wave instruction touches)
+        const float a0 = fabsf(rr[0]), a1 = fabsf(rr[1]), a2 = fabsf(rr[2]);
+        ax = a1 > a0 ? (a2 > a1 ? 2 : 1) : (a2 > a0 ? 2 : 0);
+    }
+    const int e0 = ax == 0 ? T.bx : (ax == 1 ? T.by : T.bz), e1 = ax == 0 ? T.by : T.bx;
     for (int v = threadIdx.x; v < vbox; v += kBlock) {
-        const unsigned long long x = box[v];
+        const int i0 = v % e0, rest = v / e0, i1 = rest % e1, i2 = rest / e1;
+        const int lx = ax == 0 ? i0 : i1, ly = ax == 1 ? i0 : (ax == 0 ? i1 : i2), lz = ax == 2 ? i0 : i2;
+        const unsigned long long x = box[lx + T.bx * (ly + T.by * lz)];
         if (x) {
-            const int lx = v % T.bx, rest = v / T.bx, ly = rest % T.by, lz = rest / T.by;
             int fs = T.bx0 + lx, ms = T.by0 + ly, ss = T.bz0 + lz;
             if (crot) {
                 rotate_voxel(rr, S.P.c0, S.P.c1, S.P.c2, fs, ms, ss);
