@@ -211,12 +211,13 @@ def main():
     ev = None if args.no_eval else eval_leg(args, W)
     probes = None if args.no_probes else probe_leg(iac, dev, nvox)
     if probes and ac1d:
-        # AC1D bins in LDS: each unordered pair (twice-weighted) is two linear-split LDS double adds
-        lds = probes["atomic_u64_per_s"]["lds_box_f64"]
-        for leg in (ac1d, ac1d["cq"]):
+        # AC1D bins in LDS: each unordered pair (twice-weighted) is two linear-split LDS adds, doubles
+        # for AC_uw (float values), uint64 for C(q) (its fixed point)
+        for leg, kind in ((ac1d, "lds_box_f64"), (ac1d["cq"], "lds_box")):
             upairs = leg["value"] / 2 if leg is ac1d else leg["value"]
             leg["lds_adds_per_s"] = 2 * upairs
-            leg["frac_of_lds_atomic_probe"] = 2 * upairs / lds
+            leg["lds_probe"] = kind
+            leg["frac_of_lds_atomic_probe"] = 2 * upairs / probes["atomic_u64_per_s"][kind]
     ppp = None if args.no_ppp else ppp_leg(args, iac, dev, probes)
     uneven = None if args.no_uneven else uneven_leg(args, iac, W, dev)
     sim = None if args.no_sim else sim_leg(args, iac, W, dev)
