@@ -424,10 +424,52 @@ def queue_leg(args, iac, dev, kmap, d_kmap, data, shape, d_ref, updates, batch_m
         f"{batch_ms / ms:.2f}x the batched pass), mesh equals the batched pass: {equal}")
     if not equal:
         raise SystemExit("per-frame queue mesh differs from the batched pass")
+    mirror = queue_mirror(data, d_ref, updates, batch_ms)
     return {"workload": f"the headline {nf} frames, one iac_ac3d_queue_push per frame from host hit lists "
                         "(Detector::AutoCorrelateSparseList per event), flush every 2^22 records",
             "value": rate, "unit": "pair-updates/s", "ms_per_pass": ms, "frac_of_batched_rate": batch_ms / ms,
-            "mesh_equals_batched_pass": equal}
+            "mesh_equals_batched_pass": equal, "cpp_mirror": mirror}
+
+
+def queue_mirror(data, d_ref, updates, batch_ms):
+    """The same per-event loop through the C++ host mirror: `iac_eval --queue` calls
+    Detector::AutoCorrelateSparseList once per frame (a device reservation, the mesh's queue,
+    a push into pinned staging) on hit lists it builds from the photon lists, then flushes;
+    its mesh must equal the batched pass's."""
+    import subprocess
+    import tempfile
+
+    from incohautocorrelate_amd import workloads as W
+
+    exe = ROOT / "incohautocorrelate_amd" / "iac_eval"
+    tmp = Path(tempfile.mkdtemp(prefix="iac_queue_bench_"))
+    try:
+        pm, _, _ = W.detector(1024)
+        pm.astype(np.float32).tofile(tmp / "pm.bin")
+        data["pix"].astype(np.uint32).tofile(tmp / "pix.bin")
+        data["cnt"].astype(np.uint16).tofile(tmp / "cnt.bin")
+        data["frame_off"].astype(np.uint64).tofile(tmp / "off.bin")
+        nf = len(data["frame_off"]) - 1
+        cmd = [str(exe), "--queue", str(tmp / "pm.bin"), "1024", "1024", str(tmp / "pix.bin"), str(tmp / "cnt.bin"),
+               str(tmp / "off.bin"), str(nf), "255", str(tmp / "mesh.bin")]
+        r = subprocess.run(cmd, capture_output=True, text=True, timeout=600)
+        if r.returncode != 0:
+            raise RuntimeError("iac_eval --queue failed: " + r.stderr[-2000:])
+        ms = float(r.stdout.split("queue ms ")[1].split()[0])
+        got = np.fromfile(tmp / "mesh.bin", dtype=np.uint64)
+        equal = bool(np.array_equal(got, d_ref.cpu().numpy().view(np.uint64)))
+    finally:
+        for f in tmp.iterdir():
+            f.unlink()
+        tmp.rmdir()
+    rate = updates / (ms * 1e-3)
+    log(f"[queue] C++ mirror, Detector::AutoCorrelateSparseList per frame: {ms:.2f} ms ({rate:.4e} pair-updates/s, "
+        f"{batch_ms / ms:.2f}x the batched pass), mesh equals the batched pass: {equal}")
+    if not equal:
+        raise SystemExit("C++ mirror per-event mesh differs from the batched pass")
+    return {"value": rate, "unit": "pair-updates/s", "ms_per_pass": ms, "frac_of_batched_rate": batch_ms / ms,
+            "mesh_equals_batched_pass": equal,
+            "timed": "the calls and the last flush to the device's completion (the second pass of the process)"}
 
 
 def cpu_baseline(kmap, data, shape, ctx, iac, d_kmap, dev, args):

@@ -5,6 +5,9 @@
 // PM: float32 [SS][FS][3]; STACK: float32 [N][SS][FS]; --mask: int32 [SS][FS];
 // --rotations: float32 [N][9] row major (default identity).  Outputs are raw
 // float64 arrays.  Prints the IAC_Report as one JSON line.
+#include <array>
+#include <chrono>
+#include <cstdint>
 #include <cstdio>
 #include <cstdlib>
 #include <cstring>
@@ -27,6 +30,7 @@ static void usage()
                  "       iac_eval -e Settings.xml      (evaluation settings file, hit-event list from it)\n"
                  "       iac_eval -eec Example.xml     (write an example evaluation settings file)\n"
                  "       iac_eval --lap STACK N NPIX PANELS ADU SEED COMB OUT\n"
+                 "       iac_eval --queue PIXELMAP SS FS PIX CNT OFF NFRAMES EDGE OUT\n"
                  "  (PPP::PhotonFinder_LargestAdjacentPixel frame by frame; PANELS = first:fs:ss,...)\n";
 }
 
@@ -66,9 +70,92 @@ static int lap_main(int argc, char **argv)
     return 0;
 }
 
+// Detector::AutoCorrelateSparseList once per event, as RunIAC's frame loop calls it
+// (IAC/RunIAC.cpp:524-550), on hit lists {kx, ky, kz, count} built on the host from photon
+// lists (PIX uint32 and CNT uint16 per photon, OFF uint64 frame offsets); NN/NN double map
+// into the big mesh of edge EDGE.  After one untimed pass, times the calls and the queue's
+// flush to the device's completion, then downloads the mesh to OUT (uint64).
+static int queue_main(int argc, char **argv)
+{
+    if (argc != 11) {
+        usage();
+        return 2;
+    }
+    const unsigned ss = (unsigned)std::strtoul(argv[3], nullptr, 10), fs = (unsigned)std::strtoul(argv[4], nullptr, 10);
+    const unsigned long nframes = std::strtoul(argv[8], nullptr, 10);
+    const int edge = (int)std::strtol(argv[9], nullptr, 10);
+    if (ss == 0 || fs == 0 || nframes == 0 || edge <= 0) {
+        usage();
+        return 2;
+    }
+    Settings S;
+    S.echo = false;
+    Detector D(S);
+    D.LoadPixelMap(argv[2], ss, fs);
+    D.Calc_kMap();
+    ACMesh M(&S);
+    M.CreateBigMeshForDetector(D, edge);
+    std::vector<uint64_t> off(nframes + 1);
+    iac_host::read_raw(argv[7], off.data(), off.size() * sizeof(uint64_t));
+    const uint64_t n = off[nframes];
+    std::vector<uint32_t> pix(n);
+    std::vector<uint16_t> cnt(n);
+    iac_host::read_raw(argv[5], pix.data(), n * sizeof(uint32_t));
+    iac_host::read_raw(argv[6], cnt.data(), n * sizeof(uint16_t));
+    const size_t npix = D.NumPixels();
+    std::vector<std::vector<std::array<float, 4>>> lists(nframes);
+    for (unsigned long f = 0; f < nframes; f++) {
+        for (uint64_t i = off[f]; i < off[f + 1]; i++) {
+            if (pix[i] >= npix) throw std::runtime_error("--queue: pixel index outside the detector");
+            const float *k = D.kMap + 3 * (size_t)pix[i];
+            lists[f].push_back({k[0], k[1], k[2], (float)cnt[i]});
+        }
+    }
+    Detector::AutoCorrFlags nn;
+    // the queue and the device k-map exist before the timed calls (the first event of a run
+    // makes them); the device is given back before the calls reserve it one by one
+    iac_ctx *ctx = nullptr;
+    {
+        iac_host::Reserved res(S, "iac_eval --queue");
+        ctx = res.ctx;
+        M.ctx = ctx;
+        M.DeviceMesh();
+        M.Queue(D.DeviceKMap(ctx), (uint32_t)npix, nn.InterpolationMode, nn.InterpolationMode, true);
+        iac_host::check(iac_ctx_sync(ctx), "iac_ctx_sync");
+    }
+    auto pass = [&]() {
+        const auto t0 = std::chrono::steady_clock::now();
+        for (unsigned long f = 0; f < nframes; f++) {
+            D.SparseHitList.swap(lists[f]);  // the event's list (swapped back below)
+            D.Checklist.SparseHitList = true;
+            D.AutoCorrelateSparseList(M, nn, nn, true, S, 1);
+            D.SparseHitList.swap(lists[f]);
+        }
+        M.Flush();
+        iac_host::check(iac_ctx_sync(ctx), "iac_ctx_sync");
+        return std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
+    };
+    // one untimed pass (the process's first launches load their kernels), a cleared mesh, the timed pass
+    pass();
+    iac_host::check(iac_memset_zero(ctx, M.d_Mesh, M.NumVoxels() * sizeof(uint64_t)), "iac_memset_zero");
+    const double ms = pass();
+    M.Download();
+    iac_host::write_raw(argv[10], M.Mesh, M.NumVoxels() * sizeof(uint64_t));
+    std::cout << "queue ms " << ms << " frames " << nframes << std::endl;
+    return 0;
+}
+
 int main(int argc, char **argv)
 {
     if (argc >= 2 && std::string(argv[1]) == "--lap") return lap_main(argc, argv);
+    if (argc >= 2 && std::string(argv[1]) == "--queue") {
+        try {
+            return queue_main(argc, argv);
+        } catch (const std::exception &ex) {
+            std::cerr << "ERROR: " << ex.what() << "\n";
+            return 1;
+        }
+    }
     if (argc >= 2) {
         // the reference's run modes over settings files (IAC/main.cpp:560-600)
         const std::string a1 = argv[1];

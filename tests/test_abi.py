@@ -82,3 +82,9 @@ def test_host_library_and_eval_driver():
                  ["--lap", "x", "1", "10", "0:4:4", "1", "0.5", "0.9", "y"]):
         r = subprocess.run([str(pkg / "iac_eval"), *args], capture_output=True, text=True, timeout=60)
         assert r.returncode == 2 and "--lap" in r.stderr, args
+    # so does the per-event queue mode: wrong count, a zero size, no frames, no mesh edge
+    for args in (["--queue"], ["--queue", "pm", "0", "4", "p", "c", "o", "1", "31", "m"],
+                 ["--queue", "pm", "4", "4", "p", "c", "o", "0", "31", "m"],
+                 ["--queue", "pm", "4", "4", "p", "c", "o", "1", "0", "m"]):
+        r = subprocess.run([str(pkg / "iac_eval"), *args], capture_output=True, text=True, timeout=60)
+        assert r.returncode == 2 and "--queue" in r.stderr, args
