@@ -51,9 +51,18 @@ def _run(cmd: list[str]) -> None:
             sys.stderr.write(line + "\n")
 
 
+# per-file scheduling: the pair engine's walk runs 2-3 % faster with the instruction-level-
+# parallelism strategy of the AMDGPU machine scheduler (every mesh bit-identical); the other
+# files keep the default strategy
+FILE_FLAGS = {
+    "iac_pairs": ["-mllvm", "-amdgpu-sched-strategy=max-ilp"],
+}
+
+
 def _compile(src: Path) -> Path:
     obj = BUILD / (src.stem + ".o")
-    _run([HIPCC, *HIP_FLAGS, "-I", str(PKG.parent / "include"), "-c", str(src), "-o", str(obj)])
+    _run([HIPCC, *HIP_FLAGS, *FILE_FLAGS.get(src.stem, []), "-I", str(PKG.parent / "include"), "-c", str(src),
+          "-o", str(obj)])
     return obj
 
 
