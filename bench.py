@@ -43,7 +43,7 @@ BYTES_PER_UPDATE = 16  # one 8 B bin read + one 8 B bin write per pair-update
 # Measured HBM bytes per pair-kernel launch on this workload, keyed by the engine settings
 # (pair mode, box, cell_vox, frame_chunk): FETCH_SIZE x2.0 + WRITE_SIZE x1.0 KiB from two
 # rocprofv3 --pmc passes, scaled by a 2 GiB copy kernel (profiles/pairs_lds_sparse1024_10k.md).
-MEASURED_TRAFFIC = {(1, 1920, 6, 32): (2 * 1515266 + 3118364) * 1024}
+MEASURED_TRAFFIC = {(1, 1920, 6, 32): (2 * 1511764 + 3118483) * 1024}
 # The same for sim_gemm_kernel per launch of 1,000 frames (profiles/sim_gemm_mfma_512_10k_1000.md)
 SIM_TRAFFIC = {"bf16x3": (2 * 70271911 + 1025181) * 1024}
 
